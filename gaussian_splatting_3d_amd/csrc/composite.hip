@@ -1,0 +1,928 @@
+// Per-tile alpha compositing on gfx950: forward and reverse-order adjoint.
+//
+// One 256-thread workgroup (4 wave64) per 16x16 tile; wave w owns pixel rows
+// 4w..4w+3.  Gaussians of the tile's sorted list are staged in LDS in
+// batches; every wave walks the batch in lockstep (wave-uniform loop) and
+// leaves it as soon as a ballot shows all 64 of its pixels saturated; the
+// workgroup stops loading batches once no pixel is alive.
+//
+// Decision exactness.  The skip test a*G < 1/255 and the transmittance
+// break use the reference's operation sequence (kernels.h:172-193,
+// vol_render_sh.h:120-168) with the specified exponential, so they agree bit
+// for bit with the oracle.  A per-Gaussian bound rlim lets most far-away
+// pixels be rejected from q * (1/det) alone: for r > rlim the product a*G is
+// below 1/255 by a relative margin of 5e-4, far above the few-ulp gap between
+// q*(1/det) and q/det.  Everything else takes the exact path.
+#include "gsmi_device.h"
+
+namespace gsmi {
+
+constexpr int kBatch = 128;  // Gaussians staged per LDS batch
+
+struct RasterArgs {
+  const float* mean;  // [V,2]
+  const float* cov;   // [V,4]
+  const float* sh;    // SH rows (or RGB colour [V,3] for the RGB kernels)
+  const int32_t* sh_index;
+  const float* alpha;  // [V]
+  const int32_t* start;
+  const int32_t* end;
+  const int32_t* ids;
+  const float* topleft;
+  const float* c2w;
+  const float* bg;
+  float* out;
+  float* T_final;
+  int32_t* n_done;
+  float* pre_bg;
+  // backward only
+  const float* out_saved;
+  const float* T_saved;
+  const int32_t* n_done_saved;
+  const float* pre_bg_saved;
+  const float* grad_out;
+  float* grad_mean;
+  float* grad_cov;
+  float* grad_sh;
+  float* grad_alpha;
+  int64_t sh_sg, sh_sc;
+  float psx, psy, thresh;
+  int n_tiles_h, n_tiles_w, H, W;
+  int write_empty;
+};
+
+// Per-Gaussian LDS record: {mx, my, c0, c1}, {c2, c3, det, 1/det}, {a, rlim}.
+struct alignas(16) GeoA {
+  float mx, my, c0, c1;
+};
+struct alignas(16) GeoB {
+  float c2, c3, det, inv_det;
+};
+
+// Rejection radius: for q/det > rlim the Gaussian is certainly skipped.
+// a*exp(-r/2) < 1/255  <=>  r > 2 ln(255 a); inflated by 1e-3 (relative and
+// absolute) so that approximate log and the q*(1/det) shortcut stay safe.
+__device__ __forceinline__ float reject_radius(float a) {
+  const float t = a * 255.0f;
+  if (!(t > 0.0f)) return -__builtin_huge_valf();
+  const float r = 2.0f * __logf(t);
+  return r + 1e-3f * (1.0f + fabsf(r));
+}
+
+// Exact Gaussian value from the shared quadratic form (reference order).
+__device__ __forceinline__ float gauss_exact(float q, float det) {
+  float radial = q / det;
+  if (radial < 0.0f) radial = 1000.0f;
+  return exp_spec(-0.5f * radial);
+}
+
+// The reference's kernel_gaussian_2d (double) from float inputs.
+__device__ __forceinline__ float gauss_exact_d(float dx, float dy, float c0, float c1, float c2, float c3) {
+  const double C0 = c0, C1 = c1, C2 = c2, C3 = c3;
+  const double det = C0 * C3 - C1 * C2;
+  const double x = dx, y = dy;
+  const double tmpx = x * C3 - y * C2;
+  const double tmpy = -x * C1 + y * C0;
+  double radial = tmpx * x + tmpy * y;
+  radial /= det;
+  if (radial < 0.0) radial = 1000.0;
+  return (float)exp_spec_d(-0.5 * radial);
+}
+
+template <int N>
+__device__ __forceinline__ float dot_exact(const float* a, const float* b) {
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) s = s + a[i] * b[i];
+  return s;
+}
+
+template <int N>
+__device__ __forceinline__ float dot_fast(const float* a, const float* b) {
+  float s = a[0] * b[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) s = fmaf(a[i], b[i], s);
+  return s;
+}
+
+// Stage entries [s0+base, s0+base+cnt) of the tile list into LDS.
+template <int SHN, int CC>
+__device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
+                                            GeoB* s_gb, float2* s_gc, float* s_sh) {
+  const int t = threadIdx.x;
+  if (t < cnt) {
+    const int g = a.ids[s0 + base + t];
+    s_id[t] = g;
+    const float2 m = reinterpret_cast<const float2*>(a.mean)[g];
+    const float4 c = reinterpret_cast<const float4*>(a.cov)[g];
+    const float det = c.x * c.w - c.y * c.z;
+    const float al = fminf(a.alpha[g], 0.99f);
+    s_ga[t] = GeoA{m.x, m.y, c.x, c.y};
+    s_gb[t] = GeoB{c.z, c.w, det, 1.0f / det};
+    s_gc[t] = make_float2(al, reject_radius(al));
+  }
+  __syncthreads();
+  for (int e = t; e < cnt * SHN; e += kTilePixels) {
+    const int j = e / SHN;
+    const int r = e - j * SHN;
+    const int ch = r / CC;
+    const int k = r - ch * CC;
+    const int g = s_id[j];
+    const int64_t row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
+    s_sh[e] = a.sh[row * a.sh_sg + ch * a.sh_sc + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward, SH colour.  vol_render_sh.h:171-248 (+ vol_render_bg.h:12-100).
+// ---------------------------------------------------------------------------
+template <int C, bool BG, bool EXACT>
+__global__ __launch_bounds__(256) void k_fwd_sh(RasterArgs a) {
+  constexpr int CC = C * C;
+  constexpr int SHN = 3 * CC;
+  __shared__ GeoA s_ga[kBatch];
+  __shared__ GeoB s_gb[kBatch];
+  __shared__ float2 s_gc[kBatch];
+  __shared__ int s_id[kBatch];
+  __shared__ float s_sh[kBatch * SHN];
+
+  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
+  const int tile = xcd_tile(blockIdx.x, n_tiles);
+  if (tile >= n_tiles) return;
+  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
+  const int t = threadIdx.x;
+  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
+  const bool inside = gx < a.W && gy < a.H;
+  const int64_t pix = (int64_t)gy * a.W + gx;
+  const int s0 = a.start[tile];
+  const int e0 = a.end[tile];
+  if (s0 == -1 || e0 - s0 == 0) {
+    if (inside) {
+      if (BG && s0 == -1) {
+        a.out[3 * pix + 0] = a.bg[0];
+        a.out[3 * pix + 1] = a.bg[1];
+        a.out[3 * pix + 2] = a.bg[2];
+      } else if (a.write_empty) {
+        a.out[3 * pix + 0] = 0.0f;
+        a.out[3 * pix + 1] = 0.0f;
+        a.out[3 * pix + 2] = 0.0f;
+      }
+      if (a.T_final) a.T_final[pix] = 1.0f;
+      if (a.n_done) a.n_done[pix] = 0;
+      if (BG && a.pre_bg) a.pre_bg[3 * pix] = a.pre_bg[3 * pix + 1] = a.pre_bg[3 * pix + 2] = 0.0f;
+    }
+    return;
+  }
+  const float px = a.topleft[0] + (float)gx * a.psx;
+  const float py = a.topleft[1] + (float)gy * a.psy;
+  float cw[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) cw[k] = a.c2w[k];
+  float dir[3];
+  pixel_direction(cw, px, py, dir);
+  float Y[CC];
+  sh_basis<C>(dir[0], dir[1], dir[2], Y);
+
+  const float thresh = a.thresh;
+  float T = 1.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+  bool done = !inside;
+  const int n = e0 - s0;
+  int ndone = n;
+  if (inside && T < thresh) {
+    done = true;
+    ndone = 0;
+  }
+  for (int base = 0; base < n; base += kBatch) {
+    const int cnt = min(kBatch, n - base);
+    __syncthreads();
+    stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      if (__all(done)) break;
+      if (done) continue;
+      const GeoA ga = s_ga[j];
+      const GeoB gb = s_gb[j];
+      const float2 gc = s_gc[j];
+      const float dx = px - ga.mx, dy = py - ga.my;
+      const float tmpx = dx * gb.c3 - dy * gb.c2;
+      const float tmpy = -dx * ga.c1 + dy * ga.c0;
+      const float q = tmpx * dx + tmpy * dy;
+      if (q * gb.inv_det > gc.y) continue;
+      const float G = gauss_exact(q, gb.det);
+      const float al = gc.x;
+      float coeff = al * T;
+      coeff *= G;
+      if (al * G < kMinRenderAlpha) continue;
+      if (isnan(coeff)) coeff = 0.0f;
+      const float* shj = s_sh + j * SHN;
+      float y0, y1, y2;
+      if constexpr (EXACT) {
+        y0 = sigmoid_exact(dot_exact<CC>(shj, Y));
+        y1 = sigmoid_exact(dot_exact<CC>(shj + CC, Y));
+        y2 = sigmoid_exact(dot_exact<CC>(shj + 2 * CC, Y));
+      } else {
+        y0 = sigmoid_fast(dot_fast<CC>(shj, Y));
+        y1 = sigmoid_fast(dot_fast<CC>(shj + CC, Y));
+        y2 = sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y));
+      }
+      if (isnan(y0 * coeff)) y0 = 0.0f;
+      if (isnan(y1 * coeff)) y1 = 0.0f;
+      if (isnan(y2 * coeff)) y2 = 0.0f;
+      o0 = o0 + coeff * y0;
+      o1 = o1 + coeff * y1;
+      o2 = o2 + coeff * y2;
+      T = T * (1.0f - al * G);
+      if (T < thresh) {
+        done = true;
+        ndone = base + j + 1;
+      }
+    }
+    if (!__syncthreads_or(!done)) break;
+  }
+  if (!inside) return;
+  if (BG) {
+    if (a.pre_bg) {
+      a.pre_bg[3 * pix + 0] = o0;
+      a.pre_bg[3 * pix + 1] = o1;
+      a.pre_bg[3 * pix + 2] = o2;
+    }
+    if (T > thresh) {
+      const float omt = 1.0f - T;
+      o0 = o0 * T + a.bg[0] * omt;
+      o1 = o1 * T + a.bg[1] * omt;
+      o2 = o2 * T + a.bg[2] * omt;
+    }
+  }
+  a.out[3 * pix + 0] = o0;
+  a.out[3 * pix + 1] = o1;
+  a.out[3 * pix + 2] = o2;
+  if (a.T_final) a.T_final[pix] = T;
+  if (a.n_done) a.n_done[pix] = ndone;
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 reduce-scatter: NV values per lane in, one column sum per lane out.
+// After log2(NV) halving rounds (partner = lane ^ m, m = 32, 16, ...) each
+// group of 64/NV lanes holds the partial sum of component lane >> (6 -
+// log2 NV); the remaining rounds finish the sum.  63 shuffles for NV = 64.
+// ---------------------------------------------------------------------------
+template <int H, int M, int NV>
+__device__ __forceinline__ void reduce_scatter_round(float (&v)[NV], int lane) {
+  if constexpr (H >= 1) {
+    const bool upper = (lane & M) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      const float keep = upper ? v[i + H] : v[i];
+      const float send = upper ? v[i] : v[i + H];
+      v[i] = keep + __shfl_xor(send, M);
+    }
+    reduce_scatter_round<H / 2, M / 2, NV>(v, lane);
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ float wave_reduce_scatter(float (&v)[NV]) {
+  const int lane = lane_id();
+  reduce_scatter_round<NV / 2, 32, NV>(v, lane);
+  float s = v[0];
+#pragma unroll
+  for (int m = 32 / NV; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+  return s;
+}
+
+constexpr int next_pow2(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// Gradient component layout per Gaussian in the LDS accumulator:
+// 0,1 mean; 2 cov00; 3 cov01 (= cov10); 4 cov11; 5 alpha; 6.. colour.
+template <int NCOL>
+__device__ __forceinline__ void flush_grads(const RasterArgs& a, const int* s_id, float* s_acc, int cnt) {
+  constexpr int NCOMP = 6 + NCOL;
+  const int t = threadIdx.x;
+  for (int e = t; e < cnt * NCOMP; e += kTilePixels) {
+    const float v = s_acc[e];
+    if (v == 0.0f) continue;
+    const int j = e / NCOMP;
+    const int c = e - j * NCOMP;
+    const int g = s_id[j];
+    if (c < 2) {
+      unsafeAtomicAdd(a.grad_mean + 2 * (int64_t)g + c, v);
+    } else if (c < 5) {
+      if (c == 2) unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 0, v);
+      if (c == 3) {
+        unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 1, v);
+        unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 2, v);
+      }
+      if (c == 4) unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 3, v);
+    } else if (c == 5) {
+      unsafeAtomicAdd(a.grad_alpha + g, v);
+    } else {
+      const int r = c - 6;
+      if (a.sh_sc == 0) {  // RGB colour [V,3]
+        unsafeAtomicAdd(a.grad_sh + 3 * (int64_t)g + r, v);
+      } else {
+        const int CCl = (int)(NCOL / 3);
+        const int ch = r / CCl;
+        const int k = r - ch * CCl;
+        const int64_t row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
+        unsafeAtomicAdd(a.grad_sh + row * a.sh_sg + ch * a.sh_sc + k, v);
+      }
+    }
+  }
+}
+
+// Block-wide max of a per-thread int (256 threads).
+__device__ __forceinline__ int block_max(int v, int* s_tmp) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
+  if ((t & 63) == 0) s_tmp[t >> 6] = v;
+  __syncthreads();
+  const int r = max(max(s_tmp[0], s_tmp[1]), max(s_tmp[2], s_tmp[3]));
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Backward, SH colour: reverse-order adjoint.  Same gradients as the
+// reference's forward-order recompute (vol_render_sh.h:268-455):
+//   with T_j the transmittance in front of composited Gaussian j, w = a T_j G,
+//   S_j = sum over later composited k of w_k y_k (for bg, plus F - pre_bg),
+//   P = sum_ch g_ch (y_ch T_j - S_j,ch / (1 - a G)),
+//   grad_sh += w y (1-y) g_ch Y_k;  grad_mean += P a G (s, u);
+//   grad_cov += 0.5 P a G (s s, s u, u s, u u);  grad_alpha += P G,
+// s = (dx c3 - dy c2)/det, u = (-dx c1 + dy c0)/det.  T_j is recovered from
+// the stored final transmittance as T_j = T_{j+1} / (1 - a G).
+// ---------------------------------------------------------------------------
+template <int C, bool BG>
+__global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
+  constexpr int CC = C * C;
+  constexpr int SHN = 3 * CC;
+  constexpr int NCOMP = 6 + SHN;
+  constexpr int NV = next_pow2(NCOMP);
+  static_assert(NV <= 64, "gradient vector wider than a wave");
+  __shared__ GeoA s_ga[kBatch];
+  __shared__ GeoB s_gb[kBatch];
+  __shared__ float2 s_gc[kBatch];
+  __shared__ int s_id[kBatch];
+  __shared__ float s_sh[kBatch * SHN];
+  __shared__ float s_acc[kBatch * NCOMP];
+  __shared__ int s_tmp[4];
+
+  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
+  const int tile = xcd_tile(blockIdx.x, n_tiles);
+  if (tile >= n_tiles) return;
+  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
+  const int t = threadIdx.x;
+  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
+  const bool inside = gx < a.W && gy < a.H;
+  const int64_t pix = (int64_t)gy * a.W + gx;
+  const int s0 = a.start[tile];
+  const int e0 = a.end[tile];
+  if (s0 == -1 || e0 - s0 == 0) return;
+
+  const float px = a.topleft[0] + (float)gx * a.psx;
+  const float py = a.topleft[1] + (float)gy * a.psy;
+  float cw[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) cw[k] = a.c2w[k];
+  float dir[3];
+  pixel_direction(cw, px, py, dir);
+  float Y[CC];
+  sh_basis<C>(dir[0], dir[1], dir[2], Y);
+
+  float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
+  int nd = 0;
+  if (inside) {
+    T = a.T_saved[pix];
+    nd = a.n_done_saved[pix];
+    g0 = a.grad_out[3 * pix + 0];
+    g1 = a.grad_out[3 * pix + 1];
+    g2 = a.grad_out[3 * pix + 2];
+    if (BG) {
+      S0 = a.out_saved[3 * pix + 0] - a.pre_bg_saved[3 * pix + 0];
+      S1 = a.out_saved[3 * pix + 1] - a.pre_bg_saved[3 * pix + 1];
+      S2 = a.out_saved[3 * pix + 2] - a.pre_bg_saved[3 * pix + 2];
+    }
+  }
+  const int nmax = block_max(nd, s_tmp);
+  const int lane = lane_id();
+  constexpr int kLaneShift = 6 - __builtin_ctz(NV);
+  const int my_comp = lane >> kLaneShift;
+  const bool writer = (lane & ((1 << kLaneShift) - 1)) == 0 && my_comp < NCOMP;
+
+  for (int base = ((nmax - 1) / kBatch) * kBatch; nmax > 0 && base >= 0; base -= kBatch) {
+    const int cnt = min(kBatch, nmax - base);
+    __syncthreads();
+    for (int e = t; e < cnt * NCOMP; e += kTilePixels) s_acc[e] = 0.0f;
+    stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    __syncthreads();
+    for (int j = cnt - 1; j >= 0; --j) {
+      const int idx = base + j;
+      if (!__any(idx < nd)) continue;
+      float cc0 = 0.0f, cc1 = 0.0f, cc2 = 0.0f, Ds = 0.0f, Du = 0.0f, Dss = 0.0f, Dsu = 0.0f, Duu = 0.0f,
+            PG = 0.0f;
+      bool contrib = false;
+      if (idx < nd) {
+        const GeoA ga = s_ga[j];
+        const GeoB gb = s_gb[j];
+        const float2 gc = s_gc[j];
+        const float dx = px - ga.mx, dy = py - ga.my;
+        const float tmpx = dx * gb.c3 - dy * gb.c2;
+        const float tmpy = -dx * ga.c1 + dy * ga.c0;
+        const float q = tmpx * dx + tmpy * dy;
+        if (!(q * gb.inv_det > gc.y)) {
+          const float G = gauss_exact(q, gb.det);
+          const float al = gc.x;
+          const float aG = al * G;
+          if (!(aG < kMinRenderAlpha)) {
+            contrib = true;
+            const float om = 1.0f - aG;
+            const float Tj = T / om;
+            float w = al * Tj * G;
+            if (isnan(w)) w = 0.0f;
+            const float* shj = s_sh + j * SHN;
+            float y0 = sigmoid_fast(dot_fast<CC>(shj, Y));
+            float y1 = sigmoid_fast(dot_fast<CC>(shj + CC, Y));
+            float y2 = sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y));
+            if (isnan(y0 * w)) y0 = 0.0f;
+            if (isnan(y1 * w)) y1 = 0.0f;
+            if (isnan(y2 * w)) y2 = 0.0f;
+            cc0 = w * (y0 * (1.0f - y0)) * g0;
+            cc1 = w * (y1 * (1.0f - y1)) * g1;
+            cc2 = w * (y2 * (1.0f - y2)) * g2;
+            const float inv_om = 1.0f / om;
+            float P = g0 * (y0 * Tj - S0 * inv_om);
+            P = fmaf(g1, y1 * Tj - S1 * inv_om, P);
+            P = fmaf(g2, y2 * Tj - S2 * inv_om, P);
+            S0 = fmaf(w, y0, S0);
+            S1 = fmaf(w, y1, S1);
+            S2 = fmaf(w, y2, S2);
+            const float D = P * aG;
+            const float inv_det = 1.0f / gb.det;
+            const float s = tmpx * inv_det;
+            const float u = tmpy * inv_det;
+            Ds = D * s;
+            Du = D * u;
+            Dss = 0.5f * Ds * s;
+            Dsu = 0.5f * Ds * u;
+            Duu = 0.5f * Du * u;
+            PG = P * G;
+            T = Tj;
+          }
+        }
+      }
+      if (!__any(contrib)) continue;
+      float v[NV];
+      v[0] = Ds;
+      v[1] = Du;
+      v[2] = Dss;
+      v[3] = Dsu;
+      v[4] = Duu;
+      v[5] = PG;
+#pragma unroll
+      for (int k = 0; k < CC; ++k) {
+        v[6 + k] = cc0 * Y[k];
+        v[6 + CC + k] = cc1 * Y[k];
+        v[6 + 2 * CC + k] = cc2 * Y[k];
+      }
+#pragma unroll
+      for (int k = NCOMP; k < NV; ++k) v[k] = 0.0f;
+      const float r = wave_reduce_scatter<NV>(v);
+      if (writer && r != 0.0f) atomicAdd(s_acc + j * NCOMP + my_comp, r);
+    }
+    __syncthreads();
+    flush_grads<SHN>(a, s_id, s_acc, cnt);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RGB compositing (render_start_end): vol_render.h:716-798 forward,
+// vol_render.h:800-905 backward; the Gaussian is evaluated in double
+// (kernels.h:195-224).  Colour [V,3] is staged through the SH slots with
+// sh_sg = 3, sh_sc = 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stage_rgb(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
+                                          GeoB* s_gb, float2* s_gc, float* s_col) {
+  const int t = threadIdx.x;
+  if (t < cnt) {
+    const int g = a.ids[s0 + base + t];
+    s_id[t] = g;
+    const float2 m = reinterpret_cast<const float2*>(a.mean)[g];
+    const float4 c = reinterpret_cast<const float4*>(a.cov)[g];
+    const float det = c.x * c.w - c.y * c.z;
+    const float al = fminf(a.alpha[g], 0.99f);
+    s_ga[t] = GeoA{m.x, m.y, c.x, c.y};
+    s_gb[t] = GeoB{c.z, c.w, det, 1.0f / det};
+    s_gc[t] = make_float2(al, reject_radius(al));
+    s_col[3 * t + 0] = a.sh[3 * (int64_t)g + 0];
+    s_col[3 * t + 1] = a.sh[3 * (int64_t)g + 1];
+    s_col[3 * t + 2] = a.sh[3 * (int64_t)g + 2];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fwd_rgb(RasterArgs a) {
+  __shared__ GeoA s_ga[kBatch];
+  __shared__ GeoB s_gb[kBatch];
+  __shared__ float2 s_gc[kBatch];
+  __shared__ int s_id[kBatch];
+  __shared__ float s_col[kBatch * 3];
+  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
+  const int tile = xcd_tile(blockIdx.x, n_tiles);
+  if (tile >= n_tiles) return;
+  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
+  const int t = threadIdx.x;
+  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
+  const bool inside = gx < a.W && gy < a.H;
+  const int64_t pix = (int64_t)gy * a.W + gx;
+  const int s0 = a.start[tile];
+  const int e0 = a.end[tile];
+  if (s0 == -1 || e0 - s0 == 0) {
+    if (inside) {
+      if (a.write_empty) a.out[3 * pix] = a.out[3 * pix + 1] = a.out[3 * pix + 2] = 0.0f;
+      if (a.T_final) a.T_final[pix] = 1.0f;
+      if (a.n_done) a.n_done[pix] = 0;
+    }
+    return;
+  }
+  const float px = a.topleft[0] + (float)gx * a.psx;
+  const float py = a.topleft[1] + (float)gy * a.psy;
+  const float thresh = a.thresh;
+  float T = 1.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+  bool done = !inside;
+  const int n = e0 - s0;
+  int ndone = n;
+  if (inside && T < thresh) {
+    done = true;
+    ndone = 0;
+  }
+  for (int base = 0; base < n; base += kBatch) {
+    const int cnt = min(kBatch, n - base);
+    __syncthreads();
+    stage_rgb(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_col);
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      if (__all(done)) break;
+      if (done) continue;
+      const GeoA ga = s_ga[j];
+      const GeoB gb = s_gb[j];
+      const float2 gc = s_gc[j];
+      const float dx = px - ga.mx, dy = py - ga.my;
+      const float tmpx = dx * gb.c3 - dy * gb.c2;
+      const float tmpy = -dx * ga.c1 + dy * ga.c0;
+      const float q = tmpx * dx + tmpy * dy;
+      if (q * gb.inv_det > gc.y) continue;
+      const float G = gauss_exact_d(dx, dy, ga.c0, ga.c1, gb.c2, gb.c3);
+      const float al = gc.x;
+      float coeff = al * T;
+      coeff *= G;
+      if (al * G < kMinRenderAlpha) continue;
+      o0 = o0 + s_col[3 * j + 0] * coeff;
+      o1 = o1 + s_col[3 * j + 1] * coeff;
+      o2 = o2 + s_col[3 * j + 2] * coeff;
+      T = T * (1.0f - al * G);
+      if (T < thresh) {
+        done = true;
+        ndone = base + j + 1;
+      }
+    }
+    if (!__syncthreads_or(!done)) break;
+  }
+  if (!inside) return;
+  a.out[3 * pix + 0] = o0;
+  a.out[3 * pix + 1] = o1;
+  a.out[3 * pix + 2] = o2;
+  if (a.T_final) a.T_final[pix] = T;
+  if (a.n_done) a.n_done[pix] = ndone;
+}
+
+__global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
+  constexpr int NCOMP = 9;
+  constexpr int NV = 16;
+  __shared__ GeoA s_ga[kBatch];
+  __shared__ GeoB s_gb[kBatch];
+  __shared__ float2 s_gc[kBatch];
+  __shared__ int s_id[kBatch];
+  __shared__ float s_col[kBatch * 3];
+  __shared__ float s_acc[kBatch * NCOMP];
+  __shared__ int s_tmp[4];
+  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
+  const int tile = xcd_tile(blockIdx.x, n_tiles);
+  if (tile >= n_tiles) return;
+  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
+  const int t = threadIdx.x;
+  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
+  const bool inside = gx < a.W && gy < a.H;
+  const int64_t pix = (int64_t)gy * a.W + gx;
+  const int s0 = a.start[tile];
+  const int e0 = a.end[tile];
+  if (s0 == -1 || e0 - s0 == 0) return;
+  const float px = a.topleft[0] + (float)gx * a.psx;
+  const float py = a.topleft[1] + (float)gy * a.psy;
+  float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
+  int nd = 0;
+  if (inside) {
+    T = a.T_saved[pix];
+    nd = a.n_done_saved[pix];
+    g0 = a.grad_out[3 * pix + 0];
+    g1 = a.grad_out[3 * pix + 1];
+    g2 = a.grad_out[3 * pix + 2];
+  }
+  const int nmax = block_max(nd, s_tmp);
+  const int lane = lane_id();
+  constexpr int kLaneShift = 6 - __builtin_ctz(NV);
+  const int my_comp = lane >> kLaneShift;
+  const bool writer = (lane & ((1 << kLaneShift) - 1)) == 0 && my_comp < NCOMP;
+  for (int base = ((nmax - 1) / kBatch) * kBatch; nmax > 0 && base >= 0; base -= kBatch) {
+    const int cnt = min(kBatch, nmax - base);
+    __syncthreads();
+    for (int e = t; e < cnt * NCOMP; e += kTilePixels) s_acc[e] = 0.0f;
+    stage_rgb(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_col);
+    __syncthreads();
+    for (int j = cnt - 1; j >= 0; --j) {
+      const int idx = base + j;
+      if (!__any(idx < nd)) continue;
+      float c0v = 0.0f, c1v = 0.0f, c2v = 0.0f, Ds = 0.0f, Du = 0.0f, Dss = 0.0f, Dsu = 0.0f, Duu = 0.0f,
+            PG = 0.0f;
+      bool contrib = false;
+      if (idx < nd) {
+        const GeoA ga = s_ga[j];
+        const GeoB gb = s_gb[j];
+        const float2 gc = s_gc[j];
+        const float dx = px - ga.mx, dy = py - ga.my;
+        const float tmpx = dx * gb.c3 - dy * gb.c2;
+        const float tmpy = -dx * ga.c1 + dy * ga.c0;
+        const float q = tmpx * dx + tmpy * dy;
+        if (!(q * gb.inv_det > gc.y)) {
+          const float G = gauss_exact_d(dx, dy, ga.c0, ga.c1, gb.c2, gb.c3);
+          const float al = gc.x;
+          const float aG = al * G;
+          if (!(aG < kMinRenderAlpha)) {
+            contrib = true;
+            const float om = 1.0f - aG;
+            const float Tj = T / om;
+            const float w = al * Tj * G;
+            const float y0 = s_col[3 * j], y1 = s_col[3 * j + 1], y2 = s_col[3 * j + 2];
+            c0v = w * g0;
+            c1v = w * g1;
+            c2v = w * g2;
+            const float inv_om = 1.0f / om;
+            float P = g0 * (y0 * Tj - S0 * inv_om);
+            P = fmaf(g1, y1 * Tj - S1 * inv_om, P);
+            P = fmaf(g2, y2 * Tj - S2 * inv_om, P);
+            S0 = fmaf(w, y0, S0);
+            S1 = fmaf(w, y1, S1);
+            S2 = fmaf(w, y2, S2);
+            const float D = P * aG;
+            const float inv_det = 1.0f / gb.det;
+            const float s = tmpx * inv_det;
+            const float u = tmpy * inv_det;
+            Ds = D * s;
+            Du = D * u;
+            Dss = 0.5f * Ds * s;
+            Dsu = 0.5f * Ds * u;
+            Duu = 0.5f * Du * u;
+            PG = P * G;
+            T = Tj;
+          }
+        }
+      }
+      if (!__any(contrib)) continue;
+      float v[NV] = {Ds, Du, Dss, Dsu, Duu, PG, c0v, c1v, c2v, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      const float r = wave_reduce_scatter<NV>(v);
+      if (writer && r != 0.0f) atomicAdd(s_acc + j * NCOMP + my_comp, r);
+    }
+    __syncthreads();
+    flush_grads<3>(a, s_id, s_acc, cnt);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+template <int C, bool BG>
+static int launch_fwd_sh(const RasterArgs& a, int exact, hipStream_t st) {
+  const int grid = xcd_grid(a.n_tiles_h * a.n_tiles_w);
+  if (exact)
+    hipLaunchKernelGGL((k_fwd_sh<C, BG, true>), dim3(grid), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_fwd_sh<C, BG, false>), dim3(grid), dim3(256), 0, st, a);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
+template <int C, bool BG>
+static int launch_bwd_sh(const RasterArgs& a, hipStream_t st) {
+  const int grid = xcd_grid(a.n_tiles_h * a.n_tiles_w);
+  hipLaunchKernelGGL((k_bwd_sh<C, BG>), dim3(grid), dim3(256), 0, st, a);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
+static int check_common(int64_t V, int64_t P, int tile_size, int n_tiles_h, int n_tiles_w, int H, int W) {
+  if (V < 0 || P < 0 || n_tiles_h <= 0 || n_tiles_w <= 0 || H <= 0 || W <= 0) return GSMI_ERR_ARG;
+  if (tile_size != kTile) return GSMI_ERR_UNSUPPORTED;
+  if ((int64_t)n_tiles_h * kTile < H || (int64_t)n_tiles_w * kTile < W) return GSMI_ERR_ARG;
+  if ((int64_t)n_tiles_h * n_tiles_w > (1ll << 31) / 2) return GSMI_ERR_ARG;
+  return GSMI_OK;
+}
+
+}  // namespace gsmi
+
+using namespace gsmi;
+
+extern "C" int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const float* mean, const float* cov,
+                                                const float* sh, int64_t sh_stride_g, int64_t sh_stride_c,
+                                                const int32_t* sh_index, const float* alpha, const int32_t* start,
+                                                const int32_t* end, const int32_t* gaussian_ids, float* out,
+                                                const float* topleft, const float* c2w, int tile_size,
+                                                int n_tiles_h, int n_tiles_w, float pixel_size_x,
+                                                float pixel_size_y, int H, int W, int C, float thresh,
+                                                const float* bg, float* T_final, int32_t* n_done, float* pre_bg,
+                                                int flags, int exact, void* stream) {
+  int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
+  if (rc) return rc;
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (!start || !end || !out || !topleft || !c2w) return GSMI_ERR_ARG;
+  if (P > 0 && (!mean || !cov || !sh || !alpha || !gaussian_ids)) return GSMI_ERR_ARG;
+  if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = sh;
+  a.sh_index = sh_index;
+  a.alpha = alpha;
+  a.start = start;
+  a.end = end;
+  a.ids = gaussian_ids;
+  a.topleft = topleft;
+  a.c2w = c2w;
+  a.bg = bg;
+  a.out = out;
+  a.T_final = T_final;
+  a.n_done = n_done;
+  a.pre_bg = pre_bg;
+  a.sh_sg = sh_stride_g;
+  a.sh_sc = sh_stride_c;
+  a.psx = pixel_size_x;
+  a.psy = pixel_size_y;
+  a.thresh = thresh;
+  a.n_tiles_h = n_tiles_h;
+  a.n_tiles_w = n_tiles_w;
+  a.H = H;
+  a.W = W;
+  a.write_empty = flags & 1;
+  hipStream_t st = (hipStream_t)stream;
+  const bool B = bg != nullptr;
+  switch (C) {
+    case 1: return B ? launch_fwd_sh<1, true>(a, exact, st) : launch_fwd_sh<1, false>(a, exact, st);
+    case 2: return B ? launch_fwd_sh<2, true>(a, exact, st) : launch_fwd_sh<2, false>(a, exact, st);
+    case 3: return B ? launch_fwd_sh<3, true>(a, exact, st) : launch_fwd_sh<3, false>(a, exact, st);
+    default: return B ? launch_fwd_sh<4, true>(a, exact, st) : launch_fwd_sh<4, false>(a, exact, st);
+  }
+}
+
+extern "C" int gsmi_tile_based_vol_rendering_backward_sh(
+    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
+    int64_t sh_stride_c, const int32_t* sh_index, const float* alpha, const int32_t* start, const int32_t* end,
+    const int32_t* gaussian_ids, const float* out, const float* T_final, const int32_t* n_done,
+    const float* pre_bg, const float* grad_out, float* grad_mean, float* grad_cov, float* grad_sh,
+    float* grad_alpha, const float* topleft, const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float* bg, void* stream) {
+  int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
+  if (rc) return rc;
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (!start || !end || !topleft || !c2w || !T_final || !n_done || !grad_out) return GSMI_ERR_ARG;
+  if (P > 0 && (!mean || !cov || !sh || !alpha || !gaussian_ids || !grad_mean || !grad_cov || !grad_sh ||
+                !grad_alpha))
+    return GSMI_ERR_ARG;
+  if (bg && (!out || !pre_bg)) return GSMI_ERR_ARG;
+  if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = sh;
+  a.sh_index = sh_index;
+  a.alpha = alpha;
+  a.start = start;
+  a.end = end;
+  a.ids = gaussian_ids;
+  a.topleft = topleft;
+  a.c2w = c2w;
+  a.bg = bg;
+  a.out_saved = out;
+  a.T_saved = T_final;
+  a.n_done_saved = n_done;
+  a.pre_bg_saved = pre_bg;
+  a.grad_out = grad_out;
+  a.grad_mean = grad_mean;
+  a.grad_cov = grad_cov;
+  a.grad_sh = grad_sh;
+  a.grad_alpha = grad_alpha;
+  a.sh_sg = sh_stride_g;
+  a.sh_sc = sh_stride_c;
+  a.psx = pixel_size_x;
+  a.psy = pixel_size_y;
+  a.thresh = thresh;
+  a.n_tiles_h = n_tiles_h;
+  a.n_tiles_w = n_tiles_w;
+  a.H = H;
+  a.W = W;
+  hipStream_t st = (hipStream_t)stream;
+  const bool B = bg != nullptr;
+  switch (C) {
+    case 1: return B ? launch_bwd_sh<1, true>(a, st) : launch_bwd_sh<1, false>(a, st);
+    case 2: return B ? launch_bwd_sh<2, true>(a, st) : launch_bwd_sh<2, false>(a, st);
+    case 3: return B ? launch_bwd_sh<3, true>(a, st) : launch_bwd_sh<3, false>(a, st);
+    default: return B ? launch_bwd_sh<4, true>(a, st) : launch_bwd_sh<4, false>(a, st);
+  }
+}
+
+extern "C" int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, const float* mean, const float* cov,
+                                                       const float* color, const float* alpha,
+                                                       const int32_t* start, const int32_t* end,
+                                                       const int32_t* gaussian_ids, float* out,
+                                                       const float* topleft, int tile_size, int n_tiles_h,
+                                                       int n_tiles_w, float pixel_size_x, float pixel_size_y,
+                                                       int H, int W, float thresh, float* T_final,
+                                                       int32_t* n_done, int flags, void* stream) {
+  int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
+  if (rc) return rc;
+  if (!start || !end || !out || !topleft) return GSMI_ERR_ARG;
+  if (P > 0 && (!mean || !cov || !color || !alpha || !gaussian_ids)) return GSMI_ERR_ARG;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = color;
+  a.alpha = alpha;
+  a.start = start;
+  a.end = end;
+  a.ids = gaussian_ids;
+  a.topleft = topleft;
+  a.out = out;
+  a.T_final = T_final;
+  a.n_done = n_done;
+  a.sh_sg = 3;
+  a.sh_sc = 0;
+  a.psx = pixel_size_x;
+  a.psy = pixel_size_y;
+  a.thresh = thresh;
+  a.n_tiles_h = n_tiles_h;
+  a.n_tiles_w = n_tiles_w;
+  a.H = H;
+  a.W = W;
+  a.write_empty = flags & 1;
+  hipLaunchKernelGGL(k_fwd_rgb, dim3(xcd_grid(n_tiles_h * n_tiles_w)), dim3(256), 0, (hipStream_t)stream, a);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
+extern "C" int gsmi_tile_based_vol_rendering_backward_start_end(
+    int64_t V, int64_t P, const float* mean, const float* cov, const float* color, const float* alpha,
+    const int32_t* start, const int32_t* end, const int32_t* gaussian_ids, const float* out, const float* T_final,
+    const int32_t* n_done, const float* grad_out, float* grad_mean, float* grad_cov, float* grad_color,
+    float* grad_alpha, const float* topleft, int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x,
+    float pixel_size_y, int H, int W, float thresh, void* stream) {
+  int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
+  if (rc) return rc;
+  if (!start || !end || !topleft || !T_final || !n_done || !grad_out) return GSMI_ERR_ARG;
+  if (P > 0 && (!mean || !cov || !color || !alpha || !gaussian_ids || !grad_mean || !grad_cov || !grad_color ||
+                !grad_alpha))
+    return GSMI_ERR_ARG;
+  (void)out;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = color;
+  a.alpha = alpha;
+  a.start = start;
+  a.end = end;
+  a.ids = gaussian_ids;
+  a.topleft = topleft;
+  a.T_saved = T_final;
+  a.n_done_saved = n_done;
+  a.grad_out = grad_out;
+  a.grad_mean = grad_mean;
+  a.grad_cov = grad_cov;
+  a.grad_sh = grad_color;
+  a.grad_alpha = grad_alpha;
+  a.sh_sg = 3;
+  a.sh_sc = 0;
+  a.psx = pixel_size_x;
+  a.psy = pixel_size_y;
+  a.thresh = thresh;
+  a.n_tiles_h = n_tiles_h;
+  a.n_tiles_w = n_tiles_w;
+  a.H = H;
+  a.W = W;
+  hipLaunchKernelGGL(k_bwd_rgb, dim3(xcd_grid(n_tiles_h * n_tiles_w)), dim3(256), 0, (hipStream_t)stream, a);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}

@@ -1,0 +1,168 @@
+// Projection adjoint on gfx950: one thread per raw Gaussian, chaining the
+// compositing gradients (mean2d, cov2d, alpha at the compacted index) back
+// through project_gaussians, the quaternion/scale activations and the masked
+// gathers, with the reference's detaches (gs/renderer.py:365-419: the
+// Jacobian is built under no_grad and depth is detached, so the mean receives
+// gradient only through xy / depth).
+#include "gsmi_device.h"
+
+namespace gsmi {
+
+struct Cam12 {
+  float cw[12];
+};
+
+__global__ __launch_bounds__(256) void k_project_bwd(int64_t N, const float* __restrict__ mean,
+                                                     const float* __restrict__ qvec,
+                                                     const float* __restrict__ log_s,
+                                                     const float* __restrict__ logit, Cam12 cam,
+                                                     const uint8_t* __restrict__ vis,
+                                                     const int32_t* __restrict__ voff,
+                                                     const float* __restrict__ g_m2d,
+                                                     const float* __restrict__ g_cov,
+                                                     const float* __restrict__ g_alpha, float* __restrict__ o_mean,
+                                                     float* __restrict__ o_q, float* __restrict__ o_ls,
+                                                     float* __restrict__ o_logit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (!vis[i]) {
+    o_mean[3 * i] = o_mean[3 * i + 1] = o_mean[3 * i + 2] = 0.0f;
+    reinterpret_cast<float4*>(o_q)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    o_ls[3 * i] = o_ls[3 * i + 1] = o_ls[3 * i + 2] = 0.0f;
+    o_logit[i] = 0.0f;
+    return;
+  }
+  const int64_t v = voff[i];
+  const float* cw = cam.cw;
+  // W = Rc^T, u = W (mu - t)
+  float Wm[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Wm[3 * a + b] = cw[4 * b + a];
+  const float d0 = mean[3 * i] - cw[3], d1 = mean[3 * i + 1] - cw[7], d2 = mean[3 * i + 2] - cw[11];
+  float u[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) u[a] = Wm[3 * a] * d0 + Wm[3 * a + 1] * d1 + Wm[3 * a + 2] * d2;
+  const float z = u[2];
+  const float iz = 1.0f / z;
+
+  // mean: dL/du = (g.x / z, g.y / z, 0); dL/dmu = W^T dL/du.
+  const float2 gm = reinterpret_cast<const float2*>(g_m2d)[v];
+  const float gu0 = gm.x * iz, gu1 = gm.y * iz;
+#pragma unroll
+  for (int b = 0; b < 3; ++b) o_mean[3 * i + b] = Wm[b] * gu0 + Wm[3 + b] * gu1;
+
+  // A = J W (first two rows), J00 = J11 = 1/z, J02 = -x/z^2, J12 = -y/z^2.
+  const float J02 = -u[0] * iz * iz, J12 = -u[1] * iz * iz;
+  float A[6];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    A[k] = iz * Wm[k] + J02 * Wm[6 + k];
+    A[3 + k] = iz * Wm[3 + k] + J12 * Wm[6 + k];
+  }
+  // dL/dSigma = A^T G A with G the 2x2 cov2d gradient (all four entries).
+  const float4 G = reinterpret_cast<const float4*>(g_cov)[v];
+  const float Gm[4] = {G.x, G.y, G.z, G.w};
+  float GA[6];  // G A : 2x3
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) GA[3 * r + b] = Gm[2 * r] * A[b] + Gm[2 * r + 1] * A[3 + b];
+  float gS[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) gS[3 * a + b] = A[a] * GA[b] + A[3 + a] * GA[3 + b];
+
+  // Rotation and scales.
+  const float4 q = reinterpret_cast<const float4*>(qvec)[i];
+  const float nq = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+  const float den = fmaxf(nq, 1e-12f);
+  const float w = q.x / den, x = q.y / den, y = q.z / den, zq = q.w / den;
+  float R[9];
+  R[0] = 1.0f - 2.0f * (y * y + zq * zq);
+  R[1] = 2.0f * (x * y - w * zq);
+  R[2] = 2.0f * (x * zq + w * y);
+  R[3] = 2.0f * (x * y + w * zq);
+  R[4] = 1.0f - 2.0f * (x * x + zq * zq);
+  R[5] = 2.0f * (y * zq - w * x);
+  R[6] = 2.0f * (x * zq - w * y);
+  R[7] = 2.0f * (y * zq + w * x);
+  R[8] = 1.0f - 2.0f * (x * x + y * y);
+  const float s[3] = {exp_spec(log_s[3 * i]), exp_spec(log_s[3 * i + 1]), exp_spec(log_s[3 * i + 2])};
+  float M[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) M[3 * a + b] = R[3 * a + b] * s[b];
+  // Sigma = M M^T  ->  dL/dM = (gS + gS^T) M
+  float gM[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc = fmaf(gS[3 * a + b] + gS[3 * b + a], M[3 * b + c], acc);
+      gM[3 * a + c] = acc;
+    }
+  // M = R diag(s)
+  float gR[9], gs[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      gR[3 * a + b] = gM[3 * a + b] * s[b];
+      gs[b] = fmaf(gM[3 * a + b], R[3 * a + b], gs[b]);
+    }
+  // d R / d (w,x,y,z) of the unit-quaternion matrix.
+  const float gw = 2.0f * (-zq * gR[1] + y * gR[2] + zq * gR[3] - x * gR[5] - y * gR[6] + x * gR[7]);
+  const float gx = 2.0f * (y * gR[1] + zq * gR[2] + y * gR[3] - 2.0f * x * gR[4] - w * gR[5] + zq * gR[6] +
+                           w * gR[7] - 2.0f * x * gR[8]);
+  const float gy = 2.0f * (-2.0f * y * gR[0] + x * gR[1] + w * gR[2] + x * gR[3] + zq * gR[5] - w * gR[6] +
+                           zq * gR[7] - 2.0f * y * gR[8]);
+  const float gz = 2.0f * (-2.0f * zq * gR[0] - w * gR[1] + x * gR[2] + w * gR[3] - 2.0f * zq * gR[4] +
+                           y * gR[5] + x * gR[6] + y * gR[7]);
+  // q_hat = q / max(|q|, eps): dq = (dq_hat - q_hat (q_hat . dq_hat)) / |q|.
+  float4 gq;
+  if (nq > 1e-12f) {
+    const float dotp = w * gw + x * gx + y * gy + zq * gz;
+    gq = make_float4((gw - w * dotp) / nq, (gx - x * dotp) / nq, (gy - y * dotp) / nq, (gz - zq * dotp) / nq);
+  } else {
+    gq = make_float4(gw / den, gx / den, gy / den, gz / den);
+  }
+  reinterpret_cast<float4*>(o_q)[i] = gq;
+  // s = exp(log_s)
+#pragma unroll
+  for (int b = 0; b < 3; ++b) o_ls[3 * i + b] = gs[b] * s[b];
+  // alpha = sigmoid(logit)
+  const float al = 1.0f / (1.0f + exp_spec(-logit[i]));
+  o_logit[i] = g_alpha[v] * (al * (1.0f - al));
+}
+
+}  // namespace gsmi
+
+using namespace gsmi;
+
+extern "C" int gsmi_project_backward(int64_t N, const float* mean, const float* qvec, const float* log_s,
+                                     const float* logit, const float* c2w_host, const uint8_t* vis,
+                                     const int32_t* vis_off, const float* grad_mean2d, const float* grad_cov,
+                                     const float* grad_alpha, float* grad_mean, float* grad_qvec,
+                                     float* grad_log_s, float* grad_logit, void* stream) {
+  if (N < 0 || !c2w_host) return GSMI_ERR_ARG;
+  if (N == 0) return GSMI_OK;
+  if (!mean || !qvec || !log_s || !logit || !vis || !vis_off || !grad_mean2d || !grad_cov || !grad_alpha ||
+      !grad_mean || !grad_qvec || !grad_log_s || !grad_logit)
+    return GSMI_ERR_ARG;
+  if (((uintptr_t)qvec & 15) || ((uintptr_t)grad_qvec & 15) || ((uintptr_t)grad_cov & 15) ||
+      ((uintptr_t)grad_mean2d & 7))
+    return GSMI_ERR_ARG;
+  Cam12 cam;
+  for (int k = 0; k < 12; ++k) cam.cw[k] = c2w_host[k];
+  hipLaunchKernelGGL(k_project_bwd, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, (hipStream_t)stream, N, mean,
+                     qvec, log_s, logit, cam, vis, vis_off, grad_mean2d, grad_cov, grad_alpha, grad_mean, grad_qvec,
+                     grad_log_s, grad_logit);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}

@@ -1,0 +1,211 @@
+"""Fused gfx950 pipeline for one camera: raw Gaussians -> image, and back.
+
+Forward (all on the current torch stream):
+  1. gsmi_preprocess_count  — activation, frustum cull, projection, tile
+     bounds and pair count per raw Gaussian; prefix sums give the compacted
+     index and the pair offset of every visible Gaussian.
+  2. one host read of (V, P) to size the outputs (the reference reads P the
+     same way, gs/culling.py:33-35);
+  3. gsmi_preprocess_emit   — compacted SoA records (mean2d, cov2d, depth,
+     alpha, tile bounds, original index) and the (tile<<32|depth) keys;
+  4. gsmi_sort_pairs        — stable LSD radix sort on 32 + ceil(log2 T) bits;
+  5. gsmi_tile_ranges       — start/end per tile;
+  6. gsmi_tile_based_vol_rendering_sh — compositing; SH rows are read from the
+     raw [N,3,C_max^2] tensor through the original index (no gather copy).
+Backward:
+  7. gsmi_tile_based_vol_rendering_backward_sh — reverse-order adjoint; SH
+     gradients land directly in the raw-layout [N,3,C_max^2] buffer;
+  8. gsmi_project_backward  — projection/activation adjoint with the
+     reference's detaches, zero for culled Gaussians.
+This reproduces SHRenderer.forward (gs/sh_renderer.py:188-316) and the
+gradients its autograd graph produces.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, host_f32, ptr, stream_of
+from .backend import _backend
+from .camera import CameraInfo
+
+TILE = 16
+
+
+def _bits_for(n):
+    b = 0
+    while (1 << b) < n:
+        b += 1
+    return b
+
+
+@dataclass
+class Binning:
+    """Per-view intermediates kept for the backward pass and for inspection."""
+
+    N: int
+    V: int
+    P: int
+    vis: torch.Tensor
+    vis_off: torch.Tensor
+    pair_off: torch.Tensor
+    mean2d: torch.Tensor
+    cov2d: torch.Tensor
+    depth: torch.Tensor
+    alpha: torch.Tensor
+    tl: torch.Tensor
+    br: torch.Tensor
+    orig: torch.Tensor
+    keys: torch.Tensor  # int64 storage of the uint64 keys, sorted
+    ids: torch.Tensor
+    start: torch.Tensor
+    end: torch.Tensor
+    c2w_host: np.ndarray
+    consts: dict
+
+
+def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, tile_size=TILE, D=6.0,
+                       frustum_culling_radius=1.0, skip_frustum_culling=False) -> Binning:
+    """Steps 1-5 of the module docstring."""
+    L = _lib.load()
+    dev = mean.device
+    st = stream_of(dev)
+    N = int(mean.shape[0])
+    c2w_h = np.ascontiguousarray(
+        (c2w.detach().cpu().numpy() if isinstance(c2w, torch.Tensor) else np.asarray(c2w)), np.float32
+    ).reshape(3, 4)
+    normals, pts = camera.frustum_planes_np(c2w_h)
+    ic = camera.image_constants(tile_size)
+    f32 = np.float32
+    fx, fy, cx, cy = float(f32(camera.fx)), float(f32(camera.fy)), float(f32(camera.cx)), float(f32(camera.cy))
+    vis = torch.empty(N, dtype=torch.uint8, device=dev)
+    vis_off = torch.empty(N, dtype=torch.int32, device=dev)
+    pair_off = torch.empty(N, dtype=torch.int32, device=dev)
+    totals = torch.empty(2, dtype=torch.int64, device=dev)
+    ws = torch.empty(max(int(L.gsmi_preprocess_workspace_size(N)), 1), dtype=torch.uint8, device=dev)
+    c2w_c = host_f32(c2w_h)
+    check(L.gsmi_preprocess_count(N, ptr(mean), ptr(qvec), ptr(log_s), c2w_c, host_f32(normals), host_f32(pts),
+                                  fx, fy, cx, cy, ic["W"], ic["H"], float(D), float(frustum_culling_radius),
+                                  1 if skip_frustum_culling else 0, tile_size, ptr(vis), ptr(vis_off),
+                                  ptr(pair_off), ptr(totals), ptr(ws), ws.numel(), st), "preprocess_count")
+    V, P = (int(x) for x in totals.cpu().tolist())
+    if P >= 2 ** 31:
+        raise RuntimeError(f"{P} (Gaussian, tile) pairs exceed the int32 range of the tile lists")
+    mean2d = torch.empty(V, 2, dtype=torch.float32, device=dev)
+    cov2d = torch.empty(V, 2, 2, dtype=torch.float32, device=dev)
+    depth = torch.empty(V, 1, dtype=torch.float32, device=dev)
+    alpha = torch.empty(V, dtype=torch.float32, device=dev)
+    tl = torch.empty(V, 2, dtype=torch.int32, device=dev)
+    br = torch.empty(V, 2, dtype=torch.int32, device=dev)
+    orig = torch.empty(V, dtype=torch.int32, device=dev)
+    keys = torch.empty(P, dtype=torch.int64, device=dev)
+    ids = torch.empty(P, dtype=torch.int32, device=dev)
+    check(L.gsmi_preprocess_emit(N, ptr(mean), ptr(qvec), ptr(log_s), ptr(logit), c2w_c, fx, fy, cx, cy, ic["W"],
+                                 ic["H"], float(D), tile_size, ptr(vis), ptr(vis_off), ptr(pair_off), ptr(mean2d),
+                                 ptr(cov2d), ptr(depth), ptr(alpha), ptr(tl), ptr(br), ptr(orig), ptr(keys),
+                                 ptr(ids), P, st), "preprocess_emit")
+    n_tiles = ic["n_tiles_h"] * ic["n_tiles_w"]
+    sws = torch.empty(max(int(L.gsmi_sort_workspace_size(P)), 1), dtype=torch.uint8, device=dev)
+    check(L.gsmi_sort_pairs(P, ptr(keys), ptr(ids), 32 + _bits_for(n_tiles), ptr(sws), sws.numel(), st),
+          "sort_pairs")
+    start = torch.empty(n_tiles, dtype=torch.int32, device=dev)
+    end = torch.empty(n_tiles, dtype=torch.int32, device=dev)
+    check(L.gsmi_tile_ranges(P, ptr(keys), n_tiles, ptr(start), ptr(end), st), "tile_ranges")
+    return Binning(N, V, P, vis, vis_off, pair_off, mean2d, cov2d, depth, alpha, tl, br, orig, keys, ids, start,
+                   end, c2w_h, ic)
+
+
+@dataclass
+class ViewState:
+    """Everything the backward pass of one view needs."""
+
+    b: Binning
+    C: int
+    cfg: object
+    topleft: torch.Tensor
+    c2w_d: torch.Tensor
+    bg: torch.Tensor | None
+    out: torch.Tensor
+    T: torch.Tensor
+    n_done: torch.Tensor
+    pre_bg: torch.Tensor | None
+
+
+def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact=False):
+    """Steps 1-6: returns (image [H*W*3], ViewState)."""
+    b = preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera, tile_size=cfg.tile_size, D=cfg.tile_culling_radius,
+                           frustum_culling_radius=cfg.frustum_culling_radius,
+                           skip_frustum_culling=cfg.skip_frustum_culling)
+    ic = b.consts
+    H, W = ic["H"], ic["W"]
+    dev = mean.device
+    topleft = torch.tensor(ic["topleft"], dtype=torch.float32, device=dev)
+    c2w_d = torch.from_numpy(b.c2w_host).to(dev)
+    bg = torch.tensor(cfg.bg_rgb, dtype=torch.float32, device=dev) if cfg.bg else None
+    out = torch.empty(H * W * 3, dtype=torch.float32, device=dev)
+    T = torch.empty(H * W, dtype=torch.float32, device=dev)
+    nd = torch.empty(H * W, dtype=torch.int32, device=dev)
+    pre = torch.empty(H * W * 3, dtype=torch.float32, device=dev) if cfg.bg else None
+    cmax2 = sh_coeffs.shape[-1]
+    _backend.sh_forward(b.mean2d, b.cov2d, sh_coeffs, b.alpha, b.start, b.end, b.ids, out, topleft, c2w_d,
+                        cfg.tile_size, ic["n_tiles_h"], ic["n_tiles_w"], ic["pixel_size_x"], ic["pixel_size_y"], H,
+                        W, C, cfg.T_thresh, bg_rgb=bg, T_final=T, n_done=nd, pre_bg=pre, write_empty=True,
+                        exact=exact, sh_stride_g=3 * cmax2, sh_stride_c=cmax2, sh_index=b.orig)
+    return out, ViewState(b, int(C), cfg, topleft, c2w_d, bg, out, T, nd, pre)
+
+
+def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g_sh=None):
+    """Steps 7-8: gradients w.r.t. the raw parameters.  SH gradients are
+    accumulated into g_sh when given (several views), else a new buffer.
+    Returns (g_mean, g_qvec, g_log_s, g_logit, g_sh, boundary) where boundary
+    holds the compositing gradients (mean2d, cov2d, alpha) of visible
+    Gaussians."""
+    b, cfg, ic = s.b, s.cfg, s.b.consts
+    dev = mean.device
+    L = _lib.load()
+    g_m2d = torch.zeros(b.V, 2, dtype=torch.float32, device=dev)
+    g_cov = torch.zeros(b.V, 2, 2, dtype=torch.float32, device=dev)
+    g_al = torch.zeros(b.V, dtype=torch.float32, device=dev)
+    if g_sh is None:
+        g_sh = torch.zeros_like(sh_coeffs)
+    cmax2 = sh_coeffs.shape[-1]
+    _backend.sh_backward(b.mean2d, b.cov2d, sh_coeffs, b.alpha, b.start, b.end, b.ids, s.out, s.T, s.n_done,
+                         s.pre_bg, grad_out.contiguous().view(-1), g_m2d, g_cov, g_sh, g_al, s.topleft, s.c2w_d,
+                         cfg.tile_size, ic["n_tiles_h"], ic["n_tiles_w"], ic["pixel_size_x"], ic["pixel_size_y"],
+                         ic["H"], ic["W"], s.C, cfg.T_thresh, bg_rgb=s.bg, sh_stride_g=3 * cmax2, sh_stride_c=cmax2,
+                         sh_index=b.orig)
+    g_mean = torch.empty_like(mean)
+    g_q = torch.empty_like(qvec)
+    g_ls = torch.empty_like(log_s)
+    g_lg = torch.empty_like(logit)
+    check(L.gsmi_project_backward(b.N, ptr(mean), ptr(qvec), ptr(log_s), ptr(logit), host_f32(b.c2w_host),
+                                  ptr(b.vis), ptr(b.vis_off), ptr(g_m2d), ptr(g_cov), ptr(g_al), ptr(g_mean),
+                                  ptr(g_q), ptr(g_ls), ptr(g_lg), stream_of(dev)), "project_backward")
+    return g_mean, g_q, g_ls, g_lg, g_sh, dict(mean2d=g_m2d, cov=g_cov, alpha=g_al)
+
+
+class _RasterizeSH(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg):
+        mean, qvec, log_s, logit, sh_coeffs = (t.contiguous() for t in (mean, qvec, log_s, logit, sh_coeffs))
+        out, state = forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg)
+        ctx.state = state
+        ctx.save_for_backward(mean, qvec, log_s, logit, sh_coeffs)
+        ic = state.b.consts
+        return out.view(ic["H"], ic["W"], 3)
+
+    @staticmethod
+    def backward(ctx, grad):
+        mean, qvec, log_s, logit, sh_coeffs = ctx.saved_tensors
+        g_mean, g_q, g_ls, g_lg, g_sh, _ = backward_view(ctx.state, mean, qvec, log_s, logit, sh_coeffs, grad)
+        return g_mean, g_q, g_ls, g_lg, g_sh, None, None, None, None
+
+
+def rasterize_sh(mean, qvec, svec_before_activation, alpha_before_activation, sh_coeffs, c2w, camera, C, cfg):
+    """Image [H,W,3] of one camera; differentiable w.r.t. the five raw
+    parameter tensors (SHRenderer storage convention)."""
+    return _RasterizeSH.apply(mean, qvec, svec_before_activation, alpha_before_activation, sh_coeffs, c2w, camera,
+                              int(C), cfg)

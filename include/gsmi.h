@@ -1,0 +1,264 @@
+/*
+ * gsmi.h — C ABI of the MI355X (gfx950) Gaussian-splatting rasteriser.
+ *
+ * Plain pointers and sizes only.  Every device pointer is a gfx950 device
+ * address; `stream` is a hipStream_t passed as void* (NULL = the null
+ * stream).  All calls are asynchronous on `stream` unless stated otherwise,
+ * never allocate device memory, never exit the process, and return a status
+ * code (GSMI_OK on success).  Outputs are caller-owned, as in the reference
+ * (gs/renderer.py:695, 760-763; gs/sh_renderer.py:245-249).
+ *
+ * Float conventions shared with the kernels: binary32, no multiply-add
+ * contraction where a value feeds a decision (culling, tile bounds, the
+ * 1/255 skip, the transmittance break), and the exponential evaluated by the
+ * specified algorithm documented in gsmi_exp_spec() below.
+ */
+#ifndef GSMI_H
+#define GSMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GSMI_OK = 0,
+  GSMI_ERR_ARG = 1,         /* null pointer, negative size, bad dimension */
+  GSMI_ERR_UNSUPPORTED = 2, /* e.g. SH order C outside 1..4, tile size != 16 */
+  GSMI_ERR_LAUNCH = 3,      /* a kernel launch or runtime call failed */
+  GSMI_ERR_WORKSPACE = 4,   /* workspace smaller than the size query returned */
+  GSMI_ERR_COUNT = 5        /* pair count differs from the one the caller sized */
+};
+
+/* Human-readable text for a status code (static storage). */
+const char *gsmi_status_string(int status);
+
+/* ABI version: major * 10000 + minor * 100 + patch. */
+int gsmi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Frustum culling                                                           */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces culling_gaussian_bsphere (gs/src/render.cu:15-43,
+ * gs/src/include/culling.h:11-34; bound at gs/src/bindings.cpp:6).
+ * mask[i] = 1 iff for all six planes k: dot(mean_i - pts_k, normals_k) >
+ * -max(svec_i) * thresh.  mean, svec: [N,3]; qvec [N,4] is accepted and
+ * unused, as in the reference; normals, pts: [6,3] device arrays.
+ */
+int gsmi_culling_gaussian_bsphere(int64_t N, const float *mean, const float *qvec, const float *svec,
+                                  const float *normals, const float *pts, uint8_t *mask, float thresh,
+                                  void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Tile bounds and binning                                                   */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces the PyTorch tile_culling_aabb_count (gs/culling.py:8-37 with
+ * utils/camera.py:290-303).  For V projected Gaussians (mean2d [V,2], cov2d
+ * [V,4] row-major 2x2) writes int32 tile bounds tl, br [V,2] and the pair
+ * count P = sum (brx-tlx+1)(bry-tly+1) into *d_pair_count (device int64).
+ * fx, fy, cx, cy, D are the camera/config scalars rounded to binary32.
+ */
+size_t gsmi_aabb_count_workspace_size(int64_t V);
+
+int gsmi_tile_culling_aabb_count(int64_t V, const float *mean2d, const float *cov2d, float fx, float fy,
+                                 float cx, float cy, int W, int H, float D, int tile_size, int32_t *tl,
+                                 int32_t *br, int64_t *d_pair_count, void *workspace, size_t workspace_bytes,
+                                 void *stream);
+
+/* Workspace bytes needed by gsmi_tile_culling_aabb_start_end. */
+size_t gsmi_binning_workspace_size(int64_t V, int64_t P, int n_tiles);
+
+/*
+ * Replaces tile_culling_aabb_start_end (gs/src/render.cu:380-397,
+ * gs/src/include/aabb_culling.h:15-41, 70-103, 192-260; bound at
+ * bindings.cpp:29).  Emits one (key, value) pair per covered tile,
+ * key = (int64)(y*n_tiles_w + x) << 32 | float_bits(depth[v]), value = v,
+ * in index order; sorts by key (stable LSD radix sort over the key bits that
+ * can be non-zero); fills start/end [n_tiles] (-1 for empty tiles) and
+ * gaussian_ids [P] (the sorted values).  sorted_keys [P] is optional (NULL to
+ * skip).  P must equal the pair count of tl/br, else GSMI_ERR_COUNT.
+ */
+int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int32_t *tl, const int32_t *br,
+                                     const float *depth, int n_tiles_h, int n_tiles_w, int32_t *gaussian_ids,
+                                     int32_t *start, int32_t *end, uint64_t *sorted_keys, void *workspace,
+                                     size_t workspace_bytes, void *stream);
+
+/* Workspace bytes for gsmi_sort_pairs (radix sort of P (uint64, int32) pairs). */
+size_t gsmi_sort_workspace_size(int64_t P);
+
+/*
+ * Stable ascending sort of P (key, value) pairs on key bits [0, end_bit),
+ * end_bit <= 64; the pairs are sorted in place (keys/values overwritten).
+ * Replaces cub::DeviceRadixSort::SortPairs<int64_t,int>
+ * (gs/src/include/aabb_culling.h:235-241).
+ */
+int gsmi_sort_pairs(int64_t P, uint64_t *keys, int32_t *values, int end_bit, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused preprocessing (activation, cull, projection, tile bounds, keys)     */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces, as one coalesced pass over the N raw Gaussians, the chain
+ * SHRenderer.forward (gs/sh_renderer.py:188-262): exp/sigmoid activations
+ * (sh_renderer.py:318-324), culling_gaussian_bsphere, the masked gathers
+ * (:205-209), project_gaussians (gs/renderer.py:365-419) and
+ * tile_culling_aabb_count (gs/culling.py:8-37).
+ *
+ * Phase 1 (gsmi_preprocess_count): per Gaussian i, vis[i] = survives the
+ * frustum cull; pairs[i] = its tile-pair count (0 when culled).  Then
+ * exclusive prefix sums: vis_off[i], pair_off[i], and totals[0] = V,
+ * totals[1] = P (device int64[2]).  The caller reads totals to size the
+ * phase-2 outputs.
+ * Camera: c2w_host [3,4] row-major and normals_host / pts_host [6,3]
+ * (CameraInfo.get_frustum) are HOST arrays, copied into the kernel
+ * arguments; fx, fy, cx, cy, W, H, D host scalars.  mean, qvec, log_s are
+ * device arrays [N,3], [N,4] (16-byte aligned), [N,3].
+ */
+size_t gsmi_preprocess_workspace_size(int64_t N);
+
+int gsmi_preprocess_count(int64_t N, const float *mean, const float *qvec, const float *log_s,
+                          const float *c2w_host, const float *normals_host, const float *pts_host, float fx,
+                          float fy, float cx, float cy, int W, int H, float D, float frustum_radius,
+                          int skip_culling, int tile_size, uint8_t *vis, int32_t *vis_off, int32_t *pair_off,
+                          int64_t *totals, void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * Phase 2: for every visible i with v = vis_off[i], writes the compacted
+ * mean2d[v] (2), cov2d[v] (4), depth[v], alpha[v] (sigmoid of logit),
+ * tl[v], br[v] (2 each) and orig[v] = i, and emits its tile pairs at
+ * pair_off[i] into keys/values (index order), never at or beyond P (the
+ * pair total of phase 1).
+ */
+int gsmi_preprocess_emit(int64_t N, const float *mean, const float *qvec, const float *log_s, const float *logit,
+                         const float *c2w_host, float fx, float fy, float cx, float cy, int W, int H, float D,
+                         int tile_size, const uint8_t *vis, const int32_t *vis_off, const int32_t *pair_off,
+                         float *mean2d, float *cov2d, float *depth, float *alpha, int32_t *tl, int32_t *br,
+                         int32_t *orig, uint64_t *keys, int32_t *values, int64_t P, void *stream);
+
+/*
+ * Tile ranges from sorted keys: start/end [n_tiles], -1 when empty
+ * (gs/src/include/aabb_culling.h:70-103, 248-249).
+ */
+int gsmi_tile_ranges(int64_t P, const uint64_t *sorted_keys, int n_tiles, int32_t *start, int32_t *end,
+                     void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Compositing                                                               */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces tile_based_vol_rendering_sh (gs/src/render.cu:483-544,
+ * gs/src/include/vol_render_sh.h:97-262; bound at bindings.cpp:42) and, with
+ * bg != NULL, tile_based_vol_rendering_sh_with_bg (render.cu:780-842,
+ * vol_render_bg.h:12-119).
+ *
+ * mean [V,2], cov [V,4], alpha [V] (post-sigmoid), start/end [n_tiles],
+ * gaussian_ids [P], topleft [2] and c2w [3,4] are device arrays as in the
+ * reference.  SH coefficient (g, ch, k) is read at
+ * sh[sh_row(g) * sh_stride_g + ch * sh_stride_c + k], sh_row(g) =
+ * sh_index ? sh_index[g] : g (the reference layout is sh_stride_g = 3*C*C,
+ * sh_stride_c = C*C, sh_index = NULL).
+ * out [H*W*3] (HWC).  flags bit 0: also write pixels of empty tiles (zeros;
+ * the reference leaves them untouched).  Optional per-pixel state for the
+ * backward pass (NULL to skip): T_final [H*W], n_done [H*W] (list entries
+ * examined before the transmittance break), pre_bg [H*W*3] (colour before
+ * the background blend; bg mode only).  bg: NULL or device float[3].
+ * exact != 0 evaluates colours with the oracle's exact operation sequence
+ * (bit-reproducible output); exact == 0 uses fused multiply-adds for the SH
+ * dot product and the hardware exponential for the colour sigmoid.  All
+ * compositing decisions are exact in both modes.
+ */
+int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const float *mean, const float *cov, const float *sh,
+                                     int64_t sh_stride_g, int64_t sh_stride_c, const int32_t *sh_index,
+                                     const float *alpha, const int32_t *start, const int32_t *end,
+                                     const int32_t *gaussian_ids, float *out, const float *topleft,
+                                     const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+                                     float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh,
+                                     const float *bg, float *T_final, int32_t *n_done, float *pre_bg, int flags,
+                                     int exact, void *stream);
+
+/*
+ * Replaces tile_based_vol_rendering_backward_sh (render.cu:546-624,
+ * vol_render_sh.h:268-476; bound at bindings.cpp:44) and its _with_bg
+ * variant (vol_render_bg.h:121-258).  Reverse-order adjoint: starts from the
+ * forward's T_final and n_done, recomputes transmittance backwards, reduces
+ * per-Gaussian gradients across each wavefront, then accumulates them
+ * (float atomics) into grad_mean [V,2], grad_cov [V,4] (both off-diagonals
+ * receive 0.5 * dL/dcov_offdiag, as the reference), grad_sh (same addressing
+ * as sh, strides sh_stride_g / sh_stride_c through sh_index) and grad_alpha
+ * [V] (w.r.t. the post-sigmoid alpha, straight through the 0.99 clamp).
+ * out is the forward's output (for bg, the blended colour) and pre_bg the
+ * colour before blending (bg mode only).  grad_* must be zero-initialised or
+ * hold values to accumulate onto.
+ */
+int gsmi_tile_based_vol_rendering_backward_sh(
+    int64_t V, int64_t P, const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
+    int64_t sh_stride_c, const int32_t *sh_index, const float *alpha, const int32_t *start, const int32_t *end,
+    const int32_t *gaussian_ids, const float *out, const float *T_final, const int32_t *n_done,
+    const float *pre_bg, const float *grad_out, float *grad_mean, float *grad_cov, float *grad_sh,
+    float *grad_alpha, const float *topleft, const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float *bg, void *stream);
+
+/*
+ * Replaces tile_based_vol_rendering_start_end (render.cu:399-441,
+ * vol_render.h:716-798; bound at bindings.cpp:31): RGB compositing with the
+ * Gaussian evaluated in double (kernels.h:195-224).  color [V,3].
+ */
+int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, const float *mean, const float *cov,
+                                            const float *color, const float *alpha, const int32_t *start,
+                                            const int32_t *end, const int32_t *gaussian_ids, float *out,
+                                            const float *topleft, int tile_size, int n_tiles_h, int n_tiles_w,
+                                            float pixel_size_x, float pixel_size_y, int H, int W, float thresh,
+                                            float *T_final, int32_t *n_done, int flags, void *stream);
+
+/* Replaces tile_based_vol_rendering_backward_start_end (render.cu:443-481,
+ * vol_render.h:800-925; bound at bindings.cpp:35). */
+int gsmi_tile_based_vol_rendering_backward_start_end(
+    int64_t V, int64_t P, const float *mean, const float *cov, const float *color, const float *alpha,
+    const int32_t *start, const int32_t *end, const int32_t *gaussian_ids, const float *out,
+    const float *T_final, const int32_t *n_done, const float *grad_out, float *grad_mean, float *grad_cov,
+    float *grad_color, float *grad_alpha, const float *topleft, int tile_size, int n_tiles_h, int n_tiles_w,
+    float pixel_size_x, float pixel_size_y, int H, int W, float thresh, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Projection adjoint                                                        */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces the torch autograd backward through project_gaussians and the
+ * activations (gs/renderer.py:390-419, utils/transforms.py:31-45,
+ * utils/activations.py:13-19, sh_renderer.py:205-209 masked gathers):
+ * keeps the reference's detaches (Jacobian and depth are constants, the
+ * mean gradient flows only through xy / depth).  For i < N: culled
+ * Gaussians (vis[i] == 0) get zero gradients; visible ones read the
+ * compositing gradients at v = vis_off[i] (grad_mean2d [V,2], grad_cov [V,4],
+ * grad_alpha [V]) and write grad_mean [N,3], grad_qvec [N,4], grad_log_s
+ * [N,3], grad_logit [N] (overwrite, not accumulate).  c2w_host is a host
+ * array [3,4].
+ */
+int gsmi_project_backward(int64_t N, const float *mean, const float *qvec, const float *log_s, const float *logit,
+                          const float *c2w_host, const uint8_t *vis, const int32_t *vis_off, const float *grad_mean2d,
+                          const float *grad_cov, const float *grad_alpha, float *grad_mean, float *grad_qvec,
+                          float *grad_log_s, float *grad_logit, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Test hooks                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* y[i] = the specified exponential of x[i] (binary32), n elements; used by
+ * the tests to check it bit for bit against the oracle. */
+int gsmi_exp_spec(int64_t n, const float *x, float *y, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSMI_H */

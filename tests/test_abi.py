@@ -1,0 +1,90 @@
+"""The C-ABI library loads and exports exactly what include/gsmi.h declares.
+
+No compute call is made (no GPU here): only load, symbol lookup, parameter
+counts against the header, and the pure-host helpers (status strings,
+version, workspace-size queries).
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gsmi.h")
+
+
+def _declarations():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    decls = {}
+    for m in re.finditer(r"\b(gsmi_\w+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+        args = m.group(2).strip()
+        n = 0 if args in ("", "void") else args.count(",") + 1
+        decls[m.group(1)] = n
+    return decls
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from gaussian_splatting_3d_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "gaussian_splatting_3d_amd", "csrc")], check=True)
+    return _lib.load()
+
+
+def test_header_declares_entry_points():
+    d = _declarations()
+    for name in ("gsmi_tile_based_vol_rendering_sh", "gsmi_tile_based_vol_rendering_backward_sh",
+                 "gsmi_tile_culling_aabb_start_end", "gsmi_culling_gaussian_bsphere", "gsmi_sort_pairs",
+                 "gsmi_preprocess_count", "gsmi_preprocess_emit", "gsmi_project_backward"):
+        assert name in d
+
+
+def test_library_exports_every_declared_symbol(lib):
+    from gaussian_splatting_3d_amd import _lib
+
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = set(_declarations()) - exported
+    assert not missing, missing
+    for name in _declarations():
+        assert getattr(lib, name) is not None
+
+
+def test_ctypes_signatures_match_header(lib):
+    from gaussian_splatting_3d_amd import _lib
+
+    decls = _declarations()
+    assert set(_lib.SIGNATURES) == set(decls)
+    for name, n in decls.items():
+        assert len(_lib.SIGNATURES[name][1]) == n, name
+
+
+def test_host_helpers(lib):
+    assert lib.gsmi_version() >= 100
+    assert lib.gsmi_status_string(0) == b"ok"
+    assert b"workspace" in lib.gsmi_status_string(4)
+    assert lib.gsmi_sort_workspace_size(0) > 0
+    a, b = lib.gsmi_sort_workspace_size(10_000), lib.gsmi_sort_workspace_size(1_000_000)
+    assert b > a > 10_000 * 12
+    assert lib.gsmi_binning_workspace_size(1000, 5000, 256) >= lib.gsmi_sort_workspace_size(5000)
+    assert lib.gsmi_preprocess_workspace_size(1_000_000) >= 8_000_000
+
+
+def test_argument_validation_without_gpu(lib):
+    """Invalid sizes are rejected before any launch."""
+    assert lib.gsmi_sort_pairs(-1, None, None, 40, None, 0, None) == 1
+    assert lib.gsmi_sort_pairs(10, None, None, 40, None, 0, None) == 1
+    assert lib.gsmi_tile_ranges(0, None, 0, None, None, None) == 1
+    P = ctypes.c_void_p(16)
+    # C = 5 is outside the supported SH orders
+    rc = lib.gsmi_tile_based_vol_rendering_sh(1, 1, P, P, P, 75, 25, None, P, P, P, P, P, P, P, 16, 1, 1, 1.0, 1.0,
+                                              16, 16, 5, 1e-4, None, None, None, None, 0, 0, None)
+    assert rc == 2
+    rc = lib.gsmi_tile_based_vol_rendering_sh(1, 1, P, P, P, 3, 1, None, P, P, P, P, P, P, P, 8, 1, 1, 1.0, 1.0,
+                                              16, 16, 1, 1e-4, None, None, None, None, 0, 0, None)
+    assert rc == 2  # tile size 8

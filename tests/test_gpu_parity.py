@@ -1,0 +1,323 @@
+"""GPU parity: gfx950 kernels through the C ABI against the CPU oracle.
+
+Bars: bit-exact for masks, tile bounds, keys, sorted ids and tile ranges, and
+for every compositing decision (exact colour mode gives bit-identical
+images); images within 1e-4 max-abs in fast colour mode; gradients within
+1e-3 relative (max-norm per tensor).
+"""
+import numpy as np
+import pytest
+import torch
+
+from gaussian_splatting_3d_amd import _lib, rasterize, scenes
+from gaussian_splatting_3d_amd.backend import _backend
+from gaussian_splatting_3d_amd.camera import CameraInfo
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+IMG_TOL = 1e-4
+GRAD_RTOL = 1e-3
+
+
+def ocam(cam):
+    from oracle import oracle as O
+
+    return O.Camera(cam.fx, cam.fy, cam.cx, cam.cy, cam.w, cam.h, cam.near_plane, cam.far_plane)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def dev_params(sc):
+    return [t.to(DEV).contiguous() for t in (sc.mean, sc.qvec, sc.svec_before_activation,
+                                             sc.alpha_before_activation, sc.sh_coeffs)]
+
+
+def small_sh_scene(C, n=4000, W=96, H=80, seed=11):
+    """SH-order-C scene in view of a 96x80 camera (ragged edge tiles)."""
+    g = torch.Generator().manual_seed(seed)
+    mean = torch.randn(n, 3, generator=g) * 0.8
+    log_s = torch.randn(n, 3, generator=g) * 0.5 - 3.0
+    q = torch.randn(n, 4, generator=g)
+    q = q / q.norm(dim=-1, keepdim=True)
+    logit = torch.randn(n, generator=g)
+    sh = torch.randn(n, 3, C * C, generator=g) * 0.1
+    sh[:, :, 0] = torch.randn(n, 3, generator=g) * 0.6
+    c2w, cam = scenes.orbit_camera(4.0, 0.7, 90.0, 90.0, W, H, near=0.01, far=100.0, height=0.3)
+    return scenes.Scene(mean, q, log_s, logit, sh, C, [(c2w, cam)], f"small_sh{C}")
+
+
+# ---------------------------------------------------------------- exp, cull
+
+
+def test_exp_spec_bit_exact(orc):
+    bits = np.arange(0, 0x42DC0000, 257, dtype=np.uint32)
+    x = np.concatenate([(bits | np.uint32(0x80000000)).view(np.float32),
+                        bits[bits <= 0x42B20000].view(np.float32),
+                        np.array([np.nan, np.inf, -np.inf, -0.0, 88.9, 89.1, -109.9, -110.1], np.float32)])
+    xt = torch.from_numpy(x).to(DEV)
+    yt = torch.empty_like(xt)
+    L = _lib.load()
+    _lib.check(L.gsmi_exp_spec(xt.numel(), _lib.ptr(xt), _lib.ptr(yt), _lib.stream_of()), "exp_spec")
+    got = yt.cpu().numpy()
+    want = orc.expf(x)
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), (x[~same][:5], got[~same][:5], want[~same][:5])
+
+
+@pytest.mark.parametrize("which", ["c1", "small4"])
+def test_cull_bit_exact(orc, which):
+    sc = scenes.plain_forward_10k() if which == "c1" else small_sh_scene(4)
+    c2w, cam = sc.cameras[0]
+    s, _ = orc.activate(sc.svec_before_activation, sc.alpha_before_activation)
+    normals, pts = cam.get_frustum(c2w)
+    mask = torch.zeros(sc.N, dtype=torch.bool, device=DEV)
+    _backend.culling_gaussian_bsphere(sc.mean.to(DEV), sc.qvec.to(DEV), torch.from_numpy(s).to(DEV),
+                                      normals.to(DEV), pts.to(DEV), mask, 1.0)
+    want = orc.cull(sc.mean, s, *orc.frustum(c2w, ocam(cam)), 1.0)
+    assert np.array_equal(mask.cpu().numpy(), want)
+    assert 0 < want.sum() < sc.N
+
+
+# ------------------------------------------------------- preprocess, binning
+
+
+@pytest.mark.parametrize("which", ["c1", "small1", "small4"])
+def test_fused_preprocess_and_binning_bit_exact(orc, which):
+    sc = {"c1": scenes.plain_forward_10k, "small1": lambda: small_sh_scene(1),
+          "small4": lambda: small_sh_scene(4)}[which]()
+    c2w, cam = sc.cameras[0]
+    ref = orc.pipeline_forward(sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                               sc.sh_coeffs, c2w, ocam(cam), sc.C)
+    b = rasterize.preprocess_and_bin(*dev_params(sc)[:4], c2w, cam)
+    assert (b.V, b.P) == (ref["idx"].size, ref["P"])
+    assert np.array_equal(b.vis.cpu().numpy().astype(bool), ref["mask"])
+    assert np.array_equal(b.orig.cpu().numpy(), ref["idx"])
+    for name, got, want in (("mean2d", b.mean2d, ref["mean2d"]), ("cov2d", b.cov2d, ref["cov2d"]),
+                            ("depth", b.depth, ref["depth"]), ("alpha", b.alpha, ref["a"][ref["idx"]])):
+        g = got.cpu().numpy().reshape(-1).view(np.uint32)
+        w = np.asarray(want, np.float32).reshape(-1).view(np.uint32)
+        assert np.array_equal(g, w), name
+    assert np.array_equal(b.tl.cpu().numpy(), ref["tl"])
+    assert np.array_equal(b.br.cpu().numpy(), ref["br"])
+    assert np.array_equal(b.keys.cpu().numpy().view(np.uint64), ref["sorted_keys"])
+    assert np.array_equal(b.ids.cpu().numpy(), ref["ids"])
+    assert np.array_equal(b.start.cpu().numpy(), ref["start"])
+    assert np.array_equal(b.end.cpu().numpy(), ref["end"])
+
+
+def test_dropin_aabb_and_binning_bit_exact(orc):
+    from gaussian_splatting_3d_amd.culling import tile_culling_aabb_count
+
+    sc = scenes.plain_forward_10k()
+    c2w, cam = sc.cameras[0]
+    ref = orc.pipeline_forward(sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                               sc.sh_coeffs, c2w, ocam(cam), sc.C)
+    mean2d = torch.from_numpy(ref["mean2d"]).to(DEV)
+    cov2d = torch.from_numpy(ref["cov2d"]).reshape(-1, 2, 2).to(DEV)
+    P, tl, br = tile_culling_aabb_count(mean2d, cov2d, 16, cam, 6.0)
+    assert P == ref["P"]
+    assert np.array_equal(tl.cpu().numpy(), ref["tl"]) and np.array_equal(br.cpu().numpy(), ref["br"])
+    nth, ntw = ref["n_tiles_h"], ref["n_tiles_w"]
+    start = -torch.ones(nth * ntw, dtype=torch.int32, device=DEV)
+    end = -torch.ones(nth * ntw, dtype=torch.int32, device=DEV)
+    ids = torch.zeros(P, dtype=torch.int32, device=DEV)
+    depth = torch.from_numpy(ref["depth"]).reshape(-1, 1).to(DEV)
+    _backend.tile_culling_aabb_start_end(tl, br, ids, start, end, depth, nth, ntw)
+    assert np.array_equal(ids.cpu().numpy(), ref["ids"])
+    assert np.array_equal(start.cpu().numpy(), ref["start"])
+    assert np.array_equal(end.cpu().numpy(), ref["end"])
+    with pytest.raises(RuntimeError, match="pair count"):
+        _backend.tile_culling_aabb_start_end(tl, br, torch.zeros(P + 1, dtype=torch.int32, device=DEV), start, end,
+                                             depth, nth, ntw)
+
+
+@pytest.mark.parametrize("P,end_bit", [(0, 40), (1, 40), (4095, 45), (4096, 45), (4097, 40), (100_003, 45),
+                                       (300_000, 64), (70_000, 8)])
+def test_sort_pairs_matches_stable_argsort(P, end_bit):
+    rng = np.random.default_rng(P + end_bit)
+    hi = rng.integers(0, 300, P).astype(np.uint64)
+    lo = rng.integers(0, 2 ** 32, P, dtype=np.uint64)
+    lo[rng.random(P) < 0.3] = 7  # many equal keys
+    keys = (hi << np.uint64(32)) | lo
+    if end_bit < 64:
+        keys &= np.uint64((1 << end_bit) - 1)
+    if end_bit == 64:
+        keys |= np.uint64(1) << np.uint64(63)
+    vals = np.arange(P, dtype=np.int32)
+    kt = torch.from_numpy(keys.view(np.int64).copy()).to(DEV)
+    vt = torch.from_numpy(vals).to(DEV)
+    L = _lib.load()
+    ws = torch.empty(max(int(L.gsmi_sort_workspace_size(P)), 1), dtype=torch.uint8, device=DEV)
+    _lib.check(L.gsmi_sort_pairs(P, _lib.ptr(kt), _lib.ptr(vt), end_bit, _lib.ptr(ws), ws.numel(),
+                                 _lib.stream_of()), "sort")
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(kt.cpu().numpy().view(np.uint64), keys[order])
+    assert np.array_equal(vt.cpu().numpy(), vals[order])
+
+
+# ---------------------------------------------------------------- compositing
+
+
+def _oracle_view(orc, sc):
+    c2w, cam = sc.cameras[0]
+    return orc.pipeline_forward(sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                                sc.sh_coeffs, c2w, ocam(cam), sc.C)
+
+
+@pytest.mark.parametrize("which", ["c1", "small1", "small2", "small3", "small4"])
+def test_forward_exact_and_fast(orc, which):
+    sc = scenes.plain_forward_10k() if which == "c1" else small_sh_scene(int(which[-1]))
+    c2w, cam = sc.cameras[0]
+    ref = _oracle_view(orc, sc)
+    cfg = scenes.RasterConfig()
+    p = dev_params(sc)
+    out_e, st = rasterize.forward_view(*p, c2w, cam, sc.C, cfg, exact=True)
+    got = out_e.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), ref["out"].view(np.uint32)), np.abs(got - ref["out"]).max()
+    assert np.array_equal(st.T.cpu().numpy().view(np.uint32), ref["T"].view(np.uint32))
+    assert np.array_equal(st.n_done.cpu().numpy(), ref["n_done"])
+    out_f, _ = rasterize.forward_view(*p, c2w, cam, sc.C, cfg, exact=False)
+    assert np.abs(out_f.cpu().numpy() - ref["out"]).max() <= IMG_TOL
+
+
+def test_forward_background(orc):
+    sc = small_sh_scene(3)
+    c2w, cam = sc.cameras[0]
+    bg = [0.2, 0.5, 0.9]
+    ref = orc.pipeline_forward(sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                               sc.sh_coeffs, c2w, ocam(cam), sc.C, mode=1, bg=bg)
+    cfg = scenes.RasterConfig(bg=True, bg_rgb=tuple(bg))
+    out, st = rasterize.forward_view(*dev_params(sc), c2w, cam, sc.C, cfg, exact=True)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref["out"].view(np.uint32))
+    assert np.array_equal(st.pre_bg.cpu().numpy(), ref["pre_bg"])
+
+
+def test_forward_deterministic():
+    sc = small_sh_scene(4, n=20000)
+    c2w, cam = sc.cameras[0]
+    p = dev_params(sc)
+    cfg = scenes.RasterConfig()
+    a, _ = rasterize.forward_view(*p, c2w, cam, sc.C, cfg)
+    b, _ = rasterize.forward_view(*p, c2w, cam, sc.C, cfg)
+    assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("which,bg", [("c1", False), ("small2", False), ("small4", False), ("small4", True)])
+def test_backward_boundary_gradients(orc, which, bg):
+    sc = scenes.plain_forward_10k() if which == "c1" else small_sh_scene(int(which[-1]))
+    c2w, cam = sc.cameras[0]
+    bgc = [0.3, 0.6, 0.1] if bg else None
+    ref = orc.pipeline_forward(sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                               sc.sh_coeffs, c2w, ocam(cam), sc.C, mode=1 if bg else 0, bg=bgc)
+    cfg = scenes.RasterConfig(bg=bg, bg_rgb=tuple(bgc) if bg else (1.0, 1.0, 1.0))
+    p = dev_params(sc)
+    out, st = rasterize.forward_view(*p, c2w, cam, sc.C, cfg)
+    gout = torch.rand(out.shape, generator=torch.Generator().manual_seed(5)).to(DEV) - 0.5
+    _, _, _, _, g_sh, bd = rasterize.backward_view(st, *p, gout)
+    rg, mism = orc.render_sh_bwd(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]], ref["start"],
+                                 ref["end"], ref["ids"], ref["out"], gout.cpu().numpy(), ref["topleft"], c2w, 16,
+                                 ref["n_tiles_h"], ref["n_tiles_w"], ref["psx"], ref["psy"], ref["H"], ref["W"],
+                                 sc.C, 1e-4, mode=1 if bg else 0, bg=bgc)
+    assert mism == 0
+    assert rel_err(bd["mean2d"].cpu(), rg["mean"]) <= GRAD_RTOL
+    assert rel_err(bd["cov"].cpu(), rg["cov"]) <= GRAD_RTOL
+    assert rel_err(bd["alpha"].cpu(), rg["alpha"]) <= GRAD_RTOL
+    gsh = g_sh.cpu().numpy()[ref["idx"]][..., : sc.C * sc.C]
+    assert rel_err(gsh, rg["sh"]) <= GRAD_RTOL
+
+
+def test_full_chain_gradients(orc):
+    sc = small_sh_scene(4, n=3000)
+    c2w, cam = sc.cameras[0]
+    ref = _oracle_view(orc, sc)
+    cfg = scenes.RasterConfig()
+    p = dev_params(sc)
+    out, st = rasterize.forward_view(*p, c2w, cam, sc.C, cfg)
+    gout = torch.rand(out.shape, generator=torch.Generator().manual_seed(9)).to(DEV) - 0.5
+    g_mean, g_q, g_ls, g_lg, g_sh, bd = rasterize.backward_view(st, *p, gout)
+    want = orc.projection_adjoint(sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                                  sc.sh_coeffs, c2w, ref["idx"], bd["mean2d"].cpu(), bd["cov"].cpu(),
+                                  np.zeros_like(ref["sh_v"]), bd["alpha"].cpu(), sc.C)
+    for got, w, name in ((g_mean, want[0], "mean"), (g_q, want[1], "qvec"), (g_ls, want[2], "log_s"),
+                         (g_lg, want[3], "logit")):
+        assert rel_err(got.cpu(), w) <= GRAD_RTOL, name
+
+
+def test_render_sh_dropin_autograd(orc):
+    from gaussian_splatting_3d_amd.renderer import render_sh
+
+    sc = small_sh_scene(2)
+    c2w, cam = sc.cameras[0]
+    ref = _oracle_view(orc, sc)
+    mean = torch.from_numpy(ref["mean2d"]).to(DEV).requires_grad_(True)
+    cov = torch.from_numpy(ref["cov2d"]).reshape(-1, 2, 2).to(DEV).requires_grad_(True)
+    sh = torch.from_numpy(ref["sh_v"]).to(DEV).requires_grad_(True)
+    alpha = torch.from_numpy(ref["a"][ref["idx"]]).to(DEV).requires_grad_(True)
+    ic = cam.image_constants()
+    out = render_sh(mean, cov, sh, alpha, torch.from_numpy(ref["start"]).to(DEV),
+                    torch.from_numpy(ref["end"]).to(DEV), torch.from_numpy(ref["ids"]).to(DEV),
+                    torch.tensor(ic["topleft"], device=DEV), c2w.to(DEV), 16, ic["n_tiles_h"], ic["n_tiles_w"],
+                    ic["pixel_size_x"], ic["pixel_size_y"], ic["H"], ic["W"], sc.C, 1e-4)
+    assert out.shape == (ic["H"] * ic["W"] * 3,)
+    assert np.abs(out.detach().cpu().numpy() - ref["out"]).max() <= IMG_TOL
+    gout = torch.rand_like(out)
+    out.backward(gout)
+    rg, _ = orc.render_sh_bwd(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]], ref["start"],
+                              ref["end"], ref["ids"], ref["out"], gout.cpu().numpy(), ref["topleft"], c2w, 16,
+                              ref["n_tiles_h"], ref["n_tiles_w"], ref["psx"], ref["psy"], ref["H"], ref["W"], sc.C,
+                              1e-4)
+    assert rel_err(mean.grad.cpu(), rg["mean"]) <= GRAD_RTOL
+    assert rel_err(cov.grad.cpu(), rg["cov"]) <= GRAD_RTOL
+    assert rel_err(sh.grad.cpu(), rg["sh"]) <= GRAD_RTOL
+    assert rel_err(alpha.grad.cpu(), rg["alpha"]) <= GRAD_RTOL
+
+
+def test_render_start_end_rgb(orc):
+    from gaussian_splatting_3d_amd.renderer import render_start_end
+
+    sc = small_sh_scene(1)
+    c2w, cam = sc.cameras[0]
+    ref = _oracle_view(orc, sc)
+    color = np.random.default_rng(3).random((ref["idx"].size, 3)).astype(np.float32)
+    ic = cam.image_constants()
+    want, _, _ = orc.render_rgb_fwd(ref["mean2d"], ref["cov2d"], color, ref["a"][ref["idx"]], ref["start"],
+                                    ref["end"], ref["ids"], ref["topleft"], 16, ic["n_tiles_h"], ic["n_tiles_w"],
+                                    ref["psx"], ref["psy"], ic["H"], ic["W"], 1e-4)
+    mean = torch.from_numpy(ref["mean2d"]).to(DEV).requires_grad_(True)
+    cov = torch.from_numpy(ref["cov2d"]).reshape(-1, 2, 2).to(DEV).requires_grad_(True)
+    col = torch.from_numpy(color).to(DEV).requires_grad_(True)
+    alpha = torch.from_numpy(ref["a"][ref["idx"]]).to(DEV).requires_grad_(True)
+    out = render_start_end(mean, cov, col, alpha, torch.from_numpy(ref["start"]).to(DEV),
+                           torch.from_numpy(ref["end"]).to(DEV), torch.from_numpy(ref["ids"]).to(DEV),
+                           torch.tensor(ic["topleft"], device=DEV), 16, ic["n_tiles_h"], ic["n_tiles_w"],
+                           ic["pixel_size_x"], ic["pixel_size_y"], ic["H"], ic["W"], 1e-4)
+    assert np.array_equal(out.detach().cpu().numpy().view(np.uint32), want.view(np.uint32))
+    gout = torch.rand_like(out)
+    out.backward(gout)
+    rg, _ = orc.render_rgb_bwd(ref["mean2d"], ref["cov2d"], color, ref["a"][ref["idx"]], ref["start"], ref["end"],
+                               ref["ids"], want, gout.cpu().numpy(), ref["topleft"], 16, ic["n_tiles_h"],
+                               ic["n_tiles_w"], ref["psx"], ref["psy"], ic["H"], ic["W"], 1e-4)
+    assert rel_err(mean.grad.cpu(), rg["mean"]) <= GRAD_RTOL
+    assert rel_err(cov.grad.cpu(), rg["cov"]) <= GRAD_RTOL
+    assert rel_err(col.grad.cpu(), rg["color"]) <= GRAD_RTOL
+    assert rel_err(alpha.grad.cpu(), rg["alpha"]) <= GRAD_RTOL
+
+
+def test_sh_renderer_module_matches_fused_path(orc):
+    from gaussian_splatting_3d_amd.sh_renderer import SHRenderer
+
+    sc = small_sh_scene(4, n=2000)
+    c2w, cam = sc.cameras[0]
+    m = SHRenderer.from_params(*dev_params(sc)[:4], dev_params(sc)[4], {"tile_culling_radius": 6.0})
+    img = m(c2w, cam)
+    ref = _oracle_view(orc, sc)
+    assert img.shape == (cam.h, cam.w, 3)
+    assert np.abs(img.detach().cpu().numpy().reshape(-1) - ref["out"]).max() <= IMG_TOL
+    img.sum().backward()
+    for prm in (m.mean, m.qvec, m.svec_before_activation, m.alpha_before_activation, m.sh_coeffs):
+        assert prm.grad is not None and torch.isfinite(prm.grad).all()
