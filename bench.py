@@ -1,0 +1,262 @@
+"""Benchmark: forward+backward rasterisation throughput on one MI355X.
+
+Workload (default): 3,000,000 SH-degree-3 Gaussians in a 10x10x10 box (20% in
+50 blobs), one 1920x1080 camera, L1 loss against a seeded target.  One step =
+fused preprocess + binning + sort + forward compositing + loss gradient +
+reverse-order backward compositing + projection adjoint, i.e. everything
+between raw Gaussian parameters and their gradients.  Inputs are resident in
+HBM before timing starts.
+
+Prints one JSON line (metric/value/unit per the workload definition) with a
+`roofline` object for the dominant kernel (algorithmic bytes / HIP-event time
+of that kernel on the stream it runs on, against 8 TB/s) and a
+`cpu_baseline` object (the CPU oracle on a bounded sample of the same
+workload, scaled, single thread).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+WORKLOADS = ("dense_3m", "fwd_bwd_500k", "plain_forward_10k")
+
+
+def make_scene(name):
+    from gaussian_splatting_3d_amd import scenes
+
+    return {"dense_3m": scenes.dense_3m, "fwd_bwd_500k": scenes.fwd_bwd_500k,
+            "plain_forward_10k": scenes.plain_forward_10k}[name]()
+
+
+def alg_bytes(P, H, W, C, V, N):
+    """Algorithmic HBM bytes per launch, per stage (FP32)."""
+    K = 3 * C * C
+    return {
+        "preprocess": 44 * N + 36 * V,
+        "composite_fwd": P * (28 + 4 * K) + 20 * H * W,
+        "composite_bwd": P * (28 + 4 * K) + P * 8 * (6 + K) + 32 * H * W,
+        "project_bwd": V * 24 + 44 * V + 44 * N,
+    }
+
+
+class Timed:
+    """HIP events around a region on the current stream; averaged."""
+
+    def __init__(self):
+        self.pairs = []
+
+    def __enter__(self):
+        self.s = torch.cuda.Event(enable_timing=True)
+        self.e = torch.cuda.Event(enable_timing=True)
+        self.s.record()
+        return self
+
+    def __exit__(self, *a):
+        self.e.record()
+        self.pairs.append((self.s, self.e))
+
+    def mean_ms(self):
+        return float(np.mean([s.elapsed_time(e) for s, e in self.pairs])) if self.pairs else float("nan")
+
+
+def gpu_bench(scene, steps, warmup, exact=False):
+    from gaussian_splatting_3d_amd import _lib, rasterize, scenes
+    from gaussian_splatting_3d_amd.backend import _backend
+
+    dev = torch.device("cuda:0")
+    c2w, cam = scene.cameras[0]
+    cfg = scenes.RasterConfig()
+    params = [t.to(dev).contiguous() for t in (scene.mean, scene.qvec, scene.svec_before_activation,
+                                               scene.alpha_before_activation, scene.sh_coeffs)]
+    target = scenes.target_image(cam.h, cam.w, 3).to(dev).reshape(-1)
+    t_fwd, t_bwd = Timed(), Timed()
+    orig_fwd, orig_bwd = _backend.sh_forward, _backend.sh_backward
+    state = {"on": False}
+
+    def fwd_wrap(*a, **k):
+        if not state["on"]:
+            return orig_fwd(*a, **k)
+        with t_fwd:
+            return orig_fwd(*a, **k)
+
+    def bwd_wrap(*a, **k):
+        if not state["on"]:
+            return orig_bwd(*a, **k)
+        with t_bwd:
+            return orig_bwd(*a, **k)
+
+    _backend.sh_forward, _backend.sh_backward = fwd_wrap, bwd_wrap
+    numel = target.numel()
+
+    def step():
+        out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg, exact=exact)
+        grad_out = torch.sign(out - target) / numel  # d/d out of mean |out - target|
+        grads = rasterize.backward_view(st, *params, grad_out)
+        return st, grads
+
+    try:
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        state["on"] = True
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st, grads = step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        state["on"] = False
+        # forward-only throughput (image only, no backward state kept)
+        torch.cuda.synchronize()
+        f0 = time.perf_counter()
+        for _ in range(steps):
+            rasterize.forward_view(*params, c2w, cam, scene.C, cfg, exact=exact)
+        torch.cuda.synchronize()
+        f1 = time.perf_counter()
+    finally:
+        _backend.sh_forward, _backend.sh_backward = orig_fwd, orig_bwd
+    b = st.b
+    for g in grads[:5]:
+        assert torch.isfinite(g).all()
+    return dict(ms_step=(t1 - t0) * 1e3 / steps, ms_fwd_only=(f1 - f0) * 1e3 / steps,
+                ms_composite_fwd=t_fwd.mean_ms(), ms_composite_bwd=t_bwd.mean_ms(), N=b.N, V=b.V, P=b.P,
+                H=cam.h, W=cam.w, C=scene.C)
+
+
+def cpu_baseline(scene, budget_s=20.0):
+    """CPU oracle on a bounded sample of the same workload, single thread:
+    full preprocessing (activation, cull, projection, tile bounds) and
+    binning (key emission + stable sort + ranges), compositing forward and
+    backward on a seeded sample of tiles (scaled by total/sample pair count),
+    projection adjoint on a sample of Gaussians (scaled by V)."""
+    from oracle import oracle as O
+
+    torch.set_num_threads(1)
+    c2w, cam = scene.cameras[0]
+    ocam = O.Camera(cam.fx, cam.fy, cam.cx, cam.cy, cam.w, cam.h, cam.near_plane, cam.far_plane)
+    C = scene.C
+    t0 = time.perf_counter()
+    s, a = O.activate(scene.svec_before_activation, scene.alpha_before_activation)
+    normals, pts = O.frustum(c2w, ocam)
+    mask = O.cull(scene.mean, s, normals, pts, 1.0)
+    idx = np.nonzero(mask)[0]
+    mean = scene.mean.numpy()
+    m2d, cov, depth = O.project(mean[idx], scene.qvec.numpy()[idx], s[idx], c2w)
+    P, tl, br = O.aabb(m2d, cov, ocam, 6.0)
+    ic = O.image_constants(ocam)
+    bn = O.bin_gaussians(tl, br, depth, ic["n_tiles_h"], ic["n_tiles_w"])
+    t_pre = time.perf_counter() - t0
+    # compositing on a tile sample
+    rng = np.random.default_rng(0)
+    n_tiles = ic["n_tiles_h"] * ic["n_tiles_w"]
+    counts = np.where(bn["start"] >= 0, bn["end"] - bn["start"], 0).astype(np.int64)
+    order = rng.permutation(n_tiles)
+    sh_v = np.ascontiguousarray(scene.sh_coeffs.numpy()[idx][..., : C * C])
+    al = a[idx]
+    frac = 0.0
+    t_comp = 0.0
+    sampled_pairs = 0
+    chosen = []
+    for k in range(1, n_tiles + 1):
+        chosen = order[:k]
+        if counts[chosen].sum() >= max(1, 0.004 * P) or k == n_tiles:
+            break
+    st = -np.ones(n_tiles, np.int32)
+    en = -np.ones(n_tiles, np.int32)
+    st[chosen] = bn["start"][chosen]
+    en[chosen] = bn["end"][chosen]
+    sampled_pairs = int(counts[chosen].sum())
+    c0 = time.perf_counter()
+    out, T, nd, _ = O.render_sh_fwd(m2d, cov, sh_v, al, st, en, bn["ids"], ic["topleft"], c2w, 16,
+                                    ic["n_tiles_h"], ic["n_tiles_w"], ic["psx"], ic["psy"], ic["H"], ic["W"], C,
+                                    1e-4)
+    gout = np.sign(out - 0.5).astype(np.float32) / out.size
+    O.render_sh_bwd(m2d, cov, sh_v, al, st, en, bn["ids"], out, gout, ic["topleft"], c2w, 16, ic["n_tiles_h"],
+                    ic["n_tiles_w"], ic["psx"], ic["psy"], ic["H"], ic["W"], C, 1e-4)
+    t_comp = time.perf_counter() - c0
+    frac = sampled_pairs / max(P, 1)
+    # projection adjoint on a Gaussian sample
+    ns = min(len(idx), 20000)
+    sub = idx[:ns]
+    a0 = time.perf_counter()
+    O.projection_adjoint(mean[sub], scene.qvec.numpy()[sub], scene.svec_before_activation.numpy()[sub],
+                         scene.alpha_before_activation.numpy()[sub], scene.sh_coeffs.numpy()[sub], c2w,
+                         np.arange(ns), np.zeros((ns, 2), np.float32), np.zeros((ns, 4), np.float32),
+                         np.zeros((ns, 3, C * C), np.float32), np.zeros(ns, np.float32), C)
+    t_adj = (time.perf_counter() - a0) * len(idx) / max(ns, 1)
+    t_step = t_pre + t_comp / max(frac, 1e-12) + t_adj
+    return dict(value=ic["H"] * ic["W"] / t_step / 1e6, unit="Mpix/s", cores=1, kind="port",
+                sample=(f"CPU oracle, 1 thread: full preprocess+binning of N={scene.N} ({t_pre:.1f}s); "
+                        f"fwd+bwd compositing on {len(chosen)} of {n_tiles} tiles ({sampled_pairs} of {P} pairs, "
+                        f"{t_comp:.1f}s, scaled x{1 / max(frac, 1e-12):.0f}); projection adjoint on {ns} of "
+                        f"{len(idx)} visible Gaussians, scaled; extrapolated step {t_step:.1f}s"),
+                seconds_measured=round(t_pre + t_comp + (time.perf_counter() - a0), 1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="dense_3m", choices=WORKLOADS)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--exact", action="store_true", help="exact colour arithmetic (bit-reproducible images)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        return
+    scene = make_scene(args.workload)
+    print(f"[bench] workload {scene.name}: N={scene.N}", file=sys.stderr, flush=True)
+    r = gpu_bench(scene, args.steps, args.warmup, exact=args.exact)
+    print(f"[bench] gpu done: {r}", file=sys.stderr, flush=True)
+    ab = alg_bytes(r["P"], r["H"], r["W"], r["C"], r["V"], r["N"])
+    dom = "composite_bwd" if r["ms_composite_bwd"] >= r["ms_composite_fwd"] else "composite_fwd"
+    t_dom = r["ms_" + dom] / 1e3
+    achieved = ab[dom] / t_dom / 1e9
+    mpix = r["H"] * r["W"] / (r["ms_step"] / 1e3) / 1e6
+    line = {
+        "metric": "fwd+bwd Mpix/s @3M Gaussians 1080p" if args.workload == "dense_3m" else
+        f"fwd+bwd Mpix/s ({args.workload})",
+        "value": round(mpix, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene and target, random-init Gaussians)",
+        "config": {"workload": scene.name, "gaussians": r["N"], "visible": r["V"], "pairs": r["P"],
+                   "width": r["W"], "height": r["H"], "sh_degree": r["C"] - 1, "views": 1},
+        "fwd_only_mpix_s": round(r["H"] * r["W"] / (r["ms_fwd_only"] / 1e3) / 1e6, 3),
+        "gaussians_per_s_fwd_bwd": round(r["N"] / (r["ms_step"] / 1e3), 1),
+        "kernels_ms": {"composite_fwd": round(r["ms_composite_fwd"], 4),
+                       "composite_bwd": round(r["ms_composite_bwd"], 4)},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
+                     "alg_bytes_per_launch": int(ab[dom])},
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(scene)
+        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
