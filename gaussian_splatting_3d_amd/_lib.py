@@ -33,14 +33,15 @@ SIGNATURES = {
     "gsmi_tile_culling_aabb_count": (_I32, [_I64, _P, _P, _F, _F, _F, _F, _I32, _I32, _F, _I32, _P, _P, _P, _P,
                                             _SZ, _P]),
     "gsmi_binning_workspace_size": (_SZ, [_I64, _I64, _I32]),
-    "gsmi_tile_culling_aabb_start_end": (_I32, [_I64, _I64, _P, _P, _P, _I32, _I32, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gsmi_tile_culling_aabb_start_end": (_I32, [_I64, _I64, _P, _P, _P, _I32, _I32, _P, _P, _P, _P, _P, _SZ, _I32,
+                                                _P]),
     "gsmi_sort_workspace_size": (_SZ, [_I64]),
     "gsmi_sort_pairs": (_I32, [_I64, _P, _P, _I32, _P, _SZ, _P]),
     "gsmi_preprocess_workspace_size": (_SZ, [_I64]),
     "gsmi_preprocess_count": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _F, _F, _F, _F, _I32, _I32, _F, _F, _I32, _I32,
                                      _P, _P, _P, _P, _P, _SZ, _P]),
     "gsmi_preprocess_emit": (_I32, [_I64, _P, _P, _P, _P, _P, _F, _F, _F, _F, _I32, _I32, _F, _I32, _P, _P, _P,
-                                    _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
+                                    _P, _P, _P, _P, _P, _P, _P]),
     "gsmi_tile_ranges": (_I32, [_I64, _P, _I32, _P, _P, _P]),
     "gsmi_tile_based_vol_rendering_sh": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P,
                                                 _P, _I32, _I32, _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P, _P, _P,

@@ -82,7 +82,7 @@ class _Backend:
         ws = _workspace(L.gsmi_binning_workspace_size(V, P, n_tiles), depth.device)
         check(L.gsmi_tile_culling_aabb_start_end(V, P, ptr(aabb_topleft), ptr(aabb_bottomright), ptr(depth),
                                                  int(n_tiles_h), int(n_tiles_w), ptr(gaussian_ids), ptr(start),
-                                                 ptr(end), ptr(sorted_keys), ptr(ws), ws.numel(),
+                                                 ptr(end), ptr(sorted_keys), ptr(ws), ws.numel(), 1,
                                                  stream_of(depth.device)),
               "tile_culling_aabb_start_end")
 

@@ -175,17 +175,15 @@ __global__ __launch_bounds__(256) void k_pre_count_planes(int64_t N, const float
   cnt[i] = Int2{keep ? 1 : 0, pairs};
 }
 
-// Pass 2: compacted per-Gaussian records and key emission (index order).
+// Pass 2: compacted per-Gaussian records.
 __global__ __launch_bounds__(256) void k_pre_emit(int64_t N, const float* __restrict__ mean,
                                                   const float* __restrict__ qvec, const float* __restrict__ log_s,
                                                   const float* __restrict__ logit, CamArgs c,
                                                   const uint8_t* __restrict__ vis, const int32_t* __restrict__ voff,
-                                                  const int32_t* __restrict__ poff, float* __restrict__ mean2d,
-                                                  float* __restrict__ cov2d, float* __restrict__ depth,
-                                                  float* __restrict__ alpha, int32_t* __restrict__ tl,
-                                                  int32_t* __restrict__ br, int32_t* __restrict__ orig,
-                                                  uint64_t* __restrict__ keys, int32_t* __restrict__ values,
-                                                  int64_t P) {
+                                                  float* __restrict__ mean2d, float* __restrict__ cov2d,
+                                                  float* __restrict__ depth, float* __restrict__ alpha,
+                                                  int32_t* __restrict__ tl, int32_t* __restrict__ br,
+                                                  int32_t* __restrict__ orig) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N || !vis[i]) return;
   const int64_t v = voff[i];
@@ -203,17 +201,6 @@ __global__ __launch_bounds__(256) void k_pre_emit(int64_t N, const float* __rest
   reinterpret_cast<int2*>(tl)[v] = make_int2(lx, ly);
   reinterpret_cast<int2*>(br)[v] = make_int2(hx, hy);
   orig[v] = (int32_t)i;
-  int64_t off = poff[i];
-  const uint32_t dbits = __float_as_uint(p.depth);
-  for (int x = lx; x <= hx; ++x) {
-    for (int y = ly; y <= hy; ++y) {
-      if (off >= P) return;
-      const uint64_t tile = (uint64_t)(uint32_t)(y * c.n_tiles_w + x);
-      keys[off] = (tile << 32) | (uint64_t)dbits;
-      values[off] = (int32_t)v;
-      ++off;
-    }
-  }
 }
 
 __global__ void k_split_int2(const Int2* __restrict__ in, int64_t n, int32_t* __restrict__ a,
@@ -379,21 +366,18 @@ extern "C" int gsmi_preprocess_count(int64_t N, const float* mean, const float* 
 extern "C" int gsmi_preprocess_emit(int64_t N, const float* mean, const float* qvec, const float* log_s,
                                     const float* logit, const float* c2w_host, float fx, float fy, float cx,
                                     float cy, int W, int H, float D, int tile_size, const uint8_t* vis,
-                                    const int32_t* vis_off, const int32_t* pair_off, float* mean2d, float* cov2d,
-                                    float* depth, float* alpha, int32_t* tl, int32_t* br, int32_t* orig,
-                                    uint64_t* keys, int32_t* values, int64_t P, void* stream) {
+                                    const int32_t* vis_off, float* mean2d, float* cov2d, float* depth, float* alpha,
+                                    int32_t* tl, int32_t* br, int32_t* orig, void* stream) {
   if (N < 0 || W <= 0 || H <= 0 || !c2w_host) return GSMI_ERR_ARG;
   if (tile_size != kTile) return GSMI_ERR_UNSUPPORTED;
   if (N == 0) return GSMI_OK;
-  if (!mean || !qvec || !log_s || !logit || !vis || !vis_off || !pair_off) return GSMI_ERR_ARG;
+  if (!mean || !qvec || !log_s || !logit || !vis || !vis_off) return GSMI_ERR_ARG;
   if (((uintptr_t)qvec & 15) != 0 || ((uintptr_t)cov2d & 15) != 0 || ((uintptr_t)mean2d & 7) != 0 ||
       ((uintptr_t)tl & 7) != 0 || ((uintptr_t)br & 7) != 0)
     return GSMI_ERR_ARG;
-  if (P < 0) return GSMI_ERR_ARG;
   const CamArgs c = make_cam(c2w_host, fx, fy, cx, cy, W, H, D, 0.0f, 0, tile_size);
   hipLaunchKernelGGL(k_pre_emit, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, (hipStream_t)stream, N, mean,
-                     qvec, log_s, logit, c, vis, vis_off, pair_off, mean2d, cov2d, depth, alpha, tl, br, orig, keys,
-                     values, P);
+                     qvec, log_s, logit, c, vis, vis_off, mean2d, cov2d, depth, alpha, tl, br, orig);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
 }

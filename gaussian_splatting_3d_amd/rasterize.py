@@ -7,9 +7,11 @@ Forward (all on the current torch stream):
   2. one host read of (V, P) to size the outputs (the reference reads P the
      same way, gs/culling.py:33-35);
   3. gsmi_preprocess_emit   — compacted SoA records (mean2d, cov2d, depth,
-     alpha, tile bounds, original index) and the (tile<<32|depth) keys;
-  4. gsmi_sort_pairs        — stable LSD radix sort on 32 + ceil(log2 T) bits;
-  5. gsmi_tile_ranges       — start/end per tile;
+     alpha, tile bounds, original index);
+  4-5. gsmi_tile_culling_aabb_start_end — depth-first binning: stable sort of
+     the visible Gaussians by depth, pair expansion in that order, stable sort
+     of the pairs by tile, start/end per tile (the order of the reference's
+     (tile<<32 | depth) key sort);
   6. gsmi_tile_based_vol_rendering_sh — compositing; SH rows are read from the
      raw [N,3,C_max^2] tensor through the original index (no gather copy).
 Backward:
@@ -59,7 +61,7 @@ class Binning:
     tl: torch.Tensor
     br: torch.Tensor
     orig: torch.Tensor
-    keys: torch.Tensor  # int64 storage of the uint64 keys, sorted
+    keys: torch.Tensor | None  # int64 storage of the sorted uint64 keys (when asked for)
     ids: torch.Tensor
     start: torch.Tensor
     end: torch.Tensor
@@ -68,7 +70,7 @@ class Binning:
 
 
 def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, tile_size=TILE, D=6.0,
-                       frustum_culling_radius=1.0, skip_frustum_culling=False) -> Binning:
+                       frustum_culling_radius=1.0, skip_frustum_culling=False, want_keys=False) -> Binning:
     """Steps 1-5 of the module docstring."""
     L = _lib.load()
     dev = mean.device
@@ -101,19 +103,18 @@ def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, til
     tl = torch.empty(V, 2, dtype=torch.int32, device=dev)
     br = torch.empty(V, 2, dtype=torch.int32, device=dev)
     orig = torch.empty(V, dtype=torch.int32, device=dev)
-    keys = torch.empty(P, dtype=torch.int64, device=dev)
     ids = torch.empty(P, dtype=torch.int32, device=dev)
     check(L.gsmi_preprocess_emit(N, ptr(mean), ptr(qvec), ptr(log_s), ptr(logit), c2w_c, fx, fy, cx, cy, ic["W"],
-                                 ic["H"], float(D), tile_size, ptr(vis), ptr(vis_off), ptr(pair_off), ptr(mean2d),
-                                 ptr(cov2d), ptr(depth), ptr(alpha), ptr(tl), ptr(br), ptr(orig), ptr(keys),
-                                 ptr(ids), P, st), "preprocess_emit")
+                                 ic["H"], float(D), tile_size, ptr(vis), ptr(vis_off), ptr(mean2d), ptr(cov2d),
+                                 ptr(depth), ptr(alpha), ptr(tl), ptr(br), ptr(orig), st), "preprocess_emit")
     n_tiles = ic["n_tiles_h"] * ic["n_tiles_w"]
-    sws = torch.empty(max(int(L.gsmi_sort_workspace_size(P)), 1), dtype=torch.uint8, device=dev)
-    check(L.gsmi_sort_pairs(P, ptr(keys), ptr(ids), 32 + _bits_for(n_tiles), ptr(sws), sws.numel(), st),
-          "sort_pairs")
     start = torch.empty(n_tiles, dtype=torch.int32, device=dev)
     end = torch.empty(n_tiles, dtype=torch.int32, device=dev)
-    check(L.gsmi_tile_ranges(P, ptr(keys), n_tiles, ptr(start), ptr(end), st), "tile_ranges")
+    keys = torch.empty(P, dtype=torch.int64, device=dev) if want_keys else None
+    bws = torch.empty(max(int(L.gsmi_binning_workspace_size(V, P, n_tiles)), 1), dtype=torch.uint8, device=dev)
+    check(L.gsmi_tile_culling_aabb_start_end(V, P, ptr(tl), ptr(br), ptr(depth), ic["n_tiles_h"], ic["n_tiles_w"],
+                                             ptr(ids), ptr(start), ptr(end), ptr(keys), ptr(bws), bws.numel(), 0,
+                                             st), "binning")
     return Binning(N, V, P, vis, vis_off, pair_off, mean2d, cov2d, depth, alpha, tl, br, orig, keys, ids, start,
                    end, c2w_h, ic)
 

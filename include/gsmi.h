@@ -77,17 +77,19 @@ size_t gsmi_binning_workspace_size(int64_t V, int64_t P, int n_tiles);
 /*
  * Replaces tile_culling_aabb_start_end (gs/src/render.cu:380-397,
  * gs/src/include/aabb_culling.h:15-41, 70-103, 192-260; bound at
- * bindings.cpp:29).  Emits one (key, value) pair per covered tile,
- * key = (int64)(y*n_tiles_w + x) << 32 | float_bits(depth[v]), value = v,
- * in index order; sorts by key (stable LSD radix sort over the key bits that
- * can be non-zero); fills start/end [n_tiles] (-1 for empty tiles) and
- * gaussian_ids [P] (the sorted values).  sorted_keys [P] is optional (NULL to
- * skip).  P must equal the pair count of tl/br, else GSMI_ERR_COUNT.
+ * bindings.cpp:29).  Produces the order of a stable sort of the keys
+ * (int64)(y*n_tiles_w + x) << 32 | float_bits(depth[v]) emitted per covered
+ * tile in index order: gaussian_ids [P] (sorted values v), start/end
+ * [n_tiles] (-1 for empty tiles) and, if sorted_keys != NULL, the sorted
+ * 64-bit keys [P].  Computed depth-first: stable sort of the V Gaussians by
+ * depth bits, pair expansion in that order, stable sort of the pairs by
+ * tile.  flags bit 0: check that P equals the pair count of tl/br before any
+ * pair is written (synchronises the stream; GSMI_ERR_COUNT on mismatch).
  */
 int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int32_t *tl, const int32_t *br,
                                      const float *depth, int n_tiles_h, int n_tiles_w, int32_t *gaussian_ids,
                                      int32_t *start, int32_t *end, uint64_t *sorted_keys, void *workspace,
-                                     size_t workspace_bytes, void *stream);
+                                     size_t workspace_bytes, int flags, void *stream);
 
 /* Workspace bytes for gsmi_sort_pairs (radix sort of P (uint64, int32) pairs). */
 size_t gsmi_sort_workspace_size(int64_t P);
@@ -133,15 +135,13 @@ int gsmi_preprocess_count(int64_t N, const float *mean, const float *qvec, const
 /*
  * Phase 2: for every visible i with v = vis_off[i], writes the compacted
  * mean2d[v] (2), cov2d[v] (4), depth[v], alpha[v] (sigmoid of logit),
- * tl[v], br[v] (2 each) and orig[v] = i, and emits its tile pairs at
- * pair_off[i] into keys/values (index order), never at or beyond P (the
- * pair total of phase 1).
+ * tl[v], br[v] (2 each) and orig[v] = i.  Binning then runs on (tl, br,
+ * depth) with gsmi_tile_culling_aabb_start_end.
  */
 int gsmi_preprocess_emit(int64_t N, const float *mean, const float *qvec, const float *log_s, const float *logit,
                          const float *c2w_host, float fx, float fy, float cx, float cy, int W, int H, float D,
-                         int tile_size, const uint8_t *vis, const int32_t *vis_off, const int32_t *pair_off,
-                         float *mean2d, float *cov2d, float *depth, float *alpha, int32_t *tl, int32_t *br,
-                         int32_t *orig, uint64_t *keys, int32_t *values, int64_t P, void *stream);
+                         int tile_size, const uint8_t *vis, const int32_t *vis_off, float *mean2d, float *cov2d,
+                         float *depth, float *alpha, int32_t *tl, int32_t *br, int32_t *orig, void *stream);
 
 /*
  * Tile ranges from sorted keys: start/end [n_tiles], -1 when empty

@@ -93,7 +93,7 @@ def test_fused_preprocess_and_binning_bit_exact(orc, which):
     c2w, cam = sc.cameras[0]
     ref = orc.pipeline_forward(sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
                                sc.sh_coeffs, c2w, ocam(cam), sc.C)
-    b = rasterize.preprocess_and_bin(*dev_params(sc)[:4], c2w, cam)
+    b = rasterize.preprocess_and_bin(*dev_params(sc)[:4], c2w, cam, want_keys=True)
     assert (b.V, b.P) == (ref["idx"].size, ref["P"])
     assert np.array_equal(b.vis.cpu().numpy().astype(bool), ref["mask"])
     assert np.array_equal(b.orig.cpu().numpy(), ref["idx"])
@@ -127,7 +127,9 @@ def test_dropin_aabb_and_binning_bit_exact(orc):
     end = -torch.ones(nth * ntw, dtype=torch.int32, device=DEV)
     ids = torch.zeros(P, dtype=torch.int32, device=DEV)
     depth = torch.from_numpy(ref["depth"]).reshape(-1, 1).to(DEV)
-    _backend.tile_culling_aabb_start_end(tl, br, ids, start, end, depth, nth, ntw)
+    keys = torch.empty(P, dtype=torch.int64, device=DEV)
+    _backend.tile_culling_aabb_start_end(tl, br, ids, start, end, depth, nth, ntw, sorted_keys=keys)
+    assert np.array_equal(keys.cpu().numpy().view(np.uint64), ref["sorted_keys"])
     assert np.array_equal(ids.cpu().numpy(), ref["ids"])
     assert np.array_equal(start.cpu().numpy(), ref["start"])
     assert np.array_equal(end.cpu().numpy(), ref["end"])
@@ -195,6 +197,31 @@ def test_forward_background(orc):
     out, st = rasterize.forward_view(*dev_params(sc), c2w, cam, sc.C, cfg, exact=True)
     assert np.array_equal(out.cpu().numpy().view(np.uint32), ref["out"].view(np.uint32))
     assert np.array_equal(st.pre_bg.cpu().numpy(), ref["pre_bg"])
+
+
+def test_binning_ties_and_ragged_counts(orc):
+    """Equal depths (ties broken by index), one-tile and image-wide boxes,
+    zero Gaussians, through the drop-in binning entry point."""
+    rng = np.random.default_rng(4)
+    nth, ntw = 7, 9
+    for V in (0, 1, 5, 3000):
+        lo = rng.integers(0, [ntw, nth], size=(V, 2)).astype(np.int32)
+        hi = np.minimum(lo + rng.integers(0, 4, size=(V, 2)), [ntw - 1, nth - 1]).astype(np.int32)
+        if V > 2:
+            lo[0], hi[0] = (0, 0), (ntw - 1, nth - 1)
+        depth = rng.choice(np.array([0.5, 1.0, 2.0, -1.0, 3.25], np.float32), size=V).astype(np.float32)
+        want = orc.bin_gaussians(lo, hi, depth, nth, ntw)
+        P = int(want["ids"].size)
+        ids = torch.zeros(P, dtype=torch.int32, device=DEV)
+        start = torch.empty(nth * ntw, dtype=torch.int32, device=DEV)
+        end = torch.empty(nth * ntw, dtype=torch.int32, device=DEV)
+        keys = torch.empty(P, dtype=torch.int64, device=DEV)
+        _backend.tile_culling_aabb_start_end(torch.from_numpy(lo).to(DEV), torch.from_numpy(hi).to(DEV), ids, start,
+                                             end, torch.from_numpy(depth).reshape(-1, 1).to(DEV), nth, ntw,
+                                             sorted_keys=keys)
+        assert np.array_equal(ids.cpu().numpy(), want["ids"])
+        assert np.array_equal(start.cpu().numpy(), want["start"]) and np.array_equal(end.cpu().numpy(), want["end"])
+        assert np.array_equal(keys.cpu().numpy().view(np.uint64), want["sorted_keys"])
 
 
 def test_forward_deterministic():
