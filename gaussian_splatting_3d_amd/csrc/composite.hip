@@ -76,6 +76,21 @@ __device__ __forceinline__ float gauss_exact(float q, float det) {
   return exp_spec(-0.5f * radial);
 }
 
+// Gaussian value for the backward pass: hardware exponential of
+// -q/(2 det), except when a*G lies within a relative 2^-12 of the 1/255
+// skip threshold (or is not finite), where the exact value decides, so the
+// skip decision always equals the forward's.  Values feed gradients only.
+__device__ __forceinline__ float gauss_decided(float q, float det, float inv_det, float al) {
+  float r = q * inv_det;
+  if (r < 0.0f) r = 1000.0f;
+  const float G = __builtin_amdgcn_exp2f(r * -0.72134752044448170f);  // exp(-r/2)
+  const float aG = al * G;
+  const float lo = kMinRenderAlpha * (1.0f - 0x1p-12f), hi = kMinRenderAlpha * (1.0f + 0x1p-12f);
+  if (aG > lo && aG < hi) return gauss_exact(q, det);
+  if (!(aG == aG) || q == 0.0f) return gauss_exact(q, det);
+  return G;
+}
+
 // The reference's kernel_gaussian_2d (double) from float inputs.
 __device__ __forceinline__ float gauss_exact_d(float dx, float dy, float c0, float c1, float c2, float c3) {
   const double C0 = c0, C1 = c1, C2 = c2, C3 = c3;
@@ -500,6 +515,204 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Backward for SH orders 3 and 4 with the SH-gradient reduction on the matrix
+// cores.  Per wave and per chunk of 8 list entries, the per-pixel colour
+// coefficients cc[ch](p) = w y (1-y) g_ch are staged in LDS as a 12 x 64
+// matrix (rows = (channel, entry) for 4 entries, columns = the wave's 64
+// pixels) and
+//   grad_sh[e][ch][k] += sum_p cc[ch,e](p) * Y_k(p)
+// is one 12x64 by 64x16 product: one M-tile (rows 12..15 zero) x 16 K-steps
+// of v_mfma_f32_16x16x4_f32 (the Y operand is constant per wave and lives in
+// 16 VGPRs).  The six scalar gradient components (mean, cov, alpha) are reduced
+// with an 8-wide wave reduce-scatter.  Operand layout of the 16x16x4 MFMA:
+// A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane
+// j + 16 (i / 4), accumulator register i % 4.
+// ---------------------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaBatch = 64;   // list entries staged per LDS batch
+constexpr int kChunk = 4;        // entries per MFMA reduction
+constexpr int kCcStride = 68;    // padded LDS row (64 pixels + 4)
+constexpr int kCcRows = 3 * kChunk;
+
+template <int C, bool BG>
+__global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
+  constexpr int CC = C * C;
+  constexpr int SHN = 3 * CC;
+  constexpr int NCOMP = 6 + SHN;
+  static_assert(CC <= 16, "SH basis wider than the MFMA tile");
+  __shared__ GeoA s_ga[kMfmaBatch];
+  __shared__ GeoB s_gb[kMfmaBatch];
+  __shared__ float2 s_gc[kMfmaBatch];
+  __shared__ int s_id[kMfmaBatch];
+  __shared__ float s_sh[kMfmaBatch * SHN];
+  __shared__ float s_acc[kMfmaBatch * NCOMP];
+  __shared__ float s_cc[4][kCcRows * kCcStride];
+  __shared__ int s_tmp[4];
+
+  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
+  const int tile = xcd_tile(blockIdx.x, n_tiles);
+  if (tile >= n_tiles) return;
+  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
+  const int t = threadIdx.x;
+  const int wv = t >> 6;
+  const int lane = lane_id();
+  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
+  const bool inside = gx < a.W && gy < a.H;
+  const int64_t pix = (int64_t)gy * a.W + gx;
+  const int s0 = a.start[tile];
+  const int e0 = a.end[tile];
+  if (s0 == -1 || e0 - s0 == 0) return;
+
+  const float px = a.topleft[0] + (float)gx * a.psx;
+  const float py = a.topleft[1] + (float)gy * a.psy;
+  float cw[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) cw[k] = a.c2w[k];
+  float dir[3];
+  pixel_direction(cw, px, py, dir);
+  float Y[CC];
+  sh_basis<C>(dir[0], dir[1], dir[2], Y);
+
+  // B operand: bY[s] = Y_{lane%16}(pixel 4s + lane/16 of this wave), via an
+  // LDS transpose through this wave's staging area, 8 basis columns at a
+  // time (64 x 8 floats fit in the area).
+  static_assert(kCcRows * kCcStride >= 64 * 8, "transpose does not fit the staging area");
+  float* cc_w = s_cc[wv];
+  float bY[16];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cc_w[lane * 8 + k] = (8 * h + k) < CC ? Y[(8 * h + k) < CC ? 8 * h + k : 0] : 0.0f;
+    __builtin_amdgcn_wave_barrier();
+    const int col = (lane & 15) - 8 * h;
+    if (col >= 0 && col < 8) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) bY[s] = cc_w[(4 * s + (lane >> 4)) * 8 + col];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
+  int nd = 0;
+  if (inside) {
+    T = a.T_saved[pix];
+    nd = a.n_done_saved[pix];
+    g0 = a.grad_out[3 * pix + 0];
+    g1 = a.grad_out[3 * pix + 1];
+    g2 = a.grad_out[3 * pix + 2];
+    if (BG) {
+      S0 = a.out_saved[3 * pix + 0] - a.pre_bg_saved[3 * pix + 0];
+      S1 = a.out_saved[3 * pix + 1] - a.pre_bg_saved[3 * pix + 1];
+      S2 = a.out_saved[3 * pix + 2] - a.pre_bg_saved[3 * pix + 2];
+    }
+  }
+  const int nmax = block_max(nd, s_tmp);
+  const int my_comp = lane >> 3;                      // scalar reduce-scatter (8 wide)
+  const bool writer = (lane & 7) == 0 && my_comp < 6;
+  const int arow = lane & 15, acol = lane >> 4;       // MFMA A-operand coordinates
+
+  for (int base = ((nmax - 1) / kMfmaBatch) * kMfmaBatch; nmax > 0 && base >= 0; base -= kMfmaBatch) {
+    const int cnt = min(kMfmaBatch, nmax - base);
+    __syncthreads();
+    for (int e = t; e < cnt * NCOMP; e += kTilePixels) s_acc[e] = 0.0f;
+    stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    __syncthreads();
+    int chunk_top = cnt - 1;  // list position (within the batch) of chunk row 0
+    bool chunk_any = false;
+    for (int j = cnt - 1; j >= 0; --j) {
+      const int e = chunk_top - j;  // row of this entry within the chunk
+      const int idx = base + j;
+      float cc0 = 0.0f, cc1 = 0.0f, cc2 = 0.0f;
+      float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      bool contrib = false;
+      if (__any(idx < nd) && idx < nd) {
+        const GeoA ga = s_ga[j];
+        const GeoB gb = s_gb[j];
+        const float2 gc = s_gc[j];
+        const float dx = px - ga.mx, dy = py - ga.my;
+        const float tmpx = dx * gb.c3 - dy * gb.c2;
+        const float tmpy = -dx * ga.c1 + dy * ga.c0;
+        const float q = tmpx * dx + tmpy * dy;
+        if (!(q * gb.inv_det > gc.y)) {
+          const float al = gc.x;
+          const float G = gauss_decided(q, gb.det, gb.inv_det, al);
+          const float aG = al * G;
+          if (!(aG < kMinRenderAlpha)) {
+            contrib = true;
+            const float om = 1.0f - aG;
+            const float inv_om = __builtin_amdgcn_rcpf(om);
+            const float Tj = T * inv_om;
+            float w = al * Tj * G;
+            if (isnan(w)) w = 0.0f;
+            const float* shj = s_sh + j * SHN;
+            float y0 = sigmoid_fast(dot_fast<CC>(shj, Y));
+            float y1 = sigmoid_fast(dot_fast<CC>(shj + CC, Y));
+            float y2 = sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y));
+            if (isnan(y0 * w)) y0 = 0.0f;
+            if (isnan(y1 * w)) y1 = 0.0f;
+            if (isnan(y2 * w)) y2 = 0.0f;
+            cc0 = w * (y0 * (1.0f - y0)) * g0;
+            cc1 = w * (y1 * (1.0f - y1)) * g1;
+            cc2 = w * (y2 * (1.0f - y2)) * g2;
+            float P = g0 * (y0 * Tj - S0 * inv_om);
+            P = fmaf(g1, y1 * Tj - S1 * inv_om, P);
+            P = fmaf(g2, y2 * Tj - S2 * inv_om, P);
+            S0 = fmaf(w, y0, S0);
+            S1 = fmaf(w, y1, S1);
+            S2 = fmaf(w, y2, S2);
+            const float D = P * aG;
+            const float s = tmpx * gb.inv_det;
+            const float u = tmpy * gb.inv_det;
+            v[0] = D * s;
+            v[1] = D * u;
+            v[2] = 0.5f * v[0] * s;
+            v[3] = 0.5f * v[0] * u;
+            v[4] = 0.5f * v[1] * u;
+            v[5] = P * G;
+            T = Tj;
+          }
+        }
+      }
+      const bool any = __any(contrib);
+      chunk_any = chunk_any || any;
+      cc_w[(0 * kChunk + e) * kCcStride + lane] = cc0;
+      cc_w[(1 * kChunk + e) * kCcStride + lane] = cc1;
+      cc_w[(2 * kChunk + e) * kCcStride + lane] = cc2;
+      if (any) {
+        const float r = wave_reduce_scatter<8>(v);
+        if (writer && r != 0.0f) atomicAdd(s_acc + j * NCOMP + my_comp, r);
+      }
+      if (e == kChunk - 1 || j == 0) {
+        if (chunk_any) {
+          __builtin_amdgcn_wave_barrier();
+          floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int s = 0; s < 16; ++s) {
+            const float a0 = arow < kCcRows ? cc_w[arow * kCcStride + 4 * s + acol] : 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bY[s], acc, 0, 0, 0);
+          }
+          const int k = lane & 15;
+          if (k < CC) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = 4 * (lane >> 4) + r;  // channel row / kChunk, entry row % kChunk
+              const int ee = row & (kChunk - 1);
+              if (row < kCcRows && ee <= e && acc[r] != 0.0f)
+                atomicAdd(s_acc + (chunk_top - ee) * NCOMP + 6 + (row / kChunk) * CC + k, acc[r]);
+            }
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+        chunk_top = j - 1;
+        chunk_any = false;
+      }
+    }
+    __syncthreads();
+    flush_grads<SHN>(a, s_id, s_acc, cnt);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // RGB compositing (render_start_end): vol_render.h:716-798 forward,
 // vol_render.h:800-905 backward; the Gaussian is evaluated in double
 // (kernels.h:195-224).  Colour [V,3] is staged through the SH slots with
@@ -717,7 +930,10 @@ static int launch_fwd_sh(const RasterArgs& a, int exact, hipStream_t st) {
 template <int C, bool BG>
 static int launch_bwd_sh(const RasterArgs& a, hipStream_t st) {
   const int grid = xcd_grid(a.n_tiles_h * a.n_tiles_w);
-  hipLaunchKernelGGL((k_bwd_sh<C, BG>), dim3(grid), dim3(256), 0, st, a);
+  if constexpr (C >= 3)
+    hipLaunchKernelGGL((k_bwd_sh_mfma<C, BG>), dim3(grid), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_bwd_sh<C, BG>), dim3(grid), dim3(256), 0, st, a);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
 }

@@ -234,7 +234,8 @@ def test_forward_deterministic():
     assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("which,bg", [("c1", False), ("small2", False), ("small4", False), ("small4", True)])
+@pytest.mark.parametrize("which,bg", [("c1", False), ("small2", False), ("small3", False), ("small4", False),
+                                      ("small4", True), ("small3", True)])
 def test_backward_boundary_gradients(orc, which, bg):
     sc = scenes.plain_forward_10k() if which == "c1" else small_sh_scene(int(which[-1]))
     c2w, cam = sc.cameras[0]

@@ -1,0 +1,224 @@
+"""Pins the oracle's compositing (gs_oracle.c, vol_render_sh.h:97-455) with
+closed forms, special cases, an independent float64 brute-force compositor
+and float64 central differences.  CPU only."""
+import numpy as np
+import pytest
+
+K0 = 0.28209479177387814  # Y_00
+
+
+def one_tile_setup(means, covs, alphas, sh, C=1, H=16, W=16, depth=None):
+    """All Gaussians in one 16x16 tile, list in the given order."""
+    V = len(means)
+    m = np.asarray(means, np.float32).reshape(V, 2)
+    c = np.asarray(covs, np.float32).reshape(V, 4)
+    a = np.asarray(alphas, np.float32).reshape(V)
+    s = np.asarray(sh, np.float32).reshape(V, 3, C * C)
+    start = np.array([0], np.int32)
+    end = np.array([V], np.int32)
+    ids = np.arange(V, dtype=np.int32)
+    topleft = np.array([-0.5, -0.5], np.float32)
+    c2w = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
+    return dict(mean=m, cov=c, sh=s, alpha=a, start=start, end=end, ids=ids, topleft=topleft, c2w=c2w,
+                psx=np.float32(1 / 16), psy=np.float32(1 / 16), H=H, W=W, C=C)
+
+
+def fwd(orc, d, thresh=1e-4, **kw):
+    return orc.render_sh_fwd(d["mean"], d["cov"], d["sh"], d["alpha"], d["start"], d["end"], d["ids"],
+                             d["topleft"], d["c2w"], 16, 1, 1, d["psx"], d["psy"], d["H"], d["W"], d["C"], thresh, **kw)
+
+
+def pix_pos(d, gx, gy):
+    return (np.float32(d["topleft"][0] + np.float32(gx) * d["psx"]),
+            np.float32(d["topleft"][1] + np.float32(gy) * d["psy"]))
+
+
+def sig(x):
+    return 1 / (1 + np.exp(-x))
+
+
+def test_single_gaussian_at_pixel_centre(orc):
+    """C=1: out = a * sigmoid(Y00 * c) at the Gaussian's own pixel."""
+    d = one_tile_setup([[0, 0]], [[0.01, 0, 0, 0.01]], [0.7], [[[0.5], [-1.0], [2.0]]])
+    px, py = pix_pos(d, 5, 9)
+    d["mean"][0] = (px, py)
+    out, T, nd, _ = fwd(orc, d)
+    o = out.reshape(16, 16, 3)[9, 5]
+    want = 0.7 * sig(K0 * np.array([0.5, -1.0, 2.0]))
+    assert np.allclose(o, want, rtol=1e-6, atol=1e-7)
+    assert abs(T.reshape(16, 16)[9, 5] - 0.3) < 1e-6
+
+
+def test_two_stacked_gaussians(orc):
+    """out = a1 y1 + (1 - a1) a2 y2 (front-to-back, list order)."""
+    d = one_tile_setup([[0, 0], [0, 0]], [[0.01, 0, 0, 0.01]] * 2, [0.6, 0.5],
+                       [[[1.0], [0.0], [-1.0]], [[-2.0], [3.0], [0.5]]])
+    px, py = pix_pos(d, 8, 8)
+    d["mean"][:] = (px, py)
+    out, _, _, _ = fwd(orc, d)
+    y1 = sig(K0 * np.array([1.0, 0.0, -1.0]))
+    y2 = sig(K0 * np.array([-2.0, 3.0, 0.5]))
+    assert np.allclose(out.reshape(16, 16, 3)[8, 8], 0.6 * y1 + 0.4 * 0.5 * y2, rtol=1e-6)
+
+
+def test_transmittance_break_is_tested_before_each_gaussian(orc):
+    """thresh 0.3: after two a=0.5 Gaussians T = 0.25 < 0.3, so the second is
+    still composited and the third is not (vol_render_sh.h:121-123)."""
+    d = one_tile_setup([[0, 0]] * 3, [[0.01, 0, 0, 0.01]] * 3, [0.5, 0.5, 0.5],
+                       [[[1.0], [1.0], [1.0]], [[2.0], [2.0], [2.0]], [[-3.0], [-3.0], [-3.0]]])
+    px, py = pix_pos(d, 3, 3)
+    d["mean"][:] = (px, py)
+    out, T, nd, _ = fwd(orc, d, thresh=0.3)
+    want = 0.5 * sig(K0 * 1.0) + 0.25 * sig(K0 * 2.0)
+    assert np.allclose(out.reshape(16, 16, 3)[3, 3], want, rtol=1e-6)
+    assert nd.reshape(16, 16)[3, 3] == 2
+    assert abs(T.reshape(16, 16)[3, 3] - 0.25) < 1e-7
+
+
+def test_skip_rule_leaves_transmittance(orc):
+    """a*G < 1/255 is skipped without touching T (vol_render_sh.h:130-132)."""
+    # first Gaussian far from the pixel: G tiny; second at the pixel
+    d = one_tile_setup([[0, 0], [0, 0]], [[1e-4, 0, 0, 1e-4]] * 2, [0.9, 0.4],
+                       [[[5.0], [5.0], [5.0]], [[0.0], [0.0], [0.0]]])
+    px, py = pix_pos(d, 7, 7)
+    d["mean"][0] = (px + 0.05, py)  # r = 0.0025 / 1e-4 = 25 -> G = e^-12.5
+    d["mean"][1] = (px, py)
+    out, T, nd, _ = fwd(orc, d)
+    assert np.allclose(out.reshape(16, 16, 3)[7, 7], 0.4 * 0.5, rtol=1e-6)
+    assert abs(T.reshape(16, 16)[7, 7] - 0.6) < 1e-6
+
+
+def test_opacity_clamp_and_negative_radial(orc):
+    """alpha is clamped to 0.99; a negative radial (indefinite cov) maps to
+    r = 1000, i.e. G = 0, and is skipped."""
+    d = one_tile_setup([[0, 0], [0, 0]], [[0.01, 0, 0, -0.01], [0.01, 0, 0, 0.01]], [1.0, 1.0],
+                       [[[1.0], [1.0], [1.0]], [[2.0], [0.0], [0.0]]])
+    px, py = pix_pos(d, 2, 12)
+    d["mean"][:] = (px, py + 0.01)  # offset along y: q > 0, det < 0 for Gaussian 0
+    out, T, _, _ = fwd(orc, d)
+    # Gaussian 0: det < 0 -> radial < 0 -> skipped.  Gaussian 1: a = 0.99.
+    G1 = np.exp(-0.5 * (0.01 ** 2) / 0.01)
+    want = 0.99 * G1 * sig(K0 * np.array([2.0, 0.0, 0.0]))
+    assert np.allclose(out.reshape(16, 16, 3)[12, 2], want, rtol=2e-6)
+
+
+def test_empty_tile_and_background(orc):
+    d = one_tile_setup([[0, 0]], [[0.01, 0, 0, 0.01]], [0.5], [[[0.0], [0.0], [0.0]]])
+    d["start"][:] = -1
+    d["end"][:] = -1
+    out, T, nd, _ = fwd(orc, d)
+    assert not out.any() and (T == 1).all() and not nd.any()
+    out, _, _, _ = fwd(orc, d, mode=1, bg=[0.1, 0.2, 0.3])
+    assert np.allclose(out.reshape(-1, 3), [0.1, 0.2, 0.3])
+    # non-empty tile with bg: out = o T + bg (1 - T) when T > thresh
+    d = one_tile_setup([[0, 0]], [[0.01, 0, 0, 0.01]], [0.5], [[[0.0], [0.0], [0.0]]])
+    px, py = pix_pos(d, 4, 4)
+    d["mean"][0] = (px, py)
+    out, _, _, pre = fwd(orc, d, mode=1, bg=[1.0, 0.0, 0.5])
+    o = 0.5 * 0.5
+    assert np.allclose(out.reshape(16, 16, 3)[4, 4], [o * 0.5 + 1.0 * 0.5, o * 0.5, o * 0.5 + 0.25], rtol=1e-6)
+    assert np.allclose(pre.reshape(16, 16, 3)[4, 4], o)
+
+
+def _tiny_scene(orc, seed, N=60, W=40, H=36, C=2):
+    rng = np.random.default_rng(seed)
+    from gaussian_splatting_3d_amd import scenes
+
+    c2w, cam = scenes.orbit_camera(3.0, 0.3, 40.0, 40.0, W, H, near=0.01, far=50.0, height=0.2)
+    mean = rng.normal(size=(N, 3)).astype(np.float32) * 0.5
+    q = rng.normal(size=(N, 4)).astype(np.float32)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    log_s = (rng.normal(size=(N, 3)) * 0.3 - 2.0).astype(np.float32)
+    logit = rng.normal(size=N).astype(np.float32)
+    sh = (rng.normal(size=(N, 3, C * C)) * 0.4).astype(np.float32)
+    ocam = orc.Camera(cam.fx, cam.fy, cam.cx, cam.cy, cam.w, cam.h, cam.near_plane, cam.far_plane)
+    return orc.pipeline_forward(mean, q, log_s, logit, sh, c2w.numpy(), ocam, C), c2w.numpy()
+
+
+@pytest.mark.parametrize("seed,C", [(0, 1), (1, 2), (2, 3), (3, 4)])
+def test_matches_fp64_brute_force(orc, seed, C):
+    """Independent per-pixel compositor (no tiles, batches or sort kernels)."""
+    r, c2w = _tiny_scene(orc, seed, C=C)
+    bf = orc.brute_force_render(r["mean2d"], r["cov2d"], r["sh_v"], r["a"][r["idx"]], r["depth"], r["tl"],
+                                r["br"], r["topleft"], c2w, r["psx"], r["psy"], r["H"], r["W"], C, 1e-4)
+    assert np.abs(bf.reshape(-1) - r["out"]).max() < 2e-5
+    assert r["out"].max() > 0.1
+
+
+@pytest.mark.parametrize("seed,C,mode", [(5, 1, 0), (6, 3, 0), (7, 4, 0), (8, 2, 0)])
+def test_backward_matches_central_differences(orc, seed, C, mode):
+    """Oracle gradients w.r.t. mean2d, cov2d (symmetric perturbation), SH and
+    alpha against float64 central differences of the compositor with the
+    skip/break decisions frozen."""
+    r, c2w = _tiny_scene(orc, seed, N=40, W=32, H=32, C=C)
+    rng = np.random.default_rng(seed + 100)
+    g = rng.normal(size=r["out"].shape).astype(np.float32)
+    al = r["a"][r["idx"]]
+    grads, mism = orc.render_sh_bwd(r["mean2d"], r["cov2d"], r["sh_v"], al, r["start"], r["end"], r["ids"],
+                                    r["out"], g, r["topleft"], c2w, 16, r["n_tiles_h"], r["n_tiles_w"], r["psx"],
+                                    r["psy"], r["H"], r["W"], C, 1e-4)
+    assert mism == 0
+    args = (r["start"], r["end"], r["ids"], r["topleft"], c2w, r["psx"], r["psy"], r["H"], r["W"], C, 1e-4)
+    base, dec = orc.composite_f64(r["mean2d"], r["cov2d"], r["sh_v"], al, *args)
+
+    def L(m2d, cv, sh, a):
+        o, _ = orc.composite_f64(m2d, cv, sh, a, *args, decisions=dec)
+        return float(o @ g.astype(np.float64))
+
+    m2d = r["mean2d"].astype(np.float64)
+    cv = r["cov2d"].astype(np.float64)
+    sh = r["sh_v"].astype(np.float64)
+    a = al.astype(np.float64)
+    V = len(a)
+    busy = np.argsort(-np.abs(grads["alpha"]))[:6]
+    h = 1e-5
+    for v in busy:
+        for k in range(2):
+            p, m = m2d.copy(), m2d.copy()
+            p[v, k] += h
+            m[v, k] -= h
+            fd = (L(p, cv, sh, a) - L(m, cv, sh, a)) / (2 * h)
+            assert abs(fd - grads["mean"][v, k]) <= 2e-3 * max(abs(fd), 1e-2 * np.abs(grads["mean"]).max())
+        for (i, j) in ((0, 0), (3, 3), (1, 2)):
+            p, m = cv.copy(), cv.copy()
+            p[v, i] += h * 1e-2
+            m[v, i] -= h * 1e-2
+            if i != j:
+                p[v, j] += h * 1e-2
+                m[v, j] -= h * 1e-2
+            fd = (L(m2d, p, sh, a) - L(m2d, m, sh, a)) / (2 * h * 1e-2)
+            an = grads["cov"].reshape(V, 4)[v, i] + (grads["cov"].reshape(V, 4)[v, j] if i != j else 0.0)
+            assert abs(fd - an) <= 2e-3 * max(abs(fd), 1e-2 * np.abs(grads["cov"]).max())
+        for (ch, k) in ((0, 0), (2, C * C - 1)):
+            p, m = sh.copy(), sh.copy()
+            p[v, ch, k] += h
+            m[v, ch, k] -= h
+            fd = (L(m2d, cv, p, a) - L(m2d, cv, m, a)) / (2 * h)
+            assert abs(fd - grads["sh"][v, ch, k]) <= 2e-3 * max(abs(fd), 1e-2 * np.abs(grads["sh"]).max())
+        if a[v] < 0.99:
+            p, m = a.copy(), a.copy()
+            p[v] += h
+            m[v] -= h
+            fd = (L(m2d, cv, sh, p) - L(m2d, cv, sh, m)) / (2 * h)
+            assert abs(fd - grads["alpha"][v]) <= 2e-3 * max(abs(fd), 1e-2 * np.abs(grads["alpha"]).max())
+
+
+def test_rgb_variant_closed_form_and_gradient(orc):
+    """render_start_end: out = sum a T G c with colour as given; grad_color
+    = sum a T G g (vol_render.h:252-352)."""
+    d = one_tile_setup([[0, 0], [0, 0]], [[0.01, 0, 0, 0.01]] * 2, [0.6, 0.5], [[[0]] * 3] * 2)
+    px, py = pix_pos(d, 6, 10)
+    d["mean"][:] = (px, py)
+    col = np.array([[0.2, 0.4, 0.6], [0.9, 0.1, 0.3]], np.float32)
+    out, T, nd = orc.render_rgb_fwd(d["mean"], d["cov"], col, d["alpha"], d["start"], d["end"], d["ids"],
+                                    d["topleft"], 16, 1, 1, d["psx"], d["psy"], 16, 16, 1e-4)
+    assert np.allclose(out.reshape(16, 16, 3)[10, 6], 0.6 * col[0] + 0.4 * 0.5 * col[1], rtol=1e-6)
+    g = np.zeros_like(out)
+    g.reshape(16, 16, 3)[10, 6] = [1.0, 2.0, 3.0]
+    gr, _ = orc.render_rgb_bwd(d["mean"], d["cov"], col, d["alpha"], d["start"], d["end"], d["ids"], out, g,
+                               d["topleft"], 16, 1, 1, d["psx"], d["psy"], 16, 16, 1e-4)
+    assert np.allclose(gr["color"][0], 0.6 * np.array([1, 2, 3]), rtol=1e-6)
+    assert np.allclose(gr["color"][1], 0.2 * np.array([1, 2, 3]), rtol=1e-6)
+    # dL/da1 with G = 1: d/da1 [a1 c1 + (1-a1) a2 c2] . g
+    want = col[0] @ [1, 2, 3] - 0.5 * col[1] @ [1, 2, 3]
+    assert abs(gr["alpha"][0] - want) < 1e-5
