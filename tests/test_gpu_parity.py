@@ -349,3 +349,28 @@ def test_sh_renderer_module_matches_fused_path(orc):
     img.sum().backward()
     for prm in (m.mean, m.qvec, m.svec_before_activation, m.alpha_before_activation, m.sh_coeffs):
         assert prm.grad is not None and torch.isfinite(prm.grad).all()
+
+
+@pytest.mark.parametrize("C", [1, 2])
+def test_mock_scene_golden(C):
+    """The reference's debug scene through the fused gfx950 path, against the
+    committed oracle fixture (tile lists bit-exact, sampled pixels <= 1e-4)."""
+    import os
+    import sys
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from make_golden import mock_scene
+
+    g = np.load(os.path.join(here, "mock_scene_sh.npz"))
+    mean, qvec, log_s, color, logit, c2w, ocam_ = mock_scene()
+    cam = CameraInfo(ocam_.fx, ocam_.fy, ocam_.cx, ocam_.cy, ocam_.w, ocam_.h, ocam_.near_plane, ocam_.far_plane)
+    cfg = scenes.RasterConfig(tile_culling_radius=10.0, frustum_culling_radius=4.0)
+    p = [torch.from_numpy(np.ascontiguousarray(x)).to(DEV) for x in (mean, qvec, log_s, logit, g[f"C{C}_sh"])]
+    out, st = rasterize.forward_view(*p, torch.from_numpy(c2w), cam, C, cfg)
+    assert st.b.P == int(g[f"C{C}_P"])
+    assert np.array_equal(st.b.ids.cpu().numpy(), g[f"C{C}_ids"])
+    assert np.array_equal(st.b.start.cpu().numpy(), g[f"C{C}_start"])
+    o = out.cpu().numpy()
+    assert np.abs(o[g[f"C{C}_pix"]] - g[f"C{C}_vals"]).max() <= IMG_TOL
+    assert abs(o.astype(np.float64).sum() - float(g[f"C{C}_checksum"])) <= 1e-4 * o.size * 1e-3
