@@ -202,6 +202,67 @@ def cpu_baseline(scene, budget_s=20.0):
                 seconds_measured=round(t_pre + t_comp + (time.perf_counter() - a0), 1))
 
 
+KERNEL_PREFIX = {"composite_fwd": "gsmi::k_fwd_sh", "composite_bwd": "gsmi::k_bwd_sh"}
+
+
+def _pmc_pass(counter, child_args, timeout_s=90):
+    """One rocprofv3 counter pass over a child run of this script; returns
+    {kernel name: mean counter value per dispatch} or raises."""
+    import shutil
+    import signal
+    import sqlite3
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        raise RuntimeError("rocprofv3 not found")
+    out = tempfile.mkdtemp(prefix="gsmi_pmc_", dir="/tmp")
+    cmd = [exe, "--pmc", counter, "-d", out, "-o", "run", "--", sys.executable, os.path.abspath(__file__)] + child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                            start_new_session=True)
+    try:
+        rc = proc.wait(timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        os.killpg(proc.pid, signal.SIGKILL)
+        proc.wait()
+        raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
+    db = os.path.join(out, "run_results.db")
+    if not os.path.exists(db):
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {rc} without results")
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, avg(value), count(*) from counters_collection "
+                       "where counter_name = ? group by kernel_name", (counter,)).fetchall()
+    con.close()
+    shutil.rmtree(out, ignore_errors=True)
+    return {name: (float(v), int(n)) for name, v, n in rows}
+
+
+def measure_traffic(workload, exact, kernel):
+    """HBM bytes per launch of `kernel` from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
+    for a 1 GiB device-to-device copy (bytes moved / counter KiB)."""
+    copy_bytes = float(1 << 30)
+    res = {}
+    child = ["--pmc-child", "--workload", workload] + (["--exact"] if exact else [])
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = _pmc_pass(counter, ["--copy-calibration"])
+        copy_kib = [v for k, (v, n) in cal.items() if "copyBuffer" in k]
+        if not copy_kib or copy_kib[0] <= 0:
+            raise RuntimeError(f"no copy kernel seen in the {counter} calibration run")
+        scale = copy_bytes / copy_kib[0]  # bytes per counter KiB
+        run = _pmc_pass(counter, child)
+        vals = [v for k, (v, n) in run.items() if k.startswith(KERNEL_PREFIX[kernel])]
+        if not vals:
+            raise RuntimeError(f"{kernel} not seen in the {counter} run")
+        res[counter] = (vals[0], scale)
+    fetch = res["FETCH_SIZE"][0] * res["FETCH_SIZE"][1]
+    write = res["WRITE_SIZE"][0] * res["WRITE_SIZE"][1]
+    return dict(bytes=fetch + write, fetch_bytes=fetch, write_bytes=write,
+                fetch_scale_bytes_per_kib=res["FETCH_SIZE"][1], write_scale_bytes_per_kib=res["WRITE_SIZE"][1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,12 +271,28 @@ def main():
     ap.add_argument("--workload", default="dense_3m", choices=WORKLOADS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact", action="store_true", help="exact colour arithmetic (bit-reproducible images)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 3 times (scales rocprof byte counters)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the dominant kernel's HBM bytes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": src.numel() * 4, "copy_bytes_written_per_launch": dst.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
+        return
+    if args.pmc_child:  # one step under a counter pass of measure_traffic
+        gpu_bench(make_scene(args.workload), 1, 1, exact=args.exact)
         return
     scene = make_scene(args.workload)
     print(f"[bench] workload {scene.name}: N={scene.N}", file=sys.stderr, flush=True)
@@ -250,6 +327,15 @@ def main():
                      "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
                      "alg_bytes_per_launch": int(ab[dom])},
     }
+    if not args.no_traffic:
+        try:
+            tr = measure_traffic(args.workload, args.exact, dom)
+            line["roofline"]["traffic"] = int(tr["bytes"])
+            line["roofline"]["traffic_detail"] = {k: (round(v, 3) if isinstance(v, float) else v)
+                                                  for k, v in tr.items()}
+        except Exception as e:  # counters unavailable: traffic stays null, say why
+            line["roofline"]["traffic_error"] = str(e)
+        print(f"[bench] traffic: {line['roofline'].get('traffic')}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(scene)
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
