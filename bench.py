@@ -253,7 +253,7 @@ def measure_traffic(workload, exact, kernel):
             raise RuntimeError(f"no copy kernel seen in the {counter} calibration run")
         scale = copy_bytes / copy_kib[0]  # bytes per counter KiB
         run = _pmc_pass(counter, child)
-        vals = [v for k, (v, n) in run.items() if k.startswith(KERNEL_PREFIX[kernel])]
+        vals = [v for k, (v, n) in run.items() if k.split("void ", 1)[-1].startswith(KERNEL_PREFIX[kernel])]
         if not vals:
             raise RuntimeError(f"{kernel} not seen in the {counter} run")
         res[counter] = (vals[0], scale)
