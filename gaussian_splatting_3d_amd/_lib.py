@@ -55,6 +55,17 @@ SIGNATURES = {
                                                                 _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _F,
                                                                 _F, _I32, _I32, _F, _P]),
     "gsmi_project_backward": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "gsmi_preprocess_views_workspace_size": (_SZ, [_I64, _I32]),
+    "gsmi_preprocess_views_count": (_I32, [_I64, _I32, _P, _P, _P, _P, _F, _F, _I32, _P, _P, _P, _P, _SZ, _P]),
+    "gsmi_preprocess_views_emit": (_I32, [_I64, _I32, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                          _P]),
+    "gsmi_bin_views": (_I32, [_I64, _I64, _I32, _P, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gsmi_render_views_sh": (_I32, [_I64, _I64, _I32, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P,
+                                    _I32, _F, _P, _P, _P, _P, _I32, _I32, _P]),
+    "gsmi_render_views_backward_sh": (_I32, [_I64, _I64, _I32, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P, _P,
+                                             _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _P, _P]),
+    "gsmi_project_backward_views": (_I32, [_I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                           _P]),
     "gsmi_exp_spec": (_I32, [_I64, _P, _P, _P]),
 }
 

@@ -6,12 +6,16 @@
 // order is produced here with far less traffic:
 //   1. stable radix sort of the V visible Gaussians by depth bits (32-bit
 //      keys, values = index), so ties keep index order;
-//   2. pair counts gathered in that order, exclusive scan -> offsets;
-//   3. load-balanced expansion: each output slot finds its Gaussian by binary
-//      search over the offsets (in LDS) and writes (tile, index), Gaussians in
-//      depth order and each Gaussian's tiles x-outer, y-inner;
-//   4. stable radix sort of the pairs on the tile id alone
-//      (ceil(log2 n_tiles) bits, two passes for <= 16384 tiles).
+//   2. pair counts and tile rectangles gathered in that order, exclusive
+//      scan -> offsets;
+//   3. the pairs are never written in depth order: the expansion (each output
+//      slot -> its Gaussian and tile, Gaussians in depth order, each
+//      Gaussian's tiles x-outer, y-inner) runs twice, once to count the low
+//      7 tile-id bits per 4096-slot block and once to rank the block's pairs
+//      on those bits and scatter them as the first radix pass;
+//   4. the remaining tile-id bits in 8-bit passes; the last pass also
+//      records each tile's first and last position (atomic min / max of the
+//      block-local run ends), so no separate range pass reads the keys.
 // Within a tile the entries are then ordered by (depth bits, index), i.e.
 // exactly the order of a stable sort of the 64-bit keys emitted in index
 // order.  The 64-bit keys are materialised only when a caller asks for them.
@@ -53,39 +57,43 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const K* __restric
 }
 
 template <typename K, int RB>
-__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __restrict__ kin,
-                                                                 const int32_t* __restrict__ vin,
-                                                                 K* __restrict__ kout, int32_t* __restrict__ vout,
-                                                                 int64_t P, int shift,
-                                                                 const uint32_t* __restrict__ offs, int nblk) {
+struct ScatterLds {
+  uint32_t cnt[4][1 << RB];
+  uint32_t base[4][1 << RB];
+  uint32_t dstart[1 << RB];
+  uint32_t goff[1 << RB];
+  uint32_t wsum[4];
+  K keys[kRadixTile];
+  int32_t vals[kRadixTile];
+};
+
+// Stable scatter of one block of kRadixTile items.  Item r of lane `lane` in
+// wave w sits at block position w*1024 + r*64 + lane (key[r], val[r]; keys of
+// invalid positions are all-ones).  Ranks by digit (key >> shift) & (2^RB-1)
+// with wave64 ballot matching, permutes through LDS, writes each digit's run
+// at offs[digit][block] (exclusive-scanned histogram).  kout may be null
+// (keys not needed).  With RANGES the keys are final tile ids in sorted
+// order: every block-local run boundary is folded into start (atomic min) and
+// end (atomic max of position + 1).
+template <typename K, int RB, bool RANGES>
+__device__ __forceinline__ void block_scatter(ScatterLds<K, RB>& s, K (&key)[kRadixItems],
+                                              int32_t (&val)[kRadixItems], int64_t base, int64_t P, int shift,
+                                              const uint32_t* __restrict__ offs, int nblk, K* __restrict__ kout,
+                                              int32_t* __restrict__ vout, int32_t* __restrict__ start,
+                                              int32_t* __restrict__ end, uint32_t n_tiles) {
   constexpr int BINS = 1 << RB;
   static_assert(BINS <= kRadixThreads, "one digit per thread in the block prefix");
-  __shared__ uint32_t s_cnt[4][BINS];
-  __shared__ uint32_t s_base[4][BINS];
-  __shared__ uint32_t s_dstart[BINS];
-  __shared__ uint32_t s_goff[BINS];
-  __shared__ uint32_t s_wsum[4];
-  __shared__ K s_keys[kRadixTile];
-  __shared__ int32_t s_vals[kRadixTile];
-
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int64_t base = (int64_t)blockIdx.x * kRadixTile;
-  for (int i = lane; i < BINS; i += 64) s_cnt[w][i] = 0;
-  if (t < BINS) s_goff[t] = offs[(int64_t)t * nblk + blockIdx.x];
+  for (int i = lane; i < BINS; i += 64) s.cnt[w][i] = 0;
+  if (t < BINS) s.goff[t] = offs[(int64_t)t * nblk + blockIdx.x];
   __syncthreads();
 
   const uint64_t lt_mask = (1ull << lane) - 1ull;
-  K key[kRadixItems];
-  int32_t val[kRadixItems];
   uint32_t rank[kRadixItems];
-  // Wave w ranks items [w*1024, (w+1)*1024) of the block, 64 per round, in
-  // position order, so ranks are stable.
+  // Wave w ranks its 1024 items 64 per round in position order, so ranks
+  // are stable.
 #pragma unroll
   for (int r = 0; r < kRadixItems; ++r) {
-    const int64_t p = base + w * (64 * kRadixItems) + r * 64 + lane;
-    const bool valid = p < P;
-    key[r] = valid ? kin[p] : (K)~(K)0;
-    val[r] = valid ? vin[p] : 0;
     const uint32_t d = (uint32_t)(key[r] >> shift) & (BINS - 1);
     uint64_t peers = ~0ull;
 #pragma unroll
@@ -95,9 +103,9 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __rest
       peers &= bit ? bal : ~bal;
     }
     const uint32_t lower = __popcll(peers & lt_mask);
-    const uint32_t old = s_cnt[w][d];
+    const uint32_t old = s.cnt[w][d];
     __builtin_amdgcn_wave_barrier();
-    if (lower == 0) s_cnt[w][d] = old + (uint32_t)__popcll(peers);
+    if (lower == 0) s.cnt[w][d] = old + (uint32_t)__popcll(peers);
     __builtin_amdgcn_wave_barrier();
     rank[r] = old + lower;
   }
@@ -105,10 +113,10 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __rest
   // Digit t: per-wave counts -> wave offsets; block prefix over digits.
   uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
   if (t < BINS) {
-    c0 = s_cnt[0][t];
-    c1 = s_cnt[1][t];
-    c2 = s_cnt[2][t];
-    c3 = s_cnt[3][t];
+    c0 = s.cnt[0][t];
+    c1 = s.cnt[1][t];
+    c2 = s.cnt[2][t];
+    c3 = s.cnt[3][t];
   }
   const uint32_t tot = c0 + c1 + c2 + c3;
   uint32_t inc = tot;
@@ -117,38 +125,67 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __rest
     const uint32_t o = __shfl_up(inc, d);
     if (lane >= d) inc += o;
   }
-  if (lane == 63) s_wsum[w] = inc;
+  if (lane == 63) s.wsum[w] = inc;
   __syncthreads();
   uint32_t pre = 0;
-  for (int i = 0; i < w; ++i) pre += s_wsum[i];
+  for (int i = 0; i < w; ++i) pre += s.wsum[i];
   if (t < BINS) {
     const uint32_t dstart = pre + inc - tot;
-    s_dstart[t] = dstart;
-    s_base[0][t] = dstart;
-    s_base[1][t] = dstart + c0;
-    s_base[2][t] = dstart + c0 + c1;
-    s_base[3][t] = dstart + c0 + c1 + c2;
+    s.dstart[t] = dstart;
+    s.base[0][t] = dstart;
+    s.base[1][t] = dstart + c0;
+    s.base[2][t] = dstart + c0 + c1;
+    s.base[3][t] = dstart + c0 + c1 + c2;
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kRadixItems; ++r) {
     const uint32_t d = (uint32_t)(key[r] >> shift) & (BINS - 1);
-    const uint32_t lp = s_base[w][d] + rank[r];
-    s_keys[lp] = key[r];
-    s_vals[lp] = val[r];
+    const uint32_t lp = s.base[w][d] + rank[r];
+    s.keys[lp] = key[r];
+    s.vals[lp] = val[r];
   }
   __syncthreads();
-  const int64_t nvalid = min((int64_t)kRadixTile, P - base);
+  const int nvalid = (int)min((int64_t)kRadixTile, P - base);
 #pragma unroll 4
   for (int i = t; i < kRadixTile; i += kRadixThreads) {
     if (i < nvalid) {
-      const K k = s_keys[i];
+      const K k = s.keys[i];
       const uint32_t d = (uint32_t)(k >> shift) & (BINS - 1);
-      const int64_t gp = (int64_t)s_goff[d] + (i - (int64_t)s_dstart[d]);
-      kout[gp] = k;
-      vout[gp] = s_vals[i];
+      const int64_t gp = (int64_t)s.goff[d] + (i - (int64_t)s.dstart[d]);
+      if (kout) kout[gp] = k;
+      vout[gp] = s.vals[i];
+      if constexpr (RANGES) {
+        const uint32_t tile = (uint32_t)k;
+        if (tile >= n_tiles) continue;  // only from inconsistent caller bounds
+        if (i == 0 || (uint32_t)s.keys[i - 1] != tile) atomicMin(start + tile, (int32_t)gp);
+        if (i == nvalid - 1 || (uint32_t)s.keys[i + 1] != tile) atomicMax(end + tile, (int32_t)(gp + 1));
+      }
     }
   }
+}
+
+template <typename K, int RB, bool RANGES>
+__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __restrict__ kin,
+                                                                 const int32_t* __restrict__ vin,
+                                                                 K* __restrict__ kout, int32_t* __restrict__ vout,
+                                                                 int64_t P, int shift,
+                                                                 const uint32_t* __restrict__ offs, int nblk,
+                                                                 int32_t* __restrict__ start,
+                                                                 int32_t* __restrict__ end, uint32_t n_tiles) {
+  __shared__ ScatterLds<K, RB> s;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+  K key[kRadixItems];
+  int32_t val[kRadixItems];
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const int64_t p = base + w * (64 * kRadixItems) + r * 64 + lane;
+    const bool valid = p < P;
+    key[r] = valid ? kin[p] : (K)~(K)0;
+    val[r] = valid ? vin[p] : 0;
+  }
+  block_scatter<K, RB, RANGES>(s, key, val, base, P, shift, offs, nblk, kout, vout, start, end, n_tiles);
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -178,35 +215,65 @@ __global__ void k_depth_keys(const float* __restrict__ depth, int64_t V, uint32_
   dval[v] = (int32_t)v;
 }
 
-// Pair count of the Gaussian at depth rank r (gs/culling.py:33-35).
-__global__ void k_sorted_counts(const int32_t* __restrict__ dval, const int32_t* __restrict__ tl,
-                                const int32_t* __restrict__ br, int64_t V, int32_t* __restrict__ cnt) {
+// Pair count and tile rectangle of the Gaussian at depth rank r
+// (gs/culling.py:33-35): rect = {first tile id, tiles per row, rows, index};
+// the pair of local index l is tile first + (l % rows) * per_row + l / rows
+// (x outer, y inner).  Multi-view: view[r] selects the record's camera and
+// its tile_base (first tile id of the view in the batch numbering).
+__global__ void k_sorted_rects(const int32_t* __restrict__ dval, const uint32_t* __restrict__ view,
+                               const gsmi_view* __restrict__ views, int n_tiles_w, const int32_t* __restrict__ tl,
+                               const int32_t* __restrict__ br, int64_t V, int32_t* __restrict__ cnt,
+                               int4* __restrict__ rect) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= V) return;
   const int v = dval[r];
+  int base = 0, ntw = n_tiles_w;
+  if (views) {
+    const gsmi_view& cam = views[view ? view[r] : 0];
+    base = cam.tile_base;
+    ntw = cam.n_tiles_w;
+  }
   const int2 a = reinterpret_cast<const int2*>(tl)[v];
   const int2 b = reinterpret_cast<const int2*>(br)[v];
   cnt[r] = (b.x - a.x + 1) * (b.y - a.y + 1);
+  rect[r] = make_int4(base + a.y * ntw + a.x, ntw, b.y - a.y + 1, v);
 }
 
-// Load-balanced expansion of the depth-ordered Gaussians into (tile, index)
-// pairs.  Block b owns output slots [b*kExpandSlots, (b+1)*kExpandSlots); the
-// Gaussians overlapping that range are staged in LDS (every visible Gaussian
-// has at least one pair, so at most kExpandSlots + 1 of them), and each thread
-// locates the Gaussian of its slots by binary search.
-constexpr int kExpandSlots = 2048;
-constexpr int kExpandPerThread = kExpandSlots / 256;
+// View of visible entry dval[r]: the largest b with bases[2b] <= dval[r].
+__global__ void k_view_keys(const int32_t* __restrict__ dval, int64_t V, const int64_t* __restrict__ bases, int B,
+                            uint32_t* __restrict__ view) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= V) return;
+  const int64_t v = dval[r];
+  int lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bases[2 * mid] <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  view[r] = (uint32_t)lo;
+}
 
-__global__ __launch_bounds__(256) void k_expand(const int32_t* __restrict__ off, const int32_t* __restrict__ dval,
-                                                const int32_t* __restrict__ tl, const int32_t* __restrict__ br,
-                                                int64_t V, int64_t P, int n_tiles_w, uint32_t* __restrict__ tile_out,
-                                                int32_t* __restrict__ val_out) {
-  __shared__ int32_t s_off[kExpandSlots + 2];
-  __shared__ int32_t s_v[kExpandSlots + 2];
-  __shared__ int32_t s_range[2];
+// Expansion of the depth-ordered Gaussians into (tile, index) pairs.  Block b
+// owns output slots [b*kRadixTile, (b+1)*kRadixTile), the same partition as
+// the radix passes.  The offsets of the Gaussians overlapping that range are
+// staged in LDS (every visible Gaussian has at least one pair, so at most
+// kRadixTile + 1 of them); thread t takes the kRadixItems consecutive slots
+// starting at t*kRadixItems, finds the Gaussian of the first by binary search
+// and walks forward.
+constexpr int kExpandDigitBits = 7;
+
+struct ExpandLds {
+  int32_t off[kRadixTile + 2];
+  int32_t range[2];
+};
+
+// Stages the block's Gaussian offsets; returns (r0, n) = first depth rank and
+// count of staged Gaussians.
+__device__ __forceinline__ int2 expand_stage(ExpandLds& s, const int32_t* __restrict__ off, int64_t V, int64_t P) {
   const int t = threadIdx.x;
-  const int64_t p0 = (int64_t)blockIdx.x * kExpandSlots;
-  const int64_t p1 = min(p0 + kExpandSlots, P);  // exclusive
+  const int64_t p0 = (int64_t)blockIdx.x * kRadixTile;
+  const int64_t p1 = min(p0 + kRadixTile, P);  // exclusive
   if (t < 2) {
     // largest r with off[r] <= target
     const int64_t target = t == 0 ? p0 : p1 - 1;
@@ -216,47 +283,124 @@ __global__ __launch_bounds__(256) void k_expand(const int32_t* __restrict__ off,
       if ((int64_t)off[mid] <= target) lo = mid;
       else hi = mid - 1;
     }
-    s_range[t] = (int32_t)lo;
+    s.range[t] = (int32_t)lo;
   }
   __syncthreads();
-  const int r0 = s_range[0];
-  // <= kExpandSlots + 1 when every Gaussian has at least one pair; clamped so
-  // inconsistent bounds cannot index past the LDS arrays.
-  const int n = max(1, min(s_range[1] - r0 + 1, kExpandSlots + 1));
-  for (int i = t; i < n; i += 256) {
-    s_off[i] = off[r0 + i];
-    s_v[i] = dval[r0 + i];
-  }
+  const int r0 = s.range[0];
+  // <= kRadixTile + 1 when every Gaussian has at least one pair; clamped so
+  // inconsistent bounds cannot index past the LDS array.
+  const int n = max(1, min(s.range[1] - r0 + 1, kRadixTile + 1));
+  for (int i = t; i < n; i += kRadixThreads) s.off[i] = off[r0 + i];
   __syncthreads();
-  int64_t p = p0 + (int64_t)t * kExpandPerThread;
-  if (p >= p1 || V <= 0) return;
-  // binary search for the first slot, then walk forward
+  return make_int2(r0, n);
+}
+
+// Tiles and indices of this thread's slots p0 + t*kRadixItems + k; invalid
+// slots (past P) get tile 0xffffffff.
+__device__ __forceinline__ void expand_slots(const ExpandLds& s, int2 rn, const int4* __restrict__ rect, int64_t P,
+                                             uint32_t (&tile)[kRadixItems],
+                                             int32_t (&val)[kRadixItems]) {
+  const int t = threadIdx.x;
+  const int r0 = rn.x, n = rn.y;
+  int64_t p = (int64_t)blockIdx.x * kRadixTile + (int64_t)t * kRadixItems;
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if ((int64_t)s_off[mid] <= p) lo = mid;
+    if ((int64_t)s.off[mid] <= p) lo = mid;
     else hi = mid - 1;
   }
   int j = lo;
-  int v = s_v[j];
-  int2 a = reinterpret_cast<const int2*>(tl)[v];
-  int2 b = reinterpret_cast<const int2*>(br)[v];
+  int4 rc = rect[r0 + j];
 #pragma unroll
-  for (int k = 0; k < kExpandPerThread; ++k, ++p) {
-    if (p >= p1) break;
-    while (j + 1 < n && (int64_t)s_off[j + 1] <= p) {
-      ++j;
-      v = s_v[j];
-      a = reinterpret_cast<const int2*>(tl)[v];
-      b = reinterpret_cast<const int2*>(br)[v];
+  for (int k = 0; k < kRadixItems; ++k, ++p) {
+    tile[k] = 0xffffffffu;
+    val[k] = 0;
+    if (p < P) {
+      while (j + 1 < n && (int64_t)s.off[j + 1] <= p) {
+        ++j;
+        rc = rect[r0 + j];
+      }
+      const int local = (int)(p - s.off[j]);
+      const int ny = max(rc.z, 1);
+      const int col = local / ny;
+      tile[k] = (uint32_t)(rc.x + (local - col * ny) * rc.y + col);
+      val[k] = rc.w;
     }
-    const int local = (int)(p - s_off[j]);
-    const int ny = b.y - a.y + 1;
-    const int x = a.x + local / ny;
-    const int y = a.y + local - (local / ny) * ny;
-    tile_out[p] = (uint32_t)(y * n_tiles_w + x);
-    val_out[p] = v;
   }
+}
+
+// Pass-1 histogram of the expansion: counts of the low kExpandDigitBits
+// tile-id bits per block, hist[digit][block].
+__global__ __launch_bounds__(kRadixThreads) void k_expand_hist(const int32_t* __restrict__ off,
+                                                               const int4* __restrict__ rect, int64_t V, int64_t P,
+                                                               uint32_t* __restrict__ hist,
+                                                               int nblk) {
+  constexpr int BINS = 1 << kExpandDigitBits;
+  __shared__ ExpandLds s;
+  __shared__ uint32_t s_cnt[4][BINS];
+  const int t = threadIdx.x, w = t >> 6;
+  for (int i = t; i < 4 * BINS; i += kRadixThreads) (&s_cnt[0][0])[i] = 0;
+  const int2 rn = expand_stage(s, off, V, P);
+  uint32_t tile[kRadixItems];
+  int32_t val[kRadixItems];
+  expand_slots(s, rn, rect, P, tile, val);
+#pragma unroll
+  for (int k = 0; k < kRadixItems; ++k)
+    if (tile[k] != 0xffffffffu) atomicAdd(&s_cnt[w][tile[k] & (BINS - 1)], 1u);
+  __syncthreads();
+  if (t < BINS) hist[(int64_t)t * nblk + blockIdx.x] = s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
+}
+
+// Block position q of the staged expansion -> LDS word (64-word XOR
+// swizzle: both the per-thread consecutive writes and the lane-interleaved
+// reads are free of bank conflicts).
+__device__ __forceinline__ int swz(int q) { return q ^ ((q >> 6) & 63); }
+
+// Pass 1 of the pair sort fused with the expansion: the block's pairs are
+// generated again, staged in LDS in slot order, and scattered stably on the
+// low kExpandDigitBits tile-id bits (offsets from k_expand_hist, scanned).
+template <bool RANGES>
+__global__ __launch_bounds__(kRadixThreads) void k_expand_scatter(const int32_t* __restrict__ off,
+                                                                  const int4* __restrict__ rect, int64_t V,
+                                                                  int64_t P,
+                                                                  const uint32_t* __restrict__ offs, int nblk,
+                                                                  uint32_t* __restrict__ kout,
+                                                                  int32_t* __restrict__ vout,
+                                                                  int32_t* __restrict__ start,
+                                                                  int32_t* __restrict__ end, uint32_t n_tiles) {
+  __shared__ ExpandLds se;
+  __shared__ ScatterLds<uint32_t, kExpandDigitBits> s;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int2 rn = expand_stage(se, off, V, P);
+  {
+    uint32_t tile[kRadixItems];
+    int32_t val[kRadixItems];
+    expand_slots(se, rn, rect, P, tile, val);
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+      const int q = swz(t * kRadixItems + k);
+      s.keys[q] = tile[k];
+      s.vals[q] = val[k];
+    }
+  }
+  __syncthreads();
+  uint32_t key[kRadixItems];
+  int32_t val[kRadixItems];
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const int q = swz(w * (64 * kRadixItems) + r * 64 + lane);
+    key[r] = s.keys[q];
+    val[r] = s.vals[q];
+  }
+  // block_scatter's first barrier orders these reads before its LDS writes
+  block_scatter<uint32_t, kExpandDigitBits, RANGES>(s, key, val, (int64_t)blockIdx.x * kRadixTile, P, 0, offs, nblk,
+                                                    kout, vout, start, end, n_tiles);
+}
+
+// start == INT32_MAX (no pair seen by the atomic-min) -> -1.
+__global__ void k_fix_start(int32_t* start, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && start[i] == 0x7fffffff) start[i] = -1;
 }
 
 // 64-bit keys of the sorted pairs: (tile << 32) | depth bits of the Gaussian.
@@ -310,8 +454,8 @@ static int radix_sort(int64_t P, K* keys, int32_t* values, int end_bit, void* ws
     int rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(hist), reinterpret_cast<int32_t*>(hist), nh,
                                      nullptr, scan_ws, st);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_radix_scatter<K, RB>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin, kout, vout, P,
-                       shift, hist, nblk);
+    hipLaunchKernelGGL((k_radix_scatter<K, RB, false>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin, kout,
+                       vout, P, shift, hist, nblk, nullptr, nullptr, 0u);
     GSMI_CHECK_LAUNCH();
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -339,37 +483,101 @@ static int ranges_impl(int64_t P, const K* sorted_keys, int n_tiles, int32_t* st
 
 // Workspace layout of the depth-first binning.
 struct BinWs {
-  uint32_t* dkey;
-  int32_t* dval;
-  int32_t* cnt;
+  uint32_t* dkey;  // [V] depth keys (radix-sorted in place)
+  int32_t* dval;   // [V] indices in depth order
+  int32_t* cnt;    // [V] pair counts in depth order -> exclusive offsets
+  int4* rect;      // [V] tile rectangles in depth order
+  uint32_t* view;  // [V] view of each entry in depth order (multi-view)
   int32_t* total;
-  uint32_t* tkey;
+  uint32_t *ka, *kb;  // [P] tile ids, ping-pong
+  int32_t *va, *vb;   // [P] indices, ping-pong
+  uint32_t* hist;     // [nblk * 256]
   void* scan_ws;
-  void* sort_ws;
+  void* sort_ws;  // depth sort
 };
 
+static int64_t pair_blocks(int64_t P) { return (P + kRadixTile - 1) / kRadixTile; }
+
 static size_t bin_ws_bytes(int64_t V, int64_t P) {
-  return align_up(V * 4) * 3 + 256 + align_up(P * 4) + align_up(scan_workspace_bytes(V, 4)) +
-         std::max(radix_ws_bytes<uint32_t, 8>(V), radix_ws_bytes<uint32_t, 8>(P));
+  const int64_t nh = pair_blocks(P) * 256;
+  return align_up(V * 4) * 4 + align_up(V * 16) + 256 + align_up(P * 4) * 4 + align_up(nh * 4) +
+         align_up(scan_workspace_bytes(std::max(V, nh), 4)) + radix_ws_bytes<uint32_t, 8>(V);
 }
 
 static BinWs carve_bin_ws(void* ws, int64_t V, int64_t P) {
   char* p = reinterpret_cast<char*>(ws);
+  auto take = [&p](size_t bytes) {
+    char* r = p;
+    p += align_up(bytes);
+    return r;
+  };
   BinWs b;
-  b.dkey = reinterpret_cast<uint32_t*>(p);
-  p += align_up(V * 4);
-  b.dval = reinterpret_cast<int32_t*>(p);
-  p += align_up(V * 4);
-  b.cnt = reinterpret_cast<int32_t*>(p);
-  p += align_up(V * 4);
-  b.total = reinterpret_cast<int32_t*>(p);
-  p += 256;
-  b.tkey = reinterpret_cast<uint32_t*>(p);
-  p += align_up(P * 4);
-  b.scan_ws = p;
-  p += align_up(scan_workspace_bytes(V, 4));
+  b.dkey = reinterpret_cast<uint32_t*>(take(V * 4));
+  b.dval = reinterpret_cast<int32_t*>(take(V * 4));
+  b.cnt = reinterpret_cast<int32_t*>(take(V * 4));
+  b.rect = reinterpret_cast<int4*>(take(V * 16));
+  b.view = reinterpret_cast<uint32_t*>(take(V * 4));
+  b.total = reinterpret_cast<int32_t*>(take(256));
+  b.ka = reinterpret_cast<uint32_t*>(take(P * 4));
+  b.kb = reinterpret_cast<uint32_t*>(take(P * 4));
+  b.va = reinterpret_cast<int32_t*>(take(P * 4));
+  b.vb = reinterpret_cast<int32_t*>(take(P * 4));
+  b.hist = reinterpret_cast<uint32_t*>(take(pair_blocks(P) * 256 * 4));
+  b.scan_ws = take(scan_workspace_bytes(std::max(V, pair_blocks(P) * 256), 4));
   b.sort_ws = p;
   return b;
+}
+
+// Pair sort on the tile id: pass 1 fused with the expansion (low 7 bits),
+// then 8-bit passes; the last pass writes ids (and tiles when asked) and the
+// ranges.  Returns the sorted tile ids in *tiles (null unless want_tiles).
+static int pair_sort(const BinWs& w, int64_t V, int64_t P, int n_tiles, int32_t* ids,
+                     int32_t* start, int32_t* end, bool want_tiles, uint32_t** tiles, hipStream_t st) {
+  const int nblk = (int)pair_blocks(P);
+  const int bits = std::max(bits_for(n_tiles), 1);
+  const unsigned nt = (unsigned)n_tiles;
+  *tiles = nullptr;
+  hipLaunchKernelGGL(k_expand_hist, dim3(nblk), dim3(kRadixThreads), 0, st, w.cnt, w.rect, V, P, w.hist, nblk);
+  GSMI_CHECK_LAUNCH();
+  int rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(w.hist), reinterpret_cast<int32_t*>(w.hist),
+                                   (int64_t)nblk << kExpandDigitBits, nullptr, w.scan_ws, st);
+  if (rc) return rc;
+  if (bits <= kExpandDigitBits) {
+    uint32_t* ko = want_tiles ? w.ka : nullptr;
+    hipLaunchKernelGGL((k_expand_scatter<true>), dim3(nblk), dim3(kRadixThreads), 0, st, w.cnt, w.rect, V, P,
+                       w.hist, nblk, ko, ids, start, end, nt);
+    GSMI_CHECK_LAUNCH();
+    *tiles = ko;
+    return GSMI_OK;
+  }
+  hipLaunchKernelGGL((k_expand_scatter<false>), dim3(nblk), dim3(kRadixThreads), 0, st, w.cnt, w.rect, V, P,
+                     w.hist, nblk, w.ka, w.va, nullptr, nullptr, nt);
+  GSMI_CHECK_LAUNCH();
+  uint32_t *kin = w.ka, *kout = w.kb;
+  int32_t *vin = w.va, *vout = w.vb;
+  for (int shift = kExpandDigitBits; shift < bits; shift += 8) {
+    const bool last = shift + 8 >= bits;
+    hipLaunchKernelGGL((k_radix_hist<uint32_t, 8>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, P, shift, w.hist,
+                       nblk);
+    GSMI_CHECK_LAUNCH();
+    rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(w.hist), reinterpret_cast<int32_t*>(w.hist),
+                                 (int64_t)nblk * 256, nullptr, w.scan_ws, st);
+    if (rc) return rc;
+    if (last) {
+      uint32_t* ko = want_tiles ? kout : nullptr;
+      hipLaunchKernelGGL((k_radix_scatter<uint32_t, 8, true>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin, ko,
+                         ids, P, shift, w.hist, nblk, start, end, nt);
+      GSMI_CHECK_LAUNCH();
+      *tiles = ko;
+    } else {
+      hipLaunchKernelGGL((k_radix_scatter<uint32_t, 8, false>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin,
+                         kout, vout, P, shift, w.hist, nblk, nullptr, nullptr, nt);
+      GSMI_CHECK_LAUNCH();
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
+  }
+  return GSMI_OK;
 }
 
 }  // namespace gsmi
@@ -400,20 +608,14 @@ extern "C" size_t gsmi_binning_workspace_size(int64_t V, int64_t P, int n_tiles)
   return bin_ws_bytes(V < 0 ? 0 : V, P < 0 ? 0 : P);
 }
 
-extern "C" int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int32_t* tl, const int32_t* br,
-                                                const float* depth, int n_tiles_h, int n_tiles_w,
-                                                int32_t* gaussian_ids, int32_t* start, int32_t* end,
-                                                uint64_t* sorted_keys, void* workspace, size_t workspace_bytes,
-                                                int flags, void* stream) {
-  if (V < 0 || P < 0 || n_tiles_h <= 0 || n_tiles_w <= 0 || !start || !end) return GSMI_ERR_ARG;
-  if (V > 0 && (!tl || !br || !depth)) return GSMI_ERR_ARG;
-  if (((uintptr_t)tl & 7) || ((uintptr_t)br & 7)) return GSMI_ERR_ARG;
-  if (P > 0 && !gaussian_ids) return GSMI_ERR_ARG;
-  if (P >= (int64_t(1) << 31) || V >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
-  const int n_tiles = n_tiles_h * n_tiles_w;
-  if (n_tiles > (1 << 24)) return GSMI_ERR_UNSUPPORTED;
-  if (!workspace || workspace_bytes < bin_ws_bytes(V, P)) return GSMI_ERR_WORKSPACE;
-  hipStream_t st = (hipStream_t)stream;
+// Depth-first binning of V records (tile rects tl/br, depth) into sorted
+// (tile, record) lists.  Single view: views == null, tiles numbered
+// y * n_tiles_w + x.  Multi-view: records are view-major with bases[2b] the
+// first record of view b; tiles are numbered with each view's tile_base.
+static int bin_impl(int64_t V, int64_t P, const int32_t* tl, const int32_t* br, const float* depth,
+                    const gsmi_view* views, const int64_t* bases, int B, int n_tiles, int n_tiles_w,
+                    int32_t* gaussian_ids, int32_t* start, int32_t* end, uint64_t* sorted_keys, void* workspace,
+                    int flags, hipStream_t st) {
   BinWs w = carve_bin_ws(workspace, V, P);
   int rc;
   if (V > 0) {
@@ -422,7 +624,17 @@ extern "C" int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int3
     GSMI_CHECK_LAUNCH();
     rc = radix_sort<uint32_t, 8>(V, w.dkey, w.dval, 32, w.sort_ws, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sorted_counts, dim3(g), dim3(256), 0, st, w.dval, tl, br, V, w.cnt);
+    const uint32_t* view = nullptr;
+    if (views && B > 1) {
+      // stable pass on the view id: (view, depth bits, record) order
+      hipLaunchKernelGGL(k_view_keys, dim3(g), dim3(256), 0, st, w.dval, V, bases, B, w.view);
+      GSMI_CHECK_LAUNCH();
+      rc = radix_sort<uint32_t, 8>(V, w.view, w.dval, bits_for(B), w.sort_ws, st);
+      if (rc) return rc;
+      view = w.view;
+    }
+    hipLaunchKernelGGL(k_sorted_rects, dim3(g), dim3(256), 0, st, w.dval, view, views, n_tiles_w, tl, br, V, w.cnt,
+                       w.rect);
     GSMI_CHECK_LAUNCH();
   }
   rc = exclusive_scan<int32_t>(w.cnt, w.cnt, V, w.total, w.scan_ws, st);
@@ -435,19 +647,54 @@ extern "C" int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int3
     if (hipStreamSynchronize(st) != hipSuccess) return GSMI_ERR_LAUNCH;
     if ((int64_t)h_total != P) return GSMI_ERR_COUNT;
   }
+  const int nb = (n_tiles + 255) / 256;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, start, (int64_t)n_tiles, 0x7fffffff);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, end, (int64_t)n_tiles, -1);
+  GSMI_CHECK_LAUNCH();
+  uint32_t* tiles = nullptr;
   if (P > 0 && V > 0) {
-    hipLaunchKernelGGL(k_expand, dim3((unsigned)((P + kExpandSlots - 1) / kExpandSlots)), dim3(256), 0, st, w.cnt,
-                       w.dval, tl, br, V, P, n_tiles_w, w.tkey, gaussian_ids);
-    GSMI_CHECK_LAUNCH();
-    rc = radix_sort<uint32_t, 8>(P, w.tkey, gaussian_ids, bits_for(n_tiles), w.sort_ws, st);
+    rc = pair_sort(w, V, P, n_tiles, gaussian_ids, start, end, sorted_keys != nullptr, &tiles, st);
     if (rc) return rc;
   }
-  rc = ranges_impl<uint32_t, 0>(P, w.tkey, n_tiles, start, end, st);
-  if (rc) return rc;
-  if (sorted_keys && P > 0) {
-    hipLaunchKernelGGL(k_make_keys, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, w.tkey, gaussian_ids, depth,
+  hipLaunchKernelGGL(k_fix_start, dim3(nb), dim3(256), 0, st, start, n_tiles);
+  GSMI_CHECK_LAUNCH();
+  if (sorted_keys && P > 0 && tiles) {
+    hipLaunchKernelGGL(k_make_keys, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, tiles, gaussian_ids, depth,
                        P, sorted_keys);
     GSMI_CHECK_LAUNCH();
   }
   return GSMI_OK;
+}
+
+
+extern "C" int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int32_t* tl, const int32_t* br,
+                                                const float* depth, int n_tiles_h, int n_tiles_w,
+                                                int32_t* gaussian_ids, int32_t* start, int32_t* end,
+                                                uint64_t* sorted_keys, void* workspace, size_t workspace_bytes,
+                                                int flags, void* stream) {
+  if (V < 0 || P < 0 || n_tiles_h <= 0 || n_tiles_w <= 0 || !start || !end) return GSMI_ERR_ARG;
+  if (V > 0 && (!tl || !br || !depth)) return GSMI_ERR_ARG;
+  if (((uintptr_t)tl & 7) || ((uintptr_t)br & 7)) return GSMI_ERR_ARG;
+  if (P > 0 && !gaussian_ids) return GSMI_ERR_ARG;
+  if (P >= (int64_t(1) << 31) || V >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
+  const int64_t n_tiles = (int64_t)n_tiles_h * n_tiles_w;
+  if (n_tiles > (1 << 24)) return GSMI_ERR_UNSUPPORTED;
+  if (!workspace || workspace_bytes < bin_ws_bytes(V, P)) return GSMI_ERR_WORKSPACE;
+  return bin_impl(V, P, tl, br, depth, nullptr, nullptr, 1, (int)n_tiles, n_tiles_w, gaussian_ids, start, end,
+                  sorted_keys, workspace, flags, (hipStream_t)stream);
+}
+
+extern "C" int gsmi_bin_views(int64_t V, int64_t P, int B, const gsmi_view* views, const int64_t* bases,
+                              int n_tiles_total, const int32_t* tl, const int32_t* br, const float* depth,
+                              int32_t* gaussian_ids, int32_t* start, int32_t* end, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+  if (V < 0 || P < 0 || B <= 0 || !views || !bases || n_tiles_total <= 0 || !start || !end) return GSMI_ERR_ARG;
+  if (V > 0 && (!tl || !br || !depth)) return GSMI_ERR_ARG;
+  if (((uintptr_t)tl & 7) || ((uintptr_t)br & 7) || ((uintptr_t)views & 7)) return GSMI_ERR_ARG;
+  if (P > 0 && !gaussian_ids) return GSMI_ERR_ARG;
+  if (P >= (int64_t(1) << 31) || V >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
+  if (n_tiles_total > (1 << 24) || B > (1 << 16)) return GSMI_ERR_UNSUPPORTED;
+  if (!workspace || workspace_bytes < bin_ws_bytes(V, P)) return GSMI_ERR_WORKSPACE;
+  return bin_impl(V, P, tl, br, depth, views, bases, B, n_tiles_total, 0, gaussian_ids, start, end, nullptr,
+                  workspace, 0, (hipStream_t)stream);
 }

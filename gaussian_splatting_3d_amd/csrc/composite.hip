@@ -49,7 +49,61 @@ struct RasterArgs {
   float psx, psy, thresh;
   int n_tiles_h, n_tiles_w, H, W;
   int write_empty;
+  // Multi-view batch: tiles of all views in one launch (views == null: the
+  // single view described by topleft/c2w/psx/psy/H/W/n_tiles_*).
+  const gsmi_view* views;
+  int n_views;
+  int n_tiles_total;
 };
+
+// Pixel of this thread in the block's tile.
+struct TileCtx {
+  int tile;  // global tile id (index into start/end)
+  int gx, gy;
+  bool inside;
+  int64_t pix;  // flat pixel index into out / T_final / n_done / grad_out
+  float px, py;
+  float cw[9];  // first 9 floats of the view's c2w buffer (they feed the ray)
+};
+
+__device__ __forceinline__ bool tile_setup(const RasterArgs& a, TileCtx& c) {
+  const int tile = xcd_tile(blockIdx.x, a.n_tiles_total);
+  if (tile >= a.n_tiles_total) return false;
+  int local = tile, ntw = a.n_tiles_w, W = a.W, H = a.H;
+  float tlx, tly, psx = a.psx, psy = a.psy;
+  int64_t pix_base = 0;
+  if (a.views) {
+    const gsmi_view& v = a.views[view_of_tile(a.views, a.n_views, tile)];
+    local = tile - v.tile_base;
+    ntw = v.n_tiles_w;
+    W = v.W;
+    H = v.H;
+    tlx = v.topleft_x;
+    tly = v.topleft_y;
+    psx = v.pixel_size_x;
+    psy = v.pixel_size_y;
+    pix_base = v.pix_base;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.cw[k] = v.c2w[k];
+  } else {
+    tlx = a.topleft[0];
+    tly = a.topleft[1];
+    if (a.c2w) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c.cw[k] = a.c2w[k];
+    }
+  }
+  const int tx = local % ntw, ty = local / ntw;
+  const int t = threadIdx.x;
+  c.tile = tile;
+  c.gx = tx * kTile + (t & 15);
+  c.gy = ty * kTile + (t >> 4);
+  c.inside = c.gx < W && c.gy < H;
+  c.pix = pix_base + (int64_t)c.gy * W + c.gx;
+  c.px = tlx + (float)c.gx * psx;
+  c.py = tly + (float)c.gy * psy;
+  return true;
+}
 
 // Per-Gaussian LDS record: {mx, my, c0, c1}, {c2, c3, det, 1/det}, {a, rlim}.
 struct alignas(16) GeoA {
@@ -161,14 +215,11 @@ __global__ __launch_bounds__(256) void k_fwd_sh(RasterArgs a) {
   __shared__ int s_id[kBatch];
   __shared__ float s_sh[kBatch * SHN];
 
-  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
-  const int tile = xcd_tile(blockIdx.x, n_tiles);
-  if (tile >= n_tiles) return;
-  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
-  const int t = threadIdx.x;
-  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
-  const bool inside = gx < a.W && gy < a.H;
-  const int64_t pix = (int64_t)gy * a.W + gx;
+  TileCtx tc;
+  if (!tile_setup(a, tc)) return;
+  const int tile = tc.tile;
+  const bool inside = tc.inside;
+  const int64_t pix = tc.pix;
   const int s0 = a.start[tile];
   const int e0 = a.end[tile];
   if (s0 == -1 || e0 - s0 == 0) {
@@ -188,11 +239,11 @@ __global__ __launch_bounds__(256) void k_fwd_sh(RasterArgs a) {
     }
     return;
   }
-  const float px = a.topleft[0] + (float)gx * a.psx;
-  const float py = a.topleft[1] + (float)gy * a.psy;
+  const float px = tc.px;
+  const float py = tc.py;
   float cw[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) cw[k] = a.c2w[k];
+  for (int k = 0; k < 9; ++k) cw[k] = tc.cw[k];
   float dir[3];
   pixel_direction(cw, px, py, dir);
   float Y[CC];
@@ -387,23 +438,21 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
   __shared__ float s_acc[kBatch * NCOMP];
   __shared__ int s_tmp[4];
 
-  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
-  const int tile = xcd_tile(blockIdx.x, n_tiles);
-  if (tile >= n_tiles) return;
-  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
+  TileCtx tc;
+  if (!tile_setup(a, tc)) return;
+  const int tile = tc.tile;
   const int t = threadIdx.x;
-  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
-  const bool inside = gx < a.W && gy < a.H;
-  const int64_t pix = (int64_t)gy * a.W + gx;
+  const bool inside = tc.inside;
+  const int64_t pix = tc.pix;
   const int s0 = a.start[tile];
   const int e0 = a.end[tile];
   if (s0 == -1 || e0 - s0 == 0) return;
 
-  const float px = a.topleft[0] + (float)gx * a.psx;
-  const float py = a.topleft[1] + (float)gy * a.psy;
+  const float px = tc.px;
+  const float py = tc.py;
   float cw[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) cw[k] = a.c2w[k];
+  for (int k = 0; k < 9; ++k) cw[k] = tc.cw[k];
   float dir[3];
   pixel_direction(cw, px, py, dir);
   float Y[CC];
@@ -549,25 +598,23 @@ __global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
   __shared__ float s_cc[4][kCcRows * kCcStride];
   __shared__ int s_tmp[4];
 
-  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
-  const int tile = xcd_tile(blockIdx.x, n_tiles);
-  if (tile >= n_tiles) return;
-  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
+  TileCtx tc;
+  if (!tile_setup(a, tc)) return;
+  const int tile = tc.tile;
   const int t = threadIdx.x;
   const int wv = t >> 6;
   const int lane = lane_id();
-  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
-  const bool inside = gx < a.W && gy < a.H;
-  const int64_t pix = (int64_t)gy * a.W + gx;
+  const bool inside = tc.inside;
+  const int64_t pix = tc.pix;
   const int s0 = a.start[tile];
   const int e0 = a.end[tile];
   if (s0 == -1 || e0 - s0 == 0) return;
 
-  const float px = a.topleft[0] + (float)gx * a.psx;
-  const float py = a.topleft[1] + (float)gy * a.psy;
+  const float px = tc.px;
+  const float py = tc.py;
   float cw[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) cw[k] = a.c2w[k];
+  for (int k = 0; k < 9; ++k) cw[k] = tc.cw[k];
   float dir[3];
   pixel_direction(cw, px, py, dir);
   float Y[CC];
@@ -743,14 +790,11 @@ __global__ __launch_bounds__(256) void k_fwd_rgb(RasterArgs a) {
   __shared__ float2 s_gc[kBatch];
   __shared__ int s_id[kBatch];
   __shared__ float s_col[kBatch * 3];
-  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
-  const int tile = xcd_tile(blockIdx.x, n_tiles);
-  if (tile >= n_tiles) return;
-  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
-  const int t = threadIdx.x;
-  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
-  const bool inside = gx < a.W && gy < a.H;
-  const int64_t pix = (int64_t)gy * a.W + gx;
+  TileCtx tc;
+  if (!tile_setup(a, tc)) return;
+  const int tile = tc.tile;
+  const bool inside = tc.inside;
+  const int64_t pix = tc.pix;
   const int s0 = a.start[tile];
   const int e0 = a.end[tile];
   if (s0 == -1 || e0 - s0 == 0) {
@@ -761,8 +805,8 @@ __global__ __launch_bounds__(256) void k_fwd_rgb(RasterArgs a) {
     }
     return;
   }
-  const float px = a.topleft[0] + (float)gx * a.psx;
-  const float py = a.topleft[1] + (float)gy * a.psy;
+  const float px = tc.px;
+  const float py = tc.py;
   const float thresh = a.thresh;
   float T = 1.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
   bool done = !inside;
@@ -822,19 +866,17 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
   __shared__ float s_col[kBatch * 3];
   __shared__ float s_acc[kBatch * NCOMP];
   __shared__ int s_tmp[4];
-  const int n_tiles = a.n_tiles_h * a.n_tiles_w;
-  const int tile = xcd_tile(blockIdx.x, n_tiles);
-  if (tile >= n_tiles) return;
-  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;
+  TileCtx tc;
+  if (!tile_setup(a, tc)) return;
+  const int tile = tc.tile;
   const int t = threadIdx.x;
-  const int gx = tx * kTile + (t & 15), gy = ty * kTile + (t >> 4);
-  const bool inside = gx < a.W && gy < a.H;
-  const int64_t pix = (int64_t)gy * a.W + gx;
+  const bool inside = tc.inside;
+  const int64_t pix = tc.pix;
   const int s0 = a.start[tile];
   const int e0 = a.end[tile];
   if (s0 == -1 || e0 - s0 == 0) return;
-  const float px = a.topleft[0] + (float)gx * a.psx;
-  const float py = a.topleft[1] + (float)gy * a.psy;
+  const float px = tc.px;
+  const float py = tc.py;
   float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
   int nd = 0;
   if (inside) {
@@ -918,7 +960,7 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
 // ---------------------------------------------------------------------------
 template <int C, bool BG>
 static int launch_fwd_sh(const RasterArgs& a, int exact, hipStream_t st) {
-  const int grid = xcd_grid(a.n_tiles_h * a.n_tiles_w);
+  const int grid = xcd_grid(a.n_tiles_total);
   if (exact)
     hipLaunchKernelGGL((k_fwd_sh<C, BG, true>), dim3(grid), dim3(256), 0, st, a);
   else
@@ -929,7 +971,7 @@ static int launch_fwd_sh(const RasterArgs& a, int exact, hipStream_t st) {
 
 template <int C, bool BG>
 static int launch_bwd_sh(const RasterArgs& a, hipStream_t st) {
-  const int grid = xcd_grid(a.n_tiles_h * a.n_tiles_w);
+  const int grid = xcd_grid(a.n_tiles_total);
   if constexpr (C >= 3)
     hipLaunchKernelGGL((k_bwd_sh_mfma<C, BG>), dim3(grid), dim3(256), 0, st, a);
   else
@@ -989,6 +1031,7 @@ extern "C" int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const floa
   a.thresh = thresh;
   a.n_tiles_h = n_tiles_h;
   a.n_tiles_w = n_tiles_w;
+  a.n_tiles_total = n_tiles_h * n_tiles_w;
   a.H = H;
   a.W = W;
   a.write_empty = flags & 1;
@@ -1047,6 +1090,7 @@ extern "C" int gsmi_tile_based_vol_rendering_backward_sh(
   a.thresh = thresh;
   a.n_tiles_h = n_tiles_h;
   a.n_tiles_w = n_tiles_w;
+  a.n_tiles_total = n_tiles_h * n_tiles_w;
   a.H = H;
   a.W = W;
   hipStream_t st = (hipStream_t)stream;
@@ -1091,6 +1135,7 @@ extern "C" int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, con
   a.thresh = thresh;
   a.n_tiles_h = n_tiles_h;
   a.n_tiles_w = n_tiles_w;
+  a.n_tiles_total = n_tiles_h * n_tiles_w;
   a.H = H;
   a.W = W;
   a.write_empty = flags & 1;
@@ -1136,9 +1181,121 @@ extern "C" int gsmi_tile_based_vol_rendering_backward_start_end(
   a.thresh = thresh;
   a.n_tiles_h = n_tiles_h;
   a.n_tiles_w = n_tiles_w;
+  a.n_tiles_total = n_tiles_h * n_tiles_w;
   a.H = H;
   a.W = W;
   hipLaunchKernelGGL(k_bwd_rgb, dim3(xcd_grid(n_tiles_h * n_tiles_w)), dim3(256), 0, (hipStream_t)stream, a);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Multi-view batches: one launch over the tiles of every view (tile ids
+// numbered with each view's tile_base, pixels with its pix_base).  The
+// per-pixel arithmetic is the single-view kernels', so every view's image is
+// bit-identical to rendering it alone.
+// ---------------------------------------------------------------------------
+static int check_views(int64_t V, int64_t P, int B, const gsmi_view* views, int n_tiles_total) {
+  if (V < 0 || P < 0 || B <= 0 || !views || n_tiles_total <= 0) return GSMI_ERR_ARG;
+  if ((uintptr_t)views & 7) return GSMI_ERR_ARG;
+  if (n_tiles_total > (1 << 30)) return GSMI_ERR_UNSUPPORTED;
+  return GSMI_OK;
+}
+
+extern "C" int gsmi_render_views_sh(int64_t V, int64_t P, int B, const gsmi_view* views, int n_tiles_total,
+                                    const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
+                                    int64_t sh_stride_c, const int32_t* sh_index, const float* alpha,
+                                    const int32_t* start, const int32_t* end, const int32_t* gaussian_ids,
+                                    float* out, int C, float thresh, const float* bg, float* T_final,
+                                    int32_t* n_done, float* pre_bg, int flags, int exact, void* stream) {
+  int rc = check_views(V, P, B, views, n_tiles_total);
+  if (rc) return rc;
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (!start || !end || !out) return GSMI_ERR_ARG;
+  if (P > 0 && (!mean || !cov || !sh || !alpha || !gaussian_ids)) return GSMI_ERR_ARG;
+  if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = sh;
+  a.sh_index = sh_index;
+  a.alpha = alpha;
+  a.start = start;
+  a.end = end;
+  a.ids = gaussian_ids;
+  a.bg = bg;
+  a.out = out;
+  a.T_final = T_final;
+  a.n_done = n_done;
+  a.pre_bg = pre_bg;
+  a.sh_sg = sh_stride_g;
+  a.sh_sc = sh_stride_c;
+  a.thresh = thresh;
+  a.write_empty = flags & 1;
+  a.views = views;
+  a.n_views = B;
+  a.n_tiles_total = n_tiles_total;
+  hipStream_t st = (hipStream_t)stream;
+  const bool G = bg != nullptr;
+  switch (C) {
+    case 1: return G ? launch_fwd_sh<1, true>(a, exact, st) : launch_fwd_sh<1, false>(a, exact, st);
+    case 2: return G ? launch_fwd_sh<2, true>(a, exact, st) : launch_fwd_sh<2, false>(a, exact, st);
+    case 3: return G ? launch_fwd_sh<3, true>(a, exact, st) : launch_fwd_sh<3, false>(a, exact, st);
+    default: return G ? launch_fwd_sh<4, true>(a, exact, st) : launch_fwd_sh<4, false>(a, exact, st);
+  }
+}
+
+extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view* views,
+                                             int n_tiles_total, const float* mean, const float* cov,
+                                             const float* sh, int64_t sh_stride_g, int64_t sh_stride_c,
+                                             const int32_t* sh_index, const float* alpha, const int32_t* start,
+                                             const int32_t* end, const int32_t* gaussian_ids, const float* out,
+                                             const float* T_final, const int32_t* n_done, const float* pre_bg,
+                                             const float* grad_out, float* grad_mean, float* grad_cov,
+                                             float* grad_sh, float* grad_alpha, int C, float thresh,
+                                             const float* bg, void* stream) {
+  int rc = check_views(V, P, B, views, n_tiles_total);
+  if (rc) return rc;
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (!start || !end || !T_final || !n_done || !grad_out) return GSMI_ERR_ARG;
+  if (P > 0 && (!mean || !cov || !sh || !alpha || !gaussian_ids || !grad_mean || !grad_cov || !grad_sh ||
+                !grad_alpha))
+    return GSMI_ERR_ARG;
+  if (bg && (!out || !pre_bg)) return GSMI_ERR_ARG;
+  if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = sh;
+  a.sh_index = sh_index;
+  a.alpha = alpha;
+  a.start = start;
+  a.end = end;
+  a.ids = gaussian_ids;
+  a.bg = bg;
+  a.out_saved = out;
+  a.T_saved = T_final;
+  a.n_done_saved = n_done;
+  a.pre_bg_saved = pre_bg;
+  a.grad_out = grad_out;
+  a.grad_mean = grad_mean;
+  a.grad_cov = grad_cov;
+  a.grad_sh = grad_sh;
+  a.grad_alpha = grad_alpha;
+  a.sh_sg = sh_stride_g;
+  a.sh_sc = sh_stride_c;
+  a.thresh = thresh;
+  a.views = views;
+  a.n_views = B;
+  a.n_tiles_total = n_tiles_total;
+  hipStream_t st = (hipStream_t)stream;
+  const bool G = bg != nullptr;
+  switch (C) {
+    case 1: return G ? launch_bwd_sh<1, true>(a, st) : launch_bwd_sh<1, false>(a, st);
+    case 2: return G ? launch_bwd_sh<2, true>(a, st) : launch_bwd_sh<2, false>(a, st);
+    case 3: return G ? launch_bwd_sh<3, true>(a, st) : launch_bwd_sh<3, false>(a, st);
+    default: return G ? launch_bwd_sh<4, true>(a, st) : launch_bwd_sh<4, false>(a, st);
+  }
 }

@@ -139,6 +139,19 @@ inline int xcd_grid(int n_tiles) { return ((n_tiles + 7) >> 3) << 3; }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+static_assert(sizeof(gsmi_view) == 256, "gsmi_view is a 256-byte record");
+
+// View owning global tile `tile`: the largest b with views[b].tile_base <= tile.
+__device__ __forceinline__ int view_of_tile(const gsmi_view* __restrict__ views, int n_views, int tile) {
+  int lo = 0, hi = n_views - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (views[mid].tile_base <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 }  // namespace gsmi
 
 // Launch helper: returns GSMI_ERR_LAUNCH on a launch failure.

@@ -4,26 +4,10 @@
 // key follows the reference's operation order (gs/renderer.py:365-419,
 // gs/culling.py:8-37, culling.h:11-20) with the specified exponential, so
 // masks, tile bounds, depths and keys are bit-identical to the oracle's.
+#include "project.h"
 #include "scan.h"
 
 namespace gsmi {
-
-struct Planes {
-  float n[18];
-  float p[18];
-};
-
-// kernels.h:156-170 intersect_sphere_frustum.
-__device__ __forceinline__ bool in_frustum(float mx, float my, float mz, float r, const float* __restrict__ nrm,
-                                           const float* __restrict__ pts) {
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const float dx = mx - pts[3 * k], dy = my - pts[3 * k + 1], dz = mz - pts[3 * k + 2];
-    const float d = dx * nrm[3 * k] + dy * nrm[3 * k + 1] + dz * nrm[3 * k + 2];
-    if (!(d > -r)) return false;
-  }
-  return true;
-}
 
 __global__ void k_cull(int64_t N, const float* __restrict__ mean, const float* __restrict__ svec,
                        const float* __restrict__ nrm, const float* __restrict__ pts, uint8_t* __restrict__ mask,
@@ -32,100 +16,6 @@ __global__ void k_cull(int64_t N, const float* __restrict__ mean, const float* _
   if (i >= N) return;
   const float r = fmaxf(fmaxf(svec[3 * i], svec[3 * i + 1]), svec[3 * i + 2]) * thresh;
   mask[i] = in_frustum(mean[3 * i], mean[3 * i + 1], mean[3 * i + 2], r, nrm, pts) ? 1 : 0;
-}
-
-// utils/transforms.py:31-45 (kornia WXYZ): normalise, then the unit
-// quaternion rotation matrix, row-major.
-__device__ __forceinline__ void quat_to_rot(float4 q, float* R) {
-  const float nrm = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
-  const float den = fmaxf(nrm, 1e-12f);
-  const float w = q.x / den, x = q.y / den, y = q.z / den, z = q.w / den;
-  const float tx = 2.0f * x, ty = 2.0f * y, tz = 2.0f * z;
-  const float twx = tx * w, twy = ty * w, twz = tz * w;
-  const float txx = tx * x, txy = ty * x, txz = tz * x;
-  const float tyy = ty * y, tyz = tz * y, tzz = tz * z;
-  R[0] = 1.0f - (tyy + tzz);
-  R[1] = txy - twz;
-  R[2] = txz + twy;
-  R[3] = txy + twz;
-  R[4] = 1.0f - (txx + tzz);
-  R[5] = tyz - twx;
-  R[6] = txz - twy;
-  R[7] = tyz + twx;
-  R[8] = 1.0f - (txx + tyy);
-}
-
-struct Proj {
-  float mx, my;  // mean2d
-  float c[4];    // cov2d row-major
-  float depth;
-};
-
-// gs/renderer.py:380-419 with c2w = [Rc | t]: u = Rc^T (mu - t),
-// Sigma = (R(q) diag s)(R(q) diag s)^T, A = J Rc^T (J detached),
-// cov2d = (A Sigma A^T)[:2,:2], depth = u_z, mean2d = u_xy / depth.
-__device__ __forceinline__ Proj project_one(float m0, float m1, float m2, float4 q, float s0, float s1, float s2,
-                                            const float* __restrict__ cw) {
-  float Wm[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) Wm[3 * i + j] = cw[4 * j + i];
-  const float d0 = m0 + (-cw[3]), d1 = m1 + (-cw[7]), d2 = m2 + (-cw[11]);
-  float u[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) u[i] = Wm[3 * i + 0] * d0 + Wm[3 * i + 1] * d1 + Wm[3 * i + 2] * d2;
-  float R[9], M[9], S[9];
-  quat_to_rot(q, R);
-  const float s[3] = {s0, s1, s2};
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) M[3 * i + j] = s[j] * R[3 * i + j];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-      S[3 * a + b] = M[3 * a + 0] * M[3 * b + 0] + M[3 * a + 1] * M[3 * b + 1] + M[3 * a + 2] * M[3 * b + 2];
-  const float x = u[0], y = u[1], z = u[2];
-  const float J00 = 1.0f / z, J02 = -x / z / z;
-  const float J11 = 1.0f / z, J12 = -y / z / z;
-  float A[6];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    A[k] = J00 * Wm[0 * 3 + k] + J02 * Wm[2 * 3 + k];
-    A[3 + k] = J11 * Wm[1 * 3 + k] + J12 * Wm[2 * 3 + k];
-  }
-  float T1[6];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-      T1[3 * i + b] = A[3 * i + 0] * S[0 * 3 + b] + A[3 * i + 1] * S[1 * 3 + b] + A[3 * i + 2] * S[2 * 3 + b];
-  Proj p;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-      p.c[2 * i + j] = T1[3 * i + 0] * A[3 * j + 0] + T1[3 * i + 1] * A[3 * j + 1] + T1[3 * i + 2] * A[3 * j + 2];
-  p.depth = z;
-  p.mx = x / z;
-  p.my = y / z;
-  return p;
-}
-
-// gs/culling.py:16-31 with utils/camera.py:296-301 for one axis.
-__device__ __forceinline__ void aabb_axis(float m, float cii, float D, float f, float c, int lim, int tile,
-                                          int* lo, int* hi) {
-  const float e = sqrtf(D * cii);
-  const float pl = (m - e) * f;
-  const float ph = (m + e) * f;
-  int il = f2i_sat(pl + c);
-  int ih = f2i_sat(ph + c);
-  il = il < 0 ? 0 : (il > lim ? lim : il);
-  ih = ih < 0 ? 0 : (ih > lim ? lim : ih);
-  *lo = il / tile;
-  *hi = ih / tile;
 }
 
 struct CamArgs {
@@ -378,6 +268,153 @@ extern "C" int gsmi_preprocess_emit(int64_t N, const float* mean, const float* q
   const CamArgs c = make_cam(c2w_host, fx, fy, cx, cy, W, H, D, 0.0f, 0, tile_size);
   hipLaunchKernelGGL(k_pre_emit, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, (hipStream_t)stream, N, mean,
                      qvec, log_s, logit, c, vis, vis_off, mean2d, cov2d, depth, alpha, tl, br, orig);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Multi-view batches: B cameras (gsmi_view records in device memory) over the
+// same N Gaussians.  Entry (b, i) of the [B, N] arrays is Gaussian i seen by
+// view b; one exclusive scan over the flattened array numbers the visible
+// entries view-major, so every view's records are contiguous and the whole
+// batch goes through one binning and one compositing launch.  Per entry the
+// arithmetic is the single-view kernels' (same functions, same order), so
+// every view's masks, records and keys equal its single-view ones.
+// ---------------------------------------------------------------------------
+namespace gsmi {
+
+__global__ __launch_bounds__(256) void k_mv_count(int64_t N, int B, const float* __restrict__ mean,
+                                                  const float* __restrict__ qvec, const float* __restrict__ log_s,
+                                                  const gsmi_view* __restrict__ views, float D,
+                                                  float frustum_radius, int skip_culling, uint8_t* __restrict__ vis,
+                                                  Int2* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float m0 = mean[3 * i], m1 = mean[3 * i + 1], m2 = mean[3 * i + 2];
+  const float s0 = exp_spec(log_s[3 * i]), s1 = exp_spec(log_s[3 * i + 1]), s2 = exp_spec(log_s[3 * i + 2]);
+  const float4 q = reinterpret_cast<const float4*>(qvec)[i];
+  const float r = fmaxf(fmaxf(s0, s1), s2) * frustum_radius;
+  for (int b = 0; b < B; ++b) {
+    const gsmi_view& v = views[b];
+    const bool keep = skip_culling || in_frustum(m0, m1, m2, r, v.normals, v.points);
+    int pairs = 0;
+    if (keep) {
+      const Proj p = project_one(m0, m1, m2, q, s0, s1, s2, v.c2w);
+      int lx, hx, ly, hy;
+      aabb_axis(p.mx, p.c[0], D, v.fx, v.cx, v.W - 1, kTile, &lx, &hx);
+      aabb_axis(p.my, p.c[3], D, v.fy, v.cy, v.H - 1, kTile, &ly, &hy);
+      pairs = (hx - lx + 1) * (hy - ly + 1);
+    }
+    const int64_t e = (int64_t)b * N + i;
+    vis[e] = keep ? 1 : 0;
+    cnt[e] = Int2{keep ? 1 : 0, pairs};
+  }
+}
+
+// Visible offsets (x of the scanned counts) and per-view bases:
+// bases[2b] / bases[2b+1] = first visible entry / first pair of view b,
+// bases[2B] / bases[2B+1] = totals.
+__global__ void k_mv_offsets(const Int2* __restrict__ cnt, int64_t N, int B, const Int2* __restrict__ total,
+                             int32_t* __restrict__ vis_off, int64_t* __restrict__ bases) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < (int64_t)B * N) vis_off[e] = cnt[e].x;
+  if (e <= B) {
+    const Int2 o = e < B ? (N > 0 ? cnt[e * N] : *total) : *total;
+    bases[2 * e] = o.x;
+    bases[2 * e + 1] = o.y;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mv_emit(int64_t N, int B, const float* __restrict__ mean,
+                                                 const float* __restrict__ qvec, const float* __restrict__ log_s,
+                                                 const float* __restrict__ logit,
+                                                 const gsmi_view* __restrict__ views, float D,
+                                                 const uint8_t* __restrict__ vis, const int32_t* __restrict__ voff,
+                                                 float* __restrict__ mean2d, float* __restrict__ cov2d,
+                                                 float* __restrict__ depth, float* __restrict__ alpha,
+                                                 int32_t* __restrict__ tl, int32_t* __restrict__ br,
+                                                 int32_t* __restrict__ orig) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  bool any = false;
+  for (int b = 0; b < B; ++b) any = any || vis[(int64_t)b * N + i];
+  if (!any) return;
+  const float m0 = mean[3 * i], m1 = mean[3 * i + 1], m2 = mean[3 * i + 2];
+  const float s0 = exp_spec(log_s[3 * i]), s1 = exp_spec(log_s[3 * i + 1]), s2 = exp_spec(log_s[3 * i + 2]);
+  const float4 q = reinterpret_cast<const float4*>(qvec)[i];
+  const float al = 1.0f / (1.0f + exp_spec(-logit[i]));
+  for (int b = 0; b < B; ++b) {
+    const int64_t e = (int64_t)b * N + i;
+    if (!vis[e]) continue;
+    const gsmi_view& v = views[b];
+    const int64_t o = voff[e];
+    const Proj p = project_one(m0, m1, m2, q, s0, s1, s2, v.c2w);
+    int lx, hx, ly, hy;
+    aabb_axis(p.mx, p.c[0], D, v.fx, v.cx, v.W - 1, kTile, &lx, &hx);
+    aabb_axis(p.my, p.c[3], D, v.fy, v.cy, v.H - 1, kTile, &ly, &hy);
+    reinterpret_cast<float2*>(mean2d)[o] = make_float2(p.mx, p.my);
+    reinterpret_cast<float4*>(cov2d)[o] = make_float4(p.c[0], p.c[1], p.c[2], p.c[3]);
+    depth[o] = p.depth;
+    alpha[o] = al;
+    reinterpret_cast<int2*>(tl)[o] = make_int2(lx, ly);
+    reinterpret_cast<int2*>(br)[o] = make_int2(hx, hy);
+    orig[o] = (int32_t)i;
+  }
+}
+
+}  // namespace gsmi
+
+extern "C" size_t gsmi_preprocess_views_workspace_size(int64_t N, int B) {
+  if (N < 0) N = 0;
+  if (B < 0) B = 0;
+  const int64_t n = N * B;
+  return align_up(n * 8) + align_up(scan_workspace_bytes(n, 8)) + 256;
+}
+
+extern "C" int gsmi_preprocess_views_count(int64_t N, int B, const gsmi_view* views, const float* mean,
+                                           const float* qvec, const float* log_s, float D, float frustum_radius,
+                                           int skip_culling, uint8_t* vis, int32_t* vis_off, int64_t* bases,
+                                           void* workspace, size_t workspace_bytes, void* stream) {
+  if (N < 0 || B <= 0 || !views || !bases) return GSMI_ERR_ARG;
+  if (N > 0 && (!mean || !qvec || !log_s || !vis || !vis_off)) return GSMI_ERR_ARG;
+  if (N * B >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
+  if (((uintptr_t)qvec & 15) != 0 || ((uintptr_t)views & 7) != 0) return GSMI_ERR_ARG;
+  if (!workspace || workspace_bytes < gsmi_preprocess_views_workspace_size(N, B)) return GSMI_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = N * B;
+  char* p = reinterpret_cast<char*>(workspace);
+  Int2* cnt = reinterpret_cast<Int2*>(p);
+  p += align_up(n * 8);
+  void* scan_ws = p;
+  p += align_up(scan_workspace_bytes(n, 8));
+  Int2* total = reinterpret_cast<Int2*>(p);
+  if (N > 0) {
+    hipLaunchKernelGGL(k_mv_count, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, B, mean, qvec, log_s,
+                       views, D, frustum_radius, skip_culling, vis, cnt);
+    GSMI_CHECK_LAUNCH();
+  }
+  int rc = exclusive_scan<Int2>(cnt, cnt, n, total, scan_ws, st);
+  if (rc) return rc;
+  const int64_t m = std::max<int64_t>(n, B + 1);
+  hipLaunchKernelGGL(k_mv_offsets, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, cnt, N, B, total, vis_off,
+                     bases);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
+extern "C" int gsmi_preprocess_views_emit(int64_t N, int B, const gsmi_view* views, const float* mean,
+                                          const float* qvec, const float* log_s, const float* logit, float D,
+                                          const uint8_t* vis, const int32_t* vis_off, float* mean2d, float* cov2d,
+                                          float* depth, float* alpha, int32_t* tl, int32_t* br, int32_t* orig,
+                                          void* stream) {
+  if (N < 0 || B <= 0 || !views) return GSMI_ERR_ARG;
+  if (N == 0) return GSMI_OK;
+  if (!mean || !qvec || !log_s || !logit || !vis || !vis_off) return GSMI_ERR_ARG;
+  if (((uintptr_t)qvec & 15) != 0 || ((uintptr_t)cov2d & 15) != 0 || ((uintptr_t)mean2d & 7) != 0 ||
+      ((uintptr_t)tl & 7) != 0 || ((uintptr_t)br & 7) != 0 || ((uintptr_t)views & 7) != 0)
+    return GSMI_ERR_ARG;
+  hipLaunchKernelGGL(k_mv_emit, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, (hipStream_t)stream, N, B, mean,
+                     qvec, log_s, logit, views, D, vis, vis_off, mean2d, cov2d, depth, alpha, tl, br, orig);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
 }

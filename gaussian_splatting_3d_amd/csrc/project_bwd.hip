@@ -12,28 +12,10 @@ struct Cam12 {
   float cw[12];
 };
 
-__global__ __launch_bounds__(256) void k_project_bwd(int64_t N, const float* __restrict__ mean,
-                                                     const float* __restrict__ qvec,
-                                                     const float* __restrict__ log_s,
-                                                     const float* __restrict__ logit, Cam12 cam,
-                                                     const uint8_t* __restrict__ vis,
-                                                     const int32_t* __restrict__ voff,
-                                                     const float* __restrict__ g_m2d,
-                                                     const float* __restrict__ g_cov,
-                                                     const float* __restrict__ g_alpha, float* __restrict__ o_mean,
-                                                     float* __restrict__ o_q, float* __restrict__ o_ls,
-                                                     float* __restrict__ o_logit) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  if (!vis[i]) {
-    o_mean[3 * i] = o_mean[3 * i + 1] = o_mean[3 * i + 2] = 0.0f;
-    reinterpret_cast<float4*>(o_q)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    o_ls[3 * i] = o_ls[3 * i + 1] = o_ls[3 * i + 2] = 0.0f;
-    o_logit[i] = 0.0f;
-    return;
-  }
-  const int64_t v = voff[i];
-  const float* cw = cam.cw;
+// Per-view terms of the adjoint at raw Gaussian i: adds dL/dmu (3) and
+// dL/dSigma (3x3) for the compacted record v seen through camera cw.
+__device__ __forceinline__ void view_terms(const float* __restrict__ mean, int64_t i, const float* cw,
+                                           float2 gm, float4 G, float* __restrict__ g_mean, float* __restrict__ gS) {
   // W = Rc^T, u = W (mu - t)
   float Wm[9];
 #pragma unroll
@@ -46,13 +28,10 @@ __global__ __launch_bounds__(256) void k_project_bwd(int64_t N, const float* __r
   for (int a = 0; a < 3; ++a) u[a] = Wm[3 * a] * d0 + Wm[3 * a + 1] * d1 + Wm[3 * a + 2] * d2;
   const float z = u[2];
   const float iz = 1.0f / z;
-
   // mean: dL/du = (g.x / z, g.y / z, 0); dL/dmu = W^T dL/du.
-  const float2 gm = reinterpret_cast<const float2*>(g_m2d)[v];
   const float gu0 = gm.x * iz, gu1 = gm.y * iz;
 #pragma unroll
-  for (int b = 0; b < 3; ++b) o_mean[3 * i + b] = Wm[b] * gu0 + Wm[3 + b] * gu1;
-
+  for (int b = 0; b < 3; ++b) g_mean[b] += Wm[b] * gu0 + Wm[3 + b] * gu1;
   // A = J W (first two rows), J00 = J11 = 1/z, J02 = -x/z^2, J12 = -y/z^2.
   const float J02 = -u[0] * iz * iz, J12 = -u[1] * iz * iz;
   float A[6];
@@ -62,20 +41,24 @@ __global__ __launch_bounds__(256) void k_project_bwd(int64_t N, const float* __r
     A[3 + k] = iz * Wm[3 + k] + J12 * Wm[6 + k];
   }
   // dL/dSigma = A^T G A with G the 2x2 cov2d gradient (all four entries).
-  const float4 G = reinterpret_cast<const float4*>(g_cov)[v];
   const float Gm[4] = {G.x, G.y, G.z, G.w};
   float GA[6];  // G A : 2x3
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int b = 0; b < 3; ++b) GA[3 * r + b] = Gm[2 * r] * A[b] + Gm[2 * r + 1] * A[3 + b];
-  float gS[9];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b) gS[3 * a + b] = A[a] * GA[b] + A[3 + a] * GA[3 + b];
+    for (int b = 0; b < 3; ++b) gS[3 * a + b] += A[a] * GA[b] + A[3 + a] * GA[3 + b];
+}
 
-  // Rotation and scales.
+// dL/dSigma and dL/dalpha -> gradients of qvec, log-scale and logit of raw
+// Gaussian i (writes the outputs of i).
+__device__ __forceinline__ void param_chain(int64_t i, const float* __restrict__ qvec,
+                                            const float* __restrict__ log_s, const float* __restrict__ logit,
+                                            const float* gS, float g_alpha, float* __restrict__ o_q,
+                                            float* __restrict__ o_ls, float* __restrict__ o_logit) {
   const float4 q = reinterpret_cast<const float4*>(qvec)[i];
   const float nq = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
   const float den = fmaxf(nq, 1e-12f);
@@ -138,7 +121,77 @@ __global__ __launch_bounds__(256) void k_project_bwd(int64_t N, const float* __r
   for (int b = 0; b < 3; ++b) o_ls[3 * i + b] = gs[b] * s[b];
   // alpha = sigmoid(logit)
   const float al = 1.0f / (1.0f + exp_spec(-logit[i]));
-  o_logit[i] = g_alpha[v] * (al * (1.0f - al));
+  o_logit[i] = g_alpha * (al * (1.0f - al));
+}
+
+__device__ __forceinline__ void write_zero(int64_t i, float* __restrict__ o_mean, float* __restrict__ o_q,
+                                           float* __restrict__ o_ls, float* __restrict__ o_logit) {
+  o_mean[3 * i] = o_mean[3 * i + 1] = o_mean[3 * i + 2] = 0.0f;
+  reinterpret_cast<float4*>(o_q)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  o_ls[3 * i] = o_ls[3 * i + 1] = o_ls[3 * i + 2] = 0.0f;
+  o_logit[i] = 0.0f;
+}
+
+// Single view: c2w by value.
+__global__ __launch_bounds__(256) void k_project_bwd(int64_t N, const float* __restrict__ mean,
+                                                     const float* __restrict__ qvec,
+                                                     const float* __restrict__ log_s,
+                                                     const float* __restrict__ logit, Cam12 cam,
+                                                     const uint8_t* __restrict__ vis,
+                                                     const int32_t* __restrict__ voff,
+                                                     const float* __restrict__ g_m2d,
+                                                     const float* __restrict__ g_cov,
+                                                     const float* __restrict__ g_alpha, float* __restrict__ o_mean,
+                                                     float* __restrict__ o_q, float* __restrict__ o_ls,
+                                                     float* __restrict__ o_logit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (!vis[i]) {
+    write_zero(i, o_mean, o_q, o_ls, o_logit);
+    return;
+  }
+  const int64_t v = voff[i];
+  float gm[3] = {0.f, 0.f, 0.f}, gS[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  view_terms(mean, i, cam.cw, reinterpret_cast<const float2*>(g_m2d)[v], reinterpret_cast<const float4*>(g_cov)[v],
+             gm, gS);
+#pragma unroll
+  for (int b = 0; b < 3; ++b) o_mean[3 * i + b] = gm[b];
+  param_chain(i, qvec, log_s, logit, gS, g_alpha[v], o_q, o_ls, o_logit);
+}
+
+// Multi-view: the gradients of every view that sees Gaussian i are summed
+// (views in order, so the sum is deterministic) before the parameter chain,
+// which is linear in dL/dSigma and dL/dalpha.
+__global__ __launch_bounds__(256) void k_project_bwd_views(
+    int64_t N, int B, const float* __restrict__ mean, const float* __restrict__ qvec,
+    const float* __restrict__ log_s, const float* __restrict__ logit, const gsmi_view* __restrict__ views,
+    const uint8_t* __restrict__ vis, const int32_t* __restrict__ voff, const float* __restrict__ g_m2d,
+    const float* __restrict__ g_cov, const float* __restrict__ g_alpha, float* __restrict__ o_mean,
+    float* __restrict__ o_q, float* __restrict__ o_ls, float* __restrict__ o_logit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float gm[3] = {0.f, 0.f, 0.f}, gS[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float ga = 0.0f;
+  bool any = false;
+  for (int b = 0; b < B; ++b) {
+    const int64_t e = (int64_t)b * N + i;
+    if (!vis[e]) continue;
+    any = true;
+    const int64_t v = voff[e];
+    float cw[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) cw[k] = views[b].c2w[k];
+    view_terms(mean, i, cw, reinterpret_cast<const float2*>(g_m2d)[v], reinterpret_cast<const float4*>(g_cov)[v],
+               gm, gS);
+    ga += g_alpha[v];
+  }
+  if (!any) {
+    write_zero(i, o_mean, o_q, o_ls, o_logit);
+    return;
+  }
+#pragma unroll
+  for (int b = 0; b < 3; ++b) o_mean[3 * i + b] = gm[b];
+  param_chain(i, qvec, log_s, logit, gS, ga, o_q, o_ls, o_logit);
 }
 
 }  // namespace gsmi
@@ -163,6 +216,26 @@ extern "C" int gsmi_project_backward(int64_t N, const float* mean, const float* 
   hipLaunchKernelGGL(k_project_bwd, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, (hipStream_t)stream, N, mean,
                      qvec, log_s, logit, cam, vis, vis_off, grad_mean2d, grad_cov, grad_alpha, grad_mean, grad_qvec,
                      grad_log_s, grad_logit);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
+extern "C" int gsmi_project_backward_views(int64_t N, int B, const gsmi_view* views, const float* mean,
+                                           const float* qvec, const float* log_s, const float* logit,
+                                           const uint8_t* vis, const int32_t* vis_off, const float* grad_mean2d,
+                                           const float* grad_cov, const float* grad_alpha, float* grad_mean,
+                                           float* grad_qvec, float* grad_log_s, float* grad_logit, void* stream) {
+  if (N < 0 || B <= 0 || !views) return GSMI_ERR_ARG;
+  if (N == 0) return GSMI_OK;
+  if (!mean || !qvec || !log_s || !logit || !vis || !vis_off || !grad_mean2d || !grad_cov || !grad_alpha ||
+      !grad_mean || !grad_qvec || !grad_log_s || !grad_logit)
+    return GSMI_ERR_ARG;
+  if (((uintptr_t)qvec & 15) || ((uintptr_t)grad_qvec & 15) || ((uintptr_t)grad_cov & 15) ||
+      ((uintptr_t)grad_mean2d & 7) || ((uintptr_t)views & 7))
+    return GSMI_ERR_ARG;
+  hipLaunchKernelGGL(k_project_bwd_views, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, (hipStream_t)stream, N,
+                     B, mean, qvec, log_s, logit, views, vis, vis_off, grad_mean2d, grad_cov, grad_alpha, grad_mean,
+                     grad_qvec, grad_log_s, grad_logit);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
 }

@@ -32,6 +32,26 @@ enum {
   GSMI_ERR_COUNT = 5        /* pair count differs from the one the caller sized */
 };
 
+/*
+ * One camera of a multi-view batch (256 bytes; arrays of these live in device
+ * memory).  Tiles of all views are numbered consecutively: view b owns global
+ * tile ids [tile_base, tile_base + n_tiles_w * n_tiles_h); its pixels start
+ * at pix_base in the flat [sum(H*W), 3] image.  topleft = (-cx/fx, -cy/fy),
+ * pixel size = (1/fx, 1/fy) (gs/sh_renderer.py:239-248), frustum planes as
+ * CameraInfo.get_frustum (utils/camera.py:249-283).
+ */
+typedef struct gsmi_view {
+  float c2w[12]; /* row-major 3x4 camera-to-world */
+  float normals[18];
+  float points[18];
+  float fx, fy, cx, cy;
+  float topleft_x, topleft_y, pixel_size_x, pixel_size_y;
+  int32_t W, H, n_tiles_w, n_tiles_h;
+  int32_t tile_base;
+  int32_t reserved;
+  int64_t pix_base;
+} gsmi_view;
+
 /* Human-readable text for a status code (static storage). */
 const char *gsmi_status_string(int status);
 
@@ -248,6 +268,73 @@ int gsmi_project_backward(int64_t N, const float *mean, const float *qvec, const
                           const float *c2w_host, const uint8_t *vis, const int32_t *vis_off, const float *grad_mean2d,
                           const float *grad_cov, const float *grad_alpha, float *grad_mean, float *grad_qvec,
                           float *grad_log_s, float *grad_logit, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Multi-view batches                                                        */
+/* ------------------------------------------------------------------------ */
+/*
+ * No single counterpart in the reference, which renders one camera per call
+ * (SHRenderer.forward, gs/sh_renderer.py:188-316; the viewer loops over
+ * clients, utils/viewer/viser_viewer.py:118-158).  These run that same
+ * pipeline for B cameras in one pass per stage: `views` is a device array of
+ * B gsmi_view records; per-entry arrays are [B, N] (entry b * N + i =
+ * Gaussian i in view b); visible records are numbered view-major.  Each
+ * view's masks, records, tile lists and image are bit-identical to the
+ * single-view calls on that camera.
+ */
+size_t gsmi_preprocess_views_workspace_size(int64_t N, int B);
+
+/* Visibility and pair counts of every (view, Gaussian); vis [B*N],
+ * vis_off [B*N] (visible record index, valid where vis), bases (device,
+ * int64 [2*(B+1)]): bases[2b], bases[2b+1] = first record / first pair of
+ * view b; bases[2B], bases[2B+1] = totals. */
+int gsmi_preprocess_views_count(int64_t N, int B, const gsmi_view *views, const float *mean, const float *qvec,
+                                const float *log_s, float D, float frustum_radius, int skip_culling, uint8_t *vis,
+                                int32_t *vis_off, int64_t *bases, void *workspace, size_t workspace_bytes,
+                                void *stream);
+
+/* Compacted records of every visible (view, Gaussian), as
+ * gsmi_preprocess_emit; tl/br are in the view's own tile coordinates. */
+int gsmi_preprocess_views_emit(int64_t N, int B, const gsmi_view *views, const float *mean, const float *qvec,
+                               const float *log_s, const float *logit, float D, const uint8_t *vis,
+                               const int32_t *vis_off, float *mean2d, float *cov2d, float *depth, float *alpha,
+                               int32_t *tl, int32_t *br, int32_t *orig, void *stream);
+
+/* Depth-first binning of all views' records into one tile numbering (view
+ * b's tile (x, y) is tile_base + y * n_tiles_w + x); start/end [n_tiles_total].
+ * Workspace: gsmi_binning_workspace_size(V, P, n_tiles_total). */
+int gsmi_bin_views(int64_t V, int64_t P, int B, const gsmi_view *views, const int64_t *bases, int n_tiles_total,
+                   const int32_t *tl, const int32_t *br, const float *depth, int32_t *gaussian_ids,
+                   int32_t *start, int32_t *end, void *workspace, size_t workspace_bytes, void *stream);
+
+/* SH compositing of every tile of every view in one launch; out is the flat
+ * [sum(H*W), 3] image (view b at pix_base), T_final / n_done per pixel. */
+int gsmi_render_views_sh(int64_t V, int64_t P, int B, const gsmi_view *views, int n_tiles_total,
+                         const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
+                         int64_t sh_stride_c, const int32_t *sh_index, const float *alpha, const int32_t *start,
+                         const int32_t *end, const int32_t *gaussian_ids, float *out, int C, float thresh,
+                         const float *bg, float *T_final, int32_t *n_done, float *pre_bg, int flags, int exact,
+                         void *stream);
+
+/* Reverse-order adjoint of gsmi_render_views_sh; gradients of records seen
+ * in several views accumulate (SH rows through sh_index). */
+int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view *views, int n_tiles_total,
+                                  const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
+                                  int64_t sh_stride_c, const int32_t *sh_index, const float *alpha,
+                                  const int32_t *start, const int32_t *end, const int32_t *gaussian_ids,
+                                  const float *out, const float *T_final, const int32_t *n_done,
+                                  const float *pre_bg, const float *grad_out, float *grad_mean, float *grad_cov,
+                                  float *grad_sh, float *grad_alpha, int C, float thresh, const float *bg,
+                                  void *stream);
+
+/* Projection adjoint over all views: per Gaussian the gradients of every
+ * view that sees it are summed in view order, then chained to the raw
+ * parameters (overwrite; zeros where no view sees the Gaussian). */
+int gsmi_project_backward_views(int64_t N, int B, const gsmi_view *views, const float *mean, const float *qvec,
+                                const float *log_s, const float *logit, const uint8_t *vis, const int32_t *vis_off,
+                                const float *grad_mean2d, const float *grad_cov, const float *grad_alpha,
+                                float *grad_mean, float *grad_qvec, float *grad_log_s, float *grad_logit,
+                                void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Test hooks                                                                */
