@@ -22,6 +22,7 @@ _P = ctypes.c_void_p
 _I32 = ctypes.c_int
 _I64 = ctypes.c_int64
 _F = ctypes.c_float
+_D = ctypes.c_double
 _SZ = ctypes.c_size_t
 
 # name -> (restype, argtypes); mirrors include/gsmi.h
@@ -66,6 +67,14 @@ SIGNATURES = {
                                              _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _P, _P]),
     "gsmi_project_backward_views": (_I32, [_I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                            _P]),
+    "gsmi_make_view": (_I32, [_P, _D, _D, _D, _D, _I32, _I32, _D, _D, _I32, _P]),
+    "gsmi_server_create": (_I32, [_I32, _I64, _I32, _I32, _F, _F, _F, _I32, _P]),
+    "gsmi_server_add_scene": (_I32, [_P, _I64, _I32, _I64, _P, _P, _P, _P, _P, _P]),
+    "gsmi_server_submit": (_I32, [_P, _I32, _P, _D, _D, _D, _D, _I32, _I32, _D, _D, _P, _P]),
+    "gsmi_server_poll": (_I32, [_P, _P, _P, _P, _P, _P, _I32, _P]),
+    "gsmi_server_drain": (_I32, [_P, _D]),
+    "gsmi_server_stats": (_I32, [_P, _P]),
+    "gsmi_server_destroy": (_I32, [_P]),
     "gsmi_exp_spec": (_I32, [_I64, _P, _P, _P]),
 }
 

@@ -337,6 +337,62 @@ int gsmi_project_backward_views(int64_t N, int B, const gsmi_view *views, const 
                                 void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Render server (continuous batching)                                       */
+/* ------------------------------------------------------------------------ */
+/*
+ * Replaces the viewer's serial per-client render loop
+ * (utils/viewer/viser_viewer.py:118-158, vis.py:18-32: one
+ * SHRenderer.forward under no_grad per client, one at a time).  Scenes stay
+ * resident (caller-owned device tensors, registered once); camera requests
+ * are queued and a dispatcher thread batches up to max_views pending
+ * requests of one scene into one multi-view pass on one of n_slots streams,
+ * while earlier batches are still running.  All per-batch buffers come from
+ * one device arena of arena_bytes allocated by gsmi_server_create; a batch
+ * keeps the longest prefix of its views whose key/sort buffers fit (the rest
+ * wait in the queue).  Each result equals the single-view forward of that
+ * camera bit for bit.  The only entry points that allocate device memory.
+ */
+typedef struct gsmi_server gsmi_server;
+
+/* Camera record as CameraInfo + SHRenderer.forward derive it (frustum
+ * planes utils/camera.py:249-283, tile grid and pixel mapping
+ * gs/sh_renderer.py:238-248); host only, no GPU call.  tile_base and
+ * pix_base are left 0. */
+int gsmi_make_view(const float *c2w, double fx, double fy, double cx, double cy, int W, int H, double near_plane,
+                   double far_plane, int tile_size, gsmi_view *out);
+
+int gsmi_server_create(int device, int64_t arena_bytes, int max_views, int n_slots, float D, float frustum_radius,
+                       float thresh, int exact, gsmi_server **out);
+
+/* Registers a resident scene (raw SHRenderer parameters on the device; the
+ * caller keeps them alive and unchanged while requests use them). */
+int gsmi_server_add_scene(gsmi_server *s, int64_t N, int C, int64_t sh_stride_c, const float *mean,
+                          const float *qvec, const float *log_s, const float *logit, const float *sh,
+                          int *scene_id);
+
+/* Queues one camera (c2w host [3,4]); the image [H,W,3] is written to the
+ * caller's device buffer `out`, which must stay valid until the request is
+ * reported complete.  Thread-safe; returns at once. */
+int gsmi_server_submit(gsmi_server *s, int scene_id, const float *c2w, double fx, double fy, double cx, double cy,
+                       int W, int H, double near_plane, double far_plane, float *out, int64_t *request_id);
+
+/* Up to max completed requests, in completion order (arrays may be NULL):
+ * id, submit and completion time in ms since the server started, status,
+ * number of views in the batch that served it. */
+int gsmi_server_poll(gsmi_server *s, int64_t *ids, double *t_submit, double *t_done, int32_t *status,
+                     int32_t *batch_views, int max, int *n_out);
+
+/* Blocks until nothing is queued or in flight (GSMI_ERR_LAUNCH on timeout). */
+int gsmi_server_drain(gsmi_server *s, double timeout_ms);
+
+/* stats[0..5]: batches, views served, largest batch, admission splits,
+ * failed requests, arena bytes. */
+int gsmi_server_stats(gsmi_server *s, int64_t *stats);
+
+/* Drains, stops the threads, frees the arena. */
+int gsmi_server_destroy(gsmi_server *s);
+
+/* ------------------------------------------------------------------------ */
 /* Test hooks                                                                */
 /* ------------------------------------------------------------------------ */
 

@@ -88,3 +88,23 @@ def test_argument_validation_without_gpu(lib):
     rc = lib.gsmi_tile_based_vol_rendering_sh(1, 1, P, P, P, 3, 1, None, P, P, P, P, P, P, P, 8, 1, 1, 1.0, 1.0,
                                               16, 16, 1, 1e-4, None, None, None, None, 0, 0, None)
     assert rc == 2  # tile size 8
+
+
+def test_make_view_matches_python_records(lib):
+    """The server builds camera records natively; they must equal the Python
+    ones bit for bit (frustum planes, image constants), or served images
+    would differ from the per-call pipeline."""
+    import numpy as np
+
+    from gaussian_splatting_3d_amd import multiview, scenes, serving
+    from gaussian_splatting_3d_amd.camera import CameraInfo
+
+    cams = [scenes.orbit_camera(6.0, 0.1 + 0.7 * k, 1100.0 * w / 1297, 1100.0 * w / 1297, w, h, near=0.01,
+                                far=1000.0, height=0.3 * k) for k, (w, h) in
+            enumerate([(640, 480), (1280, 720), (1920, 1080), (1297, 840), (100, 70)])]
+    cams.append((scenes.plain_forward_10k(n=10).cameras[0]))
+    cams.append((cams[0][0], CameraInfo.from_fov_camera(0.9, 1.5, 333, 0.05, 50.0)))
+    for c2w, cam in cams:
+        want, _, _, _ = multiview.make_views([(c2w, cam)])
+        got = serving.make_view_record(c2w, cam)
+        assert np.array_equal(got.view(np.uint32), want[0].view(np.uint32)), (cam.w, cam.h)

@@ -1,0 +1,138 @@
+"""Render server on gfx950: every served image equals the per-call forward of
+that camera bit for bit, whether the request was served alone or batched
+with others, split by admission, or submitted while other batches were on
+the GPU; a request whose buffers can never fit completes with
+GSMI_ERR_WORKSPACE."""
+import threading
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from gaussian_splatting_3d_amd import _lib, multiview, rasterize, scenes, serving
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def scene(C, n, seed):
+    g = torch.Generator().manual_seed(seed)
+    mean = torch.randn(n, 3, generator=g) * 0.8
+    log_s = torch.randn(n, 3, generator=g) * 0.5 - 3.0
+    q = torch.randn(n, 4, generator=g)
+    q = q / q.norm(dim=-1, keepdim=True)
+    logit = torch.randn(n, generator=g)
+    sh = torch.randn(n, 3, 16, generator=g) * 0.1  # stored with C_max = 4
+    sh[:, :, 0] = torch.randn(n, 3, generator=g) * 0.6
+    return [t.to(DEV).contiguous() for t in (mean, q, log_s, logit, sh)], C
+
+
+def camera(k):
+    sizes = [(128, 96), (100, 70), (160, 128), (64, 48)]
+    w, h = sizes[k % len(sizes)]
+    return scenes.orbit_camera(4.0 + 0.1 * k, 0.37 * k, 110.0, 110.0, w, h, near=0.01, far=100.0,
+                               height=0.1 * (k % 5) - 0.2)
+
+
+def reference(p, C, c2w, cam):
+    out, _ = rasterize.forward_view(*p, c2w, cam, C, scenes.RasterConfig())
+    return out.view(cam.h, cam.w, 3)
+
+
+def test_batched_requests_match_per_call():
+    (pa, Ca), (pb, Cb) = scene(4, 6000, 1), scene(2, 3000, 2)
+    with serving.RenderServer(arena_bytes=256 << 20, max_views=8, n_slots=2) as srv:
+        sa = srv.add_scene(*pa, Ca)
+        sb = srv.add_scene(*pb, Cb)
+        reqs = []
+        for k in range(12):
+            sid, p, C = (sa, pa, Ca) if k % 3 else (sb, pb, Cb)
+            c2w, cam = camera(k)
+            rid, out = srv.submit(sid, c2w, cam)
+            reqs.append((rid, out, p, C, c2w, cam))
+        srv.drain()
+        done = {c.request_id: c for c in srv.poll()}
+        st = srv.stats()
+    assert len(done) == len(reqs) and st["failed"] == 0
+    assert st["max_batch"] > 1  # requests were actually batched
+    for rid, out, p, C, c2w, cam in reqs:
+        assert done[rid].status == 0
+        assert torch.equal(out, reference(p, C, c2w, cam)), rid
+
+
+def _phase_bytes(N, views, bases, L):
+    """Mirror of the server's per-batch arena accounting."""
+    au = lambda x: (int(x) + 255) & ~255
+    B = len(views)
+    a = au(B * 256) + au(B * N) + au(B * N * 4) + au(2 * (B + 1) * 8) + \
+        au(L.gsmi_preprocess_views_workspace_size(N, B))
+
+    def b(k):
+        V, P = int(bases[k, 0]), int(bases[k, 1])
+        nt = sum(m["n_tiles"] for m in views[:k])
+        npx = sum(m["H"] * m["W"] for m in views[:k])
+        return au(V * 8) + au(V * 16) + au(V * 4) * 2 + au(V * 8) * 2 + au(V * 4) + au(P * 4) + au(nt * 4) * 2 + \
+            au(L.gsmi_binning_workspace_size(V, P, nt)) + au(npx * 12)
+
+    return a, b
+
+
+def test_admission_splits_batches_that_do_not_fit():
+    p, C = scene(4, 6000, 3)
+    cams = [camera(k) for k in (2, 2, 2, 2)]  # four 160x128 views
+    L = _lib.load()
+    _, st = multiview.forward_views(*p, cams, C, scenes.RasterConfig())
+    a, b = _phase_bytes(6000, st.b.meta, st.b.bases, L)
+    arena = a + b(1) + 4096  # room for the count of all four views and the buffers of one
+    assert arena < a + b(2)
+    with serving.RenderServer(arena_bytes=arena, max_views=4, n_slots=1) as srv:
+        sid = srv.add_scene(*p, C)
+        reqs = [srv.submit(sid, c2w, cam) + (c2w, cam) for c2w, cam in cams]
+        srv.drain()
+        done = {c.request_id: c for c in srv.poll()}
+        stt = srv.stats()
+    assert stt["splits"] >= 1 and stt["failed"] == 0 and stt["views"] == 4
+    for rid, out, c2w, cam in reqs:
+        assert done[rid].status == 0
+        assert torch.equal(out, reference(p, C, c2w, cam))
+
+
+def test_request_that_cannot_fit_fails_cleanly():
+    p, C = scene(4, 6000, 4)
+    with serving.RenderServer(arena_bytes=64 << 10, max_views=4, n_slots=1) as srv:
+        sid = srv.add_scene(*p, C)
+        c2w, cam = camera(0)
+        rid, _ = srv.submit(sid, c2w, cam)
+        srv.drain()
+        (c,) = srv.poll()
+    assert c.request_id == rid and c.status == 4  # GSMI_ERR_WORKSPACE
+
+
+def test_continuous_submission_while_batches_run():
+    """Requests arrive from another thread while earlier batches are on the
+    GPU; every result equals its per-call render."""
+    p, C = scene(4, 20000, 5)
+    cams = [camera(k) for k in range(40)]
+    with serving.RenderServer(arena_bytes=512 << 20, max_views=16, n_slots=3) as srv:
+        sid = srv.add_scene(*p, C)
+        reqs = [None] * len(cams)
+
+        def client():
+            for k, (c2w, cam) in enumerate(cams):
+                reqs[k] = srv.submit(sid, c2w, cam)
+                time.sleep(0.0005 * (k % 3))
+
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        srv.drain()
+        done = {c.request_id: c for c in srv.poll()}
+        stt = srv.stats()
+    assert len(done) == len(cams) and stt["failed"] == 0
+    for (rid, out), (c2w, cam) in zip(reqs, cams):
+        assert done[rid].status == 0
+        assert torch.equal(out, reference(p, C, c2w, cam)), rid
+    lat = np.array([c.latency_ms for c in done.values()])
+    assert np.all(lat >= 0)
