@@ -28,23 +28,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
-WORKLOADS = ("dense_3m", "fwd_bwd_500k", "plain_forward_10k")
+WORKLOADS = ("dense_3m", "fwd_bwd_500k", "plain_forward_10k", "multiview_6m", "serve")
 
 
 def make_scene(name):
     from gaussian_splatting_3d_amd import scenes
 
     return {"dense_3m": scenes.dense_3m, "fwd_bwd_500k": scenes.fwd_bwd_500k,
-            "plain_forward_10k": scenes.plain_forward_10k}[name]()
+            "plain_forward_10k": scenes.plain_forward_10k, "multiview_6m": scenes.multiview_6m}[name]()
 
 
-def alg_bytes(P, H, W, C, V, N):
-    """Algorithmic HBM bytes per launch, per stage (FP32)."""
+def alg_bytes(P, n_pix, C, V, N):
+    """Algorithmic HBM bytes per launch, per stage (FP32); n_pix = pixels of
+    all views."""
     K = 3 * C * C
     return {
         "preprocess": 44 * N + 36 * V,
-        "composite_fwd": P * (28 + 4 * K) + 20 * H * W,
-        "composite_bwd": P * (28 + 4 * K) + P * 8 * (6 + K) + 32 * H * W,
+        "composite_fwd": P * (28 + 4 * K) + 20 * n_pix,
+        "composite_bwd": P * (28 + 4 * K) + P * 8 * (6 + K) + 32 * n_pix,
         "project_bwd": V * 24 + 44 * V + 44 * N,
     }
 
@@ -130,6 +131,63 @@ def gpu_bench(scene, steps, warmup, exact=False):
     return dict(ms_step=(t1 - t0) * 1e3 / steps, ms_fwd_only=(f1 - f0) * 1e3 / steps,
                 ms_composite_fwd=t_fwd.mean_ms(), ms_composite_bwd=t_bwd.mean_ms(), N=b.N, V=b.V, P=b.P,
                 H=cam.h, W=cam.w, C=scene.C)
+
+
+def gpu_bench_views(scene, steps, warmup):
+    """One training step over all cameras of the scene in one batch: batched
+    forward, summed L1 against seeded targets, batched adjoint.  HIP events
+    bracket the two compositing launches (on the stream they run on)."""
+    from gaussian_splatting_3d_amd import _lib, multiview, scenes
+
+    dev = torch.device("cuda:0")
+    cfg = scenes.RasterConfig()
+    params = [t.to(dev).contiguous() for t in (scene.mean, scene.qvec, scene.svec_before_activation,
+                                               scene.alpha_before_activation, scene.sh_coeffs)]
+    g = torch.Generator(device=dev).manual_seed(105)
+    n_pix = sum(cam.h * cam.w for _, cam in scene.cameras)
+    target = torch.rand(n_pix * 3, generator=g, device=dev)
+    L = _lib.load()
+    t_fwd, t_bwd = Timed(), Timed()
+    state = {"on": False}
+    orig = {k: getattr(L, k) for k in ("gsmi_render_views_sh", "gsmi_render_views_backward_sh")}
+
+    def wrap(name, timer):
+        def f(*a):
+            if not state["on"]:
+                return orig[name](*a)
+            with timer:
+                return orig[name](*a)
+        return f
+
+    L.gsmi_render_views_sh = wrap("gsmi_render_views_sh", t_fwd)
+    L.gsmi_render_views_backward_sh = wrap("gsmi_render_views_backward_sh", t_bwd)
+
+    def step():
+        out, st = multiview.forward_views(*params, scene.cameras, scene.C, cfg)
+        grad_out = torch.sign(out - target) / out.numel()
+        grads = multiview.backward_views(st, *params, grad_out)
+        return st, grads
+
+    try:
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        state["on"] = True
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st, grads = step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        state["on"] = False
+    finally:
+        for k, v in orig.items():
+            setattr(L, k, v)
+    for gr in grads:
+        assert torch.isfinite(gr).all()
+    b = st.b
+    return dict(ms_step=(t1 - t0) * 1e3 / steps, ms_composite_fwd=t_fwd.mean_ms(), ms_composite_bwd=t_bwd.mean_ms(),
+                N=b.N, V=b.V, P=b.P, n_pix=n_pix, views=b.B, C=scene.C,
+                H=scene.cameras[0][1].h, W=scene.cameras[0][1].w)
 
 
 def cpu_baseline(scene, budget_s=20.0):
@@ -275,6 +333,7 @@ def main():
                     help="only run a 1 GiB device-to-device copy 3 times (scales rocprof byte counters)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the dominant kernel's HBM bytes")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement of the default run")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.copy_calibration:
@@ -286,26 +345,39 @@ def main():
         print(json.dumps({"copy_bytes_read_per_launch": src.numel() * 4, "copy_bytes_written_per_launch": dst.numel() * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
         return
     if args.pmc_child:  # one step under a counter pass of measure_traffic
-        gpu_bench(make_scene(args.workload), 1, 1, exact=args.exact)
+        sc = make_scene(args.workload)
+        if len(sc.cameras) > 1:
+            gpu_bench_views(sc, 1, 1)
+        else:
+            gpu_bench(sc, 1, 1, exact=args.exact)
+        return
+    if args.workload == "serve":
+        serve_main(args)
         return
     scene = make_scene(args.workload)
     print(f"[bench] workload {scene.name}: N={scene.N}", file=sys.stderr, flush=True)
-    r = gpu_bench(scene, args.steps, args.warmup, exact=args.exact)
+    views = len(scene.cameras)
+    if views > 1:
+        r = gpu_bench_views(scene, args.steps, args.warmup)
+    else:
+        r = gpu_bench(scene, args.steps, args.warmup, exact=args.exact)
+        r["n_pix"], r["views"] = r["H"] * r["W"], 1
     print(f"[bench] gpu done: {r}", file=sys.stderr, flush=True)
-    ab = alg_bytes(r["P"], r["H"], r["W"], r["C"], r["V"], r["N"])
+    ab = alg_bytes(r["P"], r["n_pix"], r["C"], r["V"], r["N"])
     dom = "composite_bwd" if r["ms_composite_bwd"] >= r["ms_composite_fwd"] else "composite_fwd"
     t_dom = r["ms_" + dom] / 1e3
     achieved = ab[dom] / t_dom / 1e9
-    mpix = r["H"] * r["W"] / (r["ms_step"] / 1e3) / 1e6
+    mpix = r["n_pix"] / (r["ms_step"] / 1e3) / 1e6
+    metric = {"dense_3m": "fwd+bwd Mpix/s @3M Gaussians 1080p",
+              "multiview_6m": "fwd+bwd Mpix/s @6M Gaussians, 8 views 3840x2160"}.get(
+        args.workload, f"fwd+bwd Mpix/s ({args.workload})")
     line = {
-        "metric": "fwd+bwd Mpix/s @3M Gaussians 1080p" if args.workload == "dense_3m" else
-        f"fwd+bwd Mpix/s ({args.workload})",
+        "metric": metric,
         "value": round(mpix, 3),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -318,8 +390,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scene and target, random-init Gaussians)",
         "config": {"workload": scene.name, "gaussians": r["N"], "visible": r["V"], "pairs": r["P"],
-                   "width": r["W"], "height": r["H"], "sh_degree": r["C"] - 1, "views": 1},
-        "fwd_only_mpix_s": round(r["H"] * r["W"] / (r["ms_fwd_only"] / 1e3) / 1e6, 3),
+                   "width": r["W"], "height": r["H"], "sh_degree": r["C"] - 1, "views": r["views"]},
         "gaussians_per_s_fwd_bwd": round(r["N"] / (r["ms_step"] / 1e3), 1),
         "kernels_ms": {"composite_fwd": round(r["ms_composite_fwd"], 4),
                        "composite_bwd": round(r["ms_composite_bwd"], 4)},
@@ -327,6 +398,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
                      "alg_bytes_per_launch": int(ab[dom])},
     }
+    if "ms_fwd_only" in r:
+        line["fwd_only_mpix_s"] = round(r["n_pix"] / (r["ms_fwd_only"] / 1e3) / 1e6, 3)
     if not args.no_traffic:
         try:
             tr = measure_traffic(args.workload, args.exact, dom)
@@ -336,13 +409,43 @@ def main():
         except Exception as e:  # counters unavailable: traffic stays null, say why
             line["roofline"]["traffic_error"] = str(e)
         print(f"[bench] traffic: {line['roofline'].get('traffic')}", file=sys.stderr, flush=True)
-    if not args.no_cpu_baseline:
+    if args.workload == "dense_3m" and not args.no_serving:
+        from gaussian_splatting_3d_amd import loadgen
+
+        sv = loadgen.serving_benchmark(log=lambda m: print(m, file=sys.stderr, flush=True))
+        line["serving"] = sv
+    if not args.no_cpu_baseline and views == 1:
         cb = cpu_baseline(scene)
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
 
+
+def serve_main(args):
+    """Served views/s and p50/p99 latency of the render server against the
+    plain per-call renderer on the same seeded request stream."""
+    from gaussian_splatting_3d_amd import loadgen
+
+    sv = loadgen.serving_benchmark(log=lambda m: print(m, file=sys.stderr, flush=True))
+    line = {
+        "metric": "served views/s at the plain renderer's p99 (1M+4M SH3 scenes, mixed 480p-1080p)",
+        "value": sv["served_views_s_at_plain_p99"],
+        "unit": "views/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scenes, Poisson arrivals, random orbit cameras)",
+        "config": {"workload": "serve_1m_4m_mixed_res", **sv["config"]},
+        "serving": sv,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
 
 if __name__ == "__main__":
     main()

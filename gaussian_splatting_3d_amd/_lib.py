@@ -67,6 +67,9 @@ SIGNATURES = {
                                              _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _P, _P]),
     "gsmi_project_backward_views": (_I32, [_I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                            _P]),
+    "gsmi_loss_workspace_size": (_SZ, [_I64]),
+    "gsmi_photometric_loss": (_I32, [_I64, _P, _P, _I32, _P, _P, _P, _SZ, _P]),
+    "gsmi_adam_step": (_I32, [_I32, _P, _P, _P, _P, _P, _P, _I32, _D, _D, _D, _I32, _P]),
     "gsmi_make_view": (_I32, [_P, _D, _D, _D, _D, _I32, _I32, _D, _D, _I32, _P]),
     "gsmi_server_create": (_I32, [_I32, _I64, _I32, _I32, _F, _F, _F, _I32, _P]),
     "gsmi_server_add_scene": (_I32, [_P, _I64, _I32, _I64, _P, _P, _P, _P, _P, _P]),

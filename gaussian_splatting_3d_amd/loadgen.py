@@ -1,0 +1,166 @@
+"""Seeded request streams and the serving measurement (views/s, p50/p99).
+
+Workload: resident scenes (built as the 500k forward+backward scene, at 1M
+and 4M Gaussians), Poisson arrivals at a fixed rate, cameras on a random
+radius-6 orbit, resolution uniform over 640x480 / 1280x720 / 1920x1080 with
+the 1297x840 camera's field of view (fx = fy = 1100 * W / 1297).
+
+Two renderers serve the same stream:
+  * plain per-call: ``rasterize.forward_view`` for one request at a time,
+    synchronised (what the reference's viewer does per client,
+    utils/viewer/viser_viewer.py:118-158).  Its service time per request is
+    measured on the GPU; being a single FIFO server, its latency under a
+    given arrival sequence follows exactly from those service times
+    (start = max(arrival, previous finish)).
+  * RenderServer: requests submitted in real time at their arrival instants
+    from a client thread; latency = completion - submission, both stamped
+    by the native server.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import rasterize, scenes, serving
+from .camera import CameraInfo, get_c2w_from_up_and_look_at
+
+RESOLUTIONS = ((640, 480), (1280, 720), (1920, 1080))
+
+
+@dataclass
+class Req:
+    t: float  # arrival, seconds from the stream start
+    scene: int
+    c2w: torch.Tensor
+    cam: CameraInfo
+
+
+def request_stream(n, qps, n_scenes, seed):
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / qps, n)
+    t = np.cumsum(gaps) - gaps[0]
+    out = []
+    for k in range(n):
+        W, H = RESOLUTIONS[rng.integers(len(RESOLUTIONS))]
+        theta = rng.uniform(0, 2 * math.pi)
+        height = rng.uniform(-1.5, 1.5)
+        pos = np.array([6.0 * math.cos(theta), 6.0 * math.sin(theta), height])
+        c2w = torch.from_numpy(get_c2w_from_up_and_look_at(np.array([0.0, 0.0, 1.0]), np.zeros(3), pos))
+        f = 1100.0 * W / 1297.0
+        out.append(Req(float(t[k]), int(rng.integers(n_scenes)), c2w, CameraInfo(f, f, W / 2, H / 2, W, H, 0.01,
+                                                                                 1000.0)))
+    return out
+
+
+def plain_service_times(scene_params, reqs, cfg):
+    """Service time (s) of each request on the plain per-call renderer."""
+    out = []
+    for r in reqs:
+        p, C = scene_params[r.scene]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        img, _ = rasterize.forward_view(*p, r.c2w, r.cam, C, cfg)
+        torch.cuda.synchronize()
+        out.append(time.perf_counter() - t0)
+    return np.array(out)
+
+
+def fifo_latencies(arrivals, service):
+    lat = np.empty_like(service)
+    free = 0.0
+    for k, (a, s) in enumerate(zip(arrivals, service)):
+        start = max(a, free)
+        free = start + s
+        lat[k] = free - a
+    return lat
+
+
+def run_server(srv, scene_ids, reqs):
+    """Submits reqs at their arrival instants; returns per-request
+    (latency ms, batch views) and the served views/s."""
+    t0 = time.perf_counter()
+    pending = {}
+    done = []
+    for r in reqs:
+        while True:  # wait for the arrival instant, polling completions meanwhile
+            now = time.perf_counter() - t0
+            if now >= r.t:
+                break
+            done += srv.poll()
+            if r.t - now > 0.002:
+                time.sleep(0.001)
+        rid, out = srv.submit(scene_ids[r.scene], r.c2w, r.cam)
+        pending[rid] = out
+    srv.drain()
+    done += srv.poll()
+    for c in done:
+        pending.pop(c.request_id, None)
+    assert not pending and all(c.status == 0 for c in done), "served requests failed"
+    lat = np.array([c.latency_ms for c in done])
+    span = (max(c.t_done_ms for c in done) - min(c.t_submit_ms for c in done)) / 1e3
+    return dict(views_s=len(done) / span, p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                mean_batch=float(np.mean([c.batch_views for c in done])), n=len(done))
+
+
+def serving_benchmark(scene_sizes=(1_000_000, 4_000_000), seed=7, n_plain=48, duration_s=2.0, max_views=32,
+                      n_slots=3, arena_bytes=64 << 30, levels=(1.0, 2.0, 3.0, 4.0, 6.0), log=print):
+    """Plain per-call capacity and latency, then the server at Poisson rates
+    that are multiples of the plain capacity.  Headline: the highest served
+    rate whose p99 stays within the plain renderer's p99 at 70% of its
+    capacity (its usual operating point), against 70% of the plain capacity."""
+    cfg = scenes.RasterConfig()
+    dev = torch.device("cuda:0")
+    params = []
+    for k, n in enumerate(scene_sizes):
+        s = scenes.serving_scene(n, seed=4 + k)
+        params.append(([t.to(dev).contiguous() for t in (s.mean, s.qvec, s.svec_before_activation,
+                                                         s.alpha_before_activation, s.sh_coeffs)], s.C))
+    # plain renderer: warm-up, then measured service times
+    warm = request_stream(6, 100.0, len(params), seed + 1)
+    plain_service_times(params, warm, cfg)
+    probe = request_stream(n_plain, 100.0, len(params), seed + 2)
+    svc = plain_service_times(params, probe, cfg)
+    cap = 1.0 / float(svc.mean())
+    log(f"[serve] plain per-call: {cap:.1f} views/s, service p50 {1e3 * np.percentile(svc, 50):.2f} ms "
+        f"p99 {1e3 * np.percentile(svc, 99):.2f} ms")
+    q70 = 0.7 * cap
+    reqs70 = request_stream(max(200, int(q70 * 4 * duration_s)), q70, len(params), seed + 3)
+    svc70 = plain_service_times(params, reqs70, cfg)
+    lat70 = fifo_latencies(np.array([r.t for r in reqs70]), svc70) * 1e3
+    slo = float(np.percentile(lat70, 99))
+    plain = dict(capacity_views_s=round(cap, 2), views_s_at_70pct=round(q70, 2), p50_ms=round(float(np.percentile(
+        lat70, 50)), 3), p99_ms=round(slo, 3))
+    log(f"[serve] plain at 70% load: p50 {plain['p50_ms']} ms p99 {slo:.2f} ms")
+    results = []
+    with serving.RenderServer(arena_bytes=arena_bytes, max_views=max_views, n_slots=n_slots, cfg=cfg) as srv:
+        ids = [srv.add_scene(*p, C) for p, C in params]
+        run_server(srv, ids, request_stream(16, 200.0, len(params), seed + 4))  # warm-up
+        for lv in levels:
+            q = lv * cap
+            n = int(max(64, min(2000, q * duration_s)))
+            r = run_server(srv, ids, request_stream(n, q, len(params), seed + 10 + int(lv * 10)))
+            r["offered_qps"] = round(q, 2)
+            results.append(r)
+            log(f"[serve] offered {q:.1f}/s: served {r['views_s']:.1f} views/s p50 {r['p50_ms']:.2f} ms "
+                f"p99 {r['p99_ms']:.2f} ms mean batch {r['mean_batch']:.1f}")
+            if r["p99_ms"] > 20 * slo:
+                break
+        stats = srv.stats()
+    ok = [r for r in results if r["p99_ms"] <= slo]
+    best = max(ok, key=lambda r: r["views_s"]) if ok else None
+    return dict(
+        plain=plain,
+        server_levels=[{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
+        served_views_s_at_plain_p99=round(best["views_s"], 2) if best else None,
+        served_p50_ms=round(best["p50_ms"], 3) if best else None,
+        served_p99_ms=round(best["p99_ms"], 3) if best else None,
+        ratio_vs_plain_at_equal_p99=round(best["views_s"] / q70, 3) if best else None,
+        slo_p99_ms=round(slo, 3),
+        server_stats=stats,
+        config=dict(scenes=list(scene_sizes), resolutions=[f"{w}x{h}" for w, h in RESOLUTIONS], max_views=max_views,
+                    n_slots=n_slots, arena_gib=arena_bytes / 2 ** 30),
+    )

@@ -337,6 +337,29 @@ int gsmi_project_backward_views(int64_t N, int B, const gsmi_view *views, const 
                                 void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Training-step stages after the adjoint                                    */
+/* ------------------------------------------------------------------------ */
+
+size_t gsmi_loss_workspace_size(int64_t n);
+
+/* Base photometric term of get_loss_fn (utils/loss.py:5-22, l1_loss /
+ * mse_loss with reduction "mean") over n image values: *loss (device float)
+ * = mean |out - gt| (l2 = 0) or mean (out - gt)^2 (l2 = 1), and grad (may be
+ * NULL) = d loss / d out.  Sums in double, fixed order. */
+int gsmi_photometric_loss(int64_t n, const float *out, const float *gt, int l2, float *grad, float *loss,
+                          void *workspace, size_t workspace_bytes, void *stream);
+
+/* One torch.optim.Adam step (gs/sh_renderer.py:720-729: per-group lr, betas
+ * (0.9, 0.99), eps) over up to 8 parameter groups in one launch; arrays of
+ * n_groups device pointers (host arrays).  fresh = 1: moments start at zero
+ * and step must be 1, exp_avg / exp_avg_sq are neither read nor written —
+ * the step the reference takes, as it rebuilds its optimizer every
+ * iteration (main_sh.py:238). */
+int gsmi_adam_step(int n_groups, float *const *params, const float *const *grads, float *const *exp_avg,
+                   float *const *exp_avg_sq, const int64_t *numel, const double *lr, int step, double beta1,
+                   double beta2, double eps, int fresh, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Render server (continuous batching)                                       */
 /* ------------------------------------------------------------------------ */
 /*
