@@ -334,6 +334,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the dominant kernel's HBM bytes")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement of the default run")
+    ap.add_argument("--serve-levels", default="1,2,3,4,6",
+                    help="server arrival rates as multiples of the plain per-call capacity")
+    ap.add_argument("--serve-duration", type=float, default=2.0, help="seconds of arrivals per serving level")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.copy_calibration:
@@ -412,7 +415,9 @@ def main():
     if args.workload == "dense_3m" and not args.no_serving:
         from gaussian_splatting_3d_amd import loadgen
 
-        sv = loadgen.serving_benchmark(log=lambda m: print(m, file=sys.stderr, flush=True))
+        sv = loadgen.serving_benchmark(levels=tuple(float(x) for x in args.serve_levels.split(",")),
+                                       duration_s=args.serve_duration,
+                                       log=lambda m: print(m, file=sys.stderr, flush=True))
         line["serving"] = sv
     if not args.no_cpu_baseline and views == 1:
         cb = cpu_baseline(scene)
@@ -427,7 +432,8 @@ def serve_main(args):
     plain per-call renderer on the same seeded request stream."""
     from gaussian_splatting_3d_amd import loadgen
 
-    sv = loadgen.serving_benchmark(log=lambda m: print(m, file=sys.stderr, flush=True))
+    sv = loadgen.serving_benchmark(levels=tuple(float(x) for x in args.serve_levels.split(",")),
+                                   duration_s=args.serve_duration, log=lambda m: print(m, file=sys.stderr, flush=True))
     line = {
         "metric": "served views/s at the plain renderer's p99 (1M+4M SH3 scenes, mixed 480p-1080p)",
         "value": sv["served_views_s_at_plain_p99"],

@@ -16,7 +16,8 @@ import os
 import torch  # noqa: F401  (must precede the library load, see module docstring)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgsmi.so")
+# GSMI_LIB: an alternative build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("GSMI_LIB") or os.path.join(_HERE, "libgsmi.so")
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int

@@ -17,7 +17,10 @@
 
 namespace gsmi {
 
-constexpr int kBatch = 128;  // Gaussians staged per LDS batch
+#ifndef GSMI_FWD_BATCH
+#define GSMI_FWD_BATCH 128
+#endif
+constexpr int kBatch = GSMI_FWD_BATCH;  // Gaussians staged per LDS batch
 
 struct RasterArgs {
   const float* mean;  // [V,2]
@@ -578,7 +581,10 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
 // j + 16 (i / 4), accumulator register i % 4.
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-constexpr int kMfmaBatch = 64;   // list entries staged per LDS batch
+#ifndef GSMI_BWD_BATCH
+#define GSMI_BWD_BATCH 64
+#endif
+constexpr int kMfmaBatch = GSMI_BWD_BATCH;  // list entries staged per LDS batch
 constexpr int kChunk = 4;        // entries per MFMA reduction
 constexpr int kCcStride = 68;    // padded LDS row (64 pixels + 4)
 constexpr int kCcRows = 3 * kChunk;

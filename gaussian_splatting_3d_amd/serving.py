@@ -82,6 +82,10 @@ class RenderServer:
         self._scenes.append(t)
         return sid.value
 
+    def add_resident(self, scene):
+        """Registers a scene_io.ResidentScene (e.g. from load_checkpoint)."""
+        return self.add_scene(*scene.params(), scene.C)
+
     def submit(self, scene_id, c2w, camera, out=None):
         """Queues one camera; returns (request id, output tensor [H,W,3]).
         The tensor is filled when the request completes (see poll)."""
