@@ -71,6 +71,13 @@ SIGNATURES = {
     "gsmi_loss_workspace_size": (_SZ, [_I64]),
     "gsmi_photometric_loss": (_I32, [_I64, _P, _P, _I32, _P, _P, _P, _SZ, _P]),
     "gsmi_adam_step": (_I32, [_I32, _P, _P, _P, _P, _P, _P, _I32, _D, _D, _D, _I32, _P]),
+    "gsmi_grad_stats": (_I32, [_I64, _P, _I32, _P, _I32, _P, _P]),
+    "gsmi_count_visible": (_I32, [_I64, _P, _P, _P]),
+    "gsmi_densify_workspace_size": (_SZ, [_I64]),
+    "gsmi_split_clone_count": (_I32, [_I64, _P, _P, _I32, _F, _P, _F, _P, _P, _SZ, _P]),
+    "gsmi_split_clone_write": (_I32, [_I64, _P, _P, _P, _P, _P, _I32, _P, _F, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gsmi_prune_alpha_count": (_I32, [_I64, _P, _F, _P, _P, _SZ, _P]),
+    "gsmi_prune_alpha_write": (_I32, [_I64, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P, _P, _SZ, _P]),
     "gsmi_make_view": (_I32, [_P, _D, _D, _D, _D, _I32, _I32, _D, _D, _I32, _P]),
     "gsmi_server_create": (_I32, [_I32, _I64, _I32, _I32, _F, _F, _F, _I32, _P]),
     "gsmi_server_add_scene": (_I32, [_P, _I64, _I32, _I64, _P, _P, _P, _P, _P, _P]),

@@ -17,10 +17,13 @@
 
 namespace gsmi {
 
+// Gaussians staged per LDS batch.  64 (not more) keeps the SH-4 forward's
+// LDS at 15 KB so that 8 wave64 per SIMD fit; measured 1.29 -> 1.10 ms at
+// 3M Gaussians / 1080p against 128.
 #ifndef GSMI_FWD_BATCH
-#define GSMI_FWD_BATCH 128
+#define GSMI_FWD_BATCH 64
 #endif
-constexpr int kBatch = GSMI_FWD_BATCH;  // Gaussians staged per LDS batch
+constexpr int kBatch = GSMI_FWD_BATCH;
 
 struct RasterArgs {
   const float* mean;  // [V,2]
@@ -359,6 +362,54 @@ __device__ __forceinline__ float wave_reduce_scatter(float (&v)[NV]) {
   return s;
 }
 
+// Same result as wave_reduce_scatter<8> (lane L returns the wave sum of
+// v[L >> 3]) without the LDS crossbar: the exchanges across 32 and 16 lanes
+// are gfx950 permlane swaps (v_permlane32_swap: upper half of the first
+// operand <-> lower half of the second; v_permlane16_swap: odd 16-lane rows of
+// the first <-> even rows of the second), the one across 8 lanes a DPP row
+// rotation by 8 (= lane ^ 8 inside a row), and the final sum over 8 lanes two
+// DPP quad permutations and a row rotation by 12 (row_ror:n makes lane i read
+// lane (i - n) mod 16 of its row).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
+}
+
+// The swaps are issued as inline VALU assembly: the compiler's lowering of
+// __builtin_amdgcn_permlane{16,32}_swap inside this kernel added the swapped
+// register to itself instead of to its partner (v_add v18, v55, v55 after
+// v_permlane32_swap v55, v60), which corrupted the sums.  One wait state
+// before each swap covers the VALU-write -> swap-read dependency.
+__device__ __forceinline__ void permlane32_swap(float& x, float& y) {
+  asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(x), "+v"(y));
+}
+__device__ __forceinline__ void permlane16_swap(float& x, float& y) {
+  asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(x), "+v"(y));
+}
+
+__device__ __forceinline__ float wave_reduce_scatter8_swap(float (&v)[8], int lane) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {  // across lane ^ 32: component i (low half) / i + 4 (high half)
+    float x = v[i], y = v[i + 4];
+    permlane32_swap(x, y);
+    v[i] = x + y;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {  // across lane ^ 16: + 2 on odd rows
+    float x = v[i], y = v[i + 2];
+    permlane16_swap(x, y);
+    v[i] = x + y;
+  }
+  const bool upper = (lane & 8) != 0;  // across lane ^ 8: + 1 on the upper half row
+  const float keep = upper ? v[1] : v[0];
+  const float send = upper ? v[0] : v[1];
+  float s = keep + dpp_f<0x128>(send);  // row_ror:8
+  s = s + dpp_f<0xB1>(s);               // quad_perm [1,0,3,2]
+  s = s + dpp_f<0x4E>(s);               // quad_perm [2,3,0,1]
+  s = s + dpp_f<0x12C>(s);              // row_ror:12 (lane i reads (i + 4) mod 16): lanes 0, 8 hold the total
+  return s;
+}
+
 constexpr int next_pow2(int x) {
   int p = 1;
   while (p < x) p <<= 1;
@@ -581,10 +632,13 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
 // j + 16 (i / 4), accumulator register i % 4.
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+// List entries staged per LDS batch of the MFMA backward: 32 keeps LDS at
+// 27.5 KB (5 workgroups per CU instead of 3); measured 4.33 -> 3.42 ms at
+// 3M Gaussians / 1080p against 64.
 #ifndef GSMI_BWD_BATCH
-#define GSMI_BWD_BATCH 64
+#define GSMI_BWD_BATCH 32
 #endif
-constexpr int kMfmaBatch = GSMI_BWD_BATCH;  // list entries staged per LDS batch
+constexpr int kMfmaBatch = GSMI_BWD_BATCH;
 constexpr int kChunk = 4;        // entries per MFMA reduction
 constexpr int kCcStride = 68;    // padded LDS row (64 pixels + 4)
 constexpr int kCcRows = 3 * kChunk;
@@ -732,7 +786,7 @@ __global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
       cc_w[(1 * kChunk + e) * kCcStride + lane] = cc1;
       cc_w[(2 * kChunk + e) * kCcStride + lane] = cc2;
       if (any) {
-        const float r = wave_reduce_scatter<8>(v);
+        const float r = wave_reduce_scatter8_swap(v, lane);
         if (writer && r != 0.0f) atomicAdd(s_acc + j * NCOMP + my_comp, r);
       }
       if (e == kChunk - 1 || j == 0) {

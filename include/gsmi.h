@@ -360,6 +360,44 @@ int gsmi_adam_step(int n_groups, float *const *params, const float *const *grads
                    double beta2, double eps, int fresh, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Adaptive density control (gs/sh_renderer.py:423-661)                      */
+/* ------------------------------------------------------------------------ */
+
+/* update_grads (sh_renderer.py:602-623): acc[index[r] or r] = max(acc, |g_r|)
+ * (use_max) or acc + |g_r| for the n rows of grad [n, dim], dim 3 (mean
+ * gradient) or 2 (mean2d gradient of the visible rows, index = original). */
+int gsmi_grad_stats(int64_t n, const float *grad, int dim, const int32_t *index, int use_max, float *acc,
+                    void *stream);
+
+/* cnt[i] += vis[i] (sh_renderer.py:215-216). */
+int gsmi_count_visible(int64_t N, const uint8_t *vis, int32_t *cnt, void *stream);
+
+size_t gsmi_densify_workspace_size(int64_t N);
+
+/* split_gaussians, phase 1 (sh_renderer.py:426-460): classify, scan and
+ * count; synchronises and writes the host array counts[3] = (unsplit,
+ * clone, split).  The new N is unsplit + clone + 2 * split. */
+int gsmi_split_clone_count(int64_t N, const float *acc, const int32_t *cnt, int use_mean, float grad_thresh,
+                           const float *log_s, float scale_thresh, int64_t *counts, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
+/* split_gaussians, phase 2 (:461-534): writes the new raw parameter arrays
+ * (unsplit rows, clones, then the split rows twice) from the same workspace;
+ * noise [2 * split, 3] standard normal; sh_row = floats per sh row. */
+int gsmi_split_clone_write(int64_t N, const float *mean, const float *qvec, const float *log_s, const float *logit,
+                           const float *sh, int sh_row, const float *noise, float shrink, float *out_mean,
+                           float *out_qvec, float *out_log_s, float *out_logit, float *out_sh, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
+/* remove_low_alpha_gaussians (:536-558): keep sigmoid(logit) >= alpha_thresh,
+ * order preserved; phase 1 synchronises and writes the host *kept. */
+int gsmi_prune_alpha_count(int64_t N, const float *logit, float alpha_thresh, int64_t *kept, void *workspace,
+                           size_t workspace_bytes, void *stream);
+int gsmi_prune_alpha_write(int64_t N, const float *mean, const float *qvec, const float *log_s, const float *logit,
+                           const float *sh, int sh_row, float *out_mean, float *out_qvec, float *out_log_s,
+                           float *out_logit, float *out_sh, void *workspace, size_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Render server (continuous batching)                                       */
 /* ------------------------------------------------------------------------ */
 /*

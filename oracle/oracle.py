@@ -628,3 +628,58 @@ def composite_f64(mean2d, cov2d, sh, alpha, start, end, ids, topleft, c2w, psx, 
                 used[pix] = set(chosen)
             out[3 * pix: 3 * pix + 3] = o
     return out, used
+
+
+# --------------------------------------------------------------------------
+# Adaptive density control (gs/sh_renderer.py:423-661), numpy float32
+# --------------------------------------------------------------------------
+
+def grad_stats(acc, grad, index=None, use_max=True):
+    """update_grads (sh_renderer.py:602-623): acc[i] = max(acc[i], |g|) or
+    acc[i] + |g|; index maps rows of grad to Gaussians (2D variant)."""
+    acc = np.array(acc, np.float32, copy=True)
+    g = np.asarray(grad, np.float32)
+    nrm = np.sqrt((g * g).sum(-1, dtype=np.float32)).astype(np.float32)
+    idx = np.arange(len(g)) if index is None else np.asarray(index)
+    acc[idx] = np.maximum(acc[idx], nrm) if use_max else (acc[idx] + nrm).astype(np.float32)
+    return acc
+
+
+def split_clone(mean, qvec, log_s, logit, sh, acc, cnt, use_mean, grad_thresh, scale_thresh, shrink, noise):
+    """split_gaussians (sh_renderer.py:426-534) with the Gaussian noise given:
+    returns the five new arrays and (n_unsplit, n_clone, n_split)."""
+    mean, qvec, log_s, logit, sh = (np.asarray(x, np.float32) for x in (mean, qvec, log_s, logit, sh))
+    acc = np.asarray(acc, np.float32)
+    grad = acc / (np.asarray(cnt).astype(np.float32) + np.float32(1e-5)) if use_mean else acc
+    mask = grad > np.float32(grad_thresh)                                   # :435 / :437
+    svec = expf(log_s).reshape(log_s.shape)                                 # self.svec (exp activation)
+    split = mask & (svec > np.float32(scale_thresh)).any(-1)               # :450-452
+    clone = mask & ~split                                                   # :454
+    k = int(split.sum())
+    s_rep = np.concatenate([svec[split]] * 2)                               # .repeat(2, 1)
+    qs = np.concatenate([qvec[split]] * 2)
+    R = np.stack([quat_to_rot(q) for q in qs]) if len(qs) else np.zeros((0, 3, 3), np.float32)
+    gn = (np.asarray(noise, np.float32).reshape(2 * k, 3) * s_rep).astype(np.float32)
+    # einsum("bij, bj -> bi", R^T, gn): (R^T gn)_a = sum_b R[b, a] gn[b]
+    rt_gn = (R[:, 0, :] * gn[:, 0:1] + R[:, 1, :] * gn[:, 1:2] + R[:, 2, :] * gn[:, 2:3]).astype(np.float32)
+    new_split_mean = (np.concatenate([mean[split]] * 2) + rt_gn).astype(np.float32)
+    keep = ~split
+    out = []
+    for a, sp in ((mean, new_split_mean), (qvec, None), (log_s, None), (logit, None), (sh, None)):
+        parts = [a[keep], a[clone]]
+        if sp is not None:
+            parts.append(sp)
+        elif a is log_s:
+            parts.append(np.log(s_rep / np.float32(shrink)).astype(np.float32))  # svec_inv_act(s / shrink)
+        else:
+            parts.append(np.concatenate([a[split]] * 2))
+        out.append(np.concatenate(parts).astype(np.float32))
+    return out, (int(keep.sum()), int(clone.sum()), k)
+
+
+def prune_alpha(mean, qvec, log_s, logit, sh, alpha_thresh):
+    """remove_low_alpha_gaussians (sh_renderer.py:536-558)."""
+    lg = np.asarray(logit, np.float32)
+    _, a = activate(np.zeros((lg.size, 3), np.float32), lg)
+    m = a >= np.float32(alpha_thresh)
+    return [np.asarray(x, np.float32)[m] for x in (mean, qvec, log_s, logit, sh)], int(m.sum())
