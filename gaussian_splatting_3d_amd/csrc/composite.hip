@@ -297,12 +297,20 @@ __global__ __launch_bounds__(256) void k_fwd_sh(RasterArgs a) {
         y1 = sigmoid_fast(dot_fast<CC>(shj + CC, Y));
         y2 = sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y));
       }
-      if (isnan(y0 * coeff)) y0 = 0.0f;
-      if (isnan(y1 * coeff)) y1 = 0.0f;
-      if (isnan(y2 * coeff)) y2 = 0.0f;
-      o0 = o0 + coeff * y0;
-      o1 = o1 + coeff * y1;
-      o2 = o2 + coeff * y2;
+      if constexpr (EXACT) {
+        if (isnan(y0 * coeff)) y0 = 0.0f;
+        if (isnan(y1 * coeff)) y1 = 0.0f;
+        if (isnan(y2 * coeff)) y2 = 0.0f;
+        o0 = o0 + coeff * y0;
+        o1 = o1 + coeff * y1;
+        o2 = o2 + coeff * y2;
+      } else {
+        // coeff is finite here and a sigmoid is NaN or in [0, 1], so
+        // isnan(y * coeff) == isnan(y), and max(y, 0) maps NaN to 0.
+        o0 = fmaf(coeff, fmaxf(y0, 0.0f), o0);
+        o1 = fmaf(coeff, fmaxf(y1, 0.0f), o1);
+        o2 = fmaf(coeff, fmaxf(y2, 0.0f), o2);
+      }
       T = T * (1.0f - al * G);
       if (T < thresh) {
         done = true;
@@ -563,12 +571,10 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
             float w = al * Tj * G;
             if (isnan(w)) w = 0.0f;
             const float* shj = s_sh + j * SHN;
-            float y0 = sigmoid_fast(dot_fast<CC>(shj, Y));
-            float y1 = sigmoid_fast(dot_fast<CC>(shj + CC, Y));
-            float y2 = sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y));
-            if (isnan(y0 * w)) y0 = 0.0f;
-            if (isnan(y1 * w)) y1 = 0.0f;
-            if (isnan(y2 * w)) y2 = 0.0f;
+            // w is finite, a sigmoid NaN or in [0, 1]: isnan(y * w) == isnan(y)
+            const float y0 = fmaxf(sigmoid_fast(dot_fast<CC>(shj, Y)), 0.0f);
+            const float y1 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + CC, Y)), 0.0f);
+            const float y2 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y)), 0.0f);
             cc0 = w * (y0 * (1.0f - y0)) * g0;
             cc1 = w * (y1 * (1.0f - y1)) * g1;
             cc2 = w * (y2 * (1.0f - y2)) * g2;
@@ -752,12 +758,10 @@ __global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
             float w = al * Tj * G;
             if (isnan(w)) w = 0.0f;
             const float* shj = s_sh + j * SHN;
-            float y0 = sigmoid_fast(dot_fast<CC>(shj, Y));
-            float y1 = sigmoid_fast(dot_fast<CC>(shj + CC, Y));
-            float y2 = sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y));
-            if (isnan(y0 * w)) y0 = 0.0f;
-            if (isnan(y1 * w)) y1 = 0.0f;
-            if (isnan(y2 * w)) y2 = 0.0f;
+            // w is finite, a sigmoid NaN or in [0, 1]: isnan(y * w) == isnan(y)
+            const float y0 = fmaxf(sigmoid_fast(dot_fast<CC>(shj, Y)), 0.0f);
+            const float y1 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + CC, Y)), 0.0f);
+            const float y2 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y)), 0.0f);
             cc0 = w * (y0 * (1.0f - y0)) * g0;
             cc1 = w * (y1 * (1.0f - y1)) * g1;
             cc2 = w * (y2 * (1.0f - y2)) * g2;
