@@ -32,8 +32,8 @@
 namespace gsmi {
 
 constexpr int kRadixThreads = 256;
-constexpr int kRadixItems = 16;
-constexpr int kRadixTile = kRadixThreads * kRadixItems;  // 4096 keys per block
+constexpr int kRadixItems = 8;
+constexpr int kRadixTile = kRadixThreads * kRadixItems;  // 2048 keys per block
 
 template <typename K, int RB>
 __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const K* __restrict__ keys, int64_t P, int shift,
@@ -56,28 +56,28 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const K* __restric
   if (t < BINS) hist[(int64_t)t * nblk + blockIdx.x] = s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
 }
 
-template <typename K, int RB>
+template <typename K, int RB, int TILE>
 struct ScatterLds {
   uint32_t cnt[4][1 << RB];
   uint32_t base[4][1 << RB];
   uint32_t dstart[1 << RB];
   uint32_t goff[1 << RB];
   uint32_t wsum[4];
-  K keys[kRadixTile];
-  int32_t vals[kRadixTile];
+  K keys[TILE];
+  int32_t vals[TILE];
 };
 
-// Stable scatter of one block of kRadixTile items.  Item r of lane `lane` in
-// wave w sits at block position w*1024 + r*64 + lane (key[r], val[r]; keys of
+// Stable scatter of one block of 256 * ITEMS items.  Item r of lane `lane` in
+// wave w sits at block position w*64*ITEMS + r*64 + lane (key[r], val[r]; keys of
 // invalid positions are all-ones).  Ranks by digit (key >> shift) & (2^RB-1)
 // with wave64 ballot matching, permutes through LDS, writes each digit's run
 // at offs[digit][block] (exclusive-scanned histogram).  kout may be null
 // (keys not needed).  With RANGES the keys are final tile ids in sorted
 // order: every block-local run boundary is folded into start (atomic min) and
 // end (atomic max of position + 1).
-template <typename K, int RB, bool RANGES>
-__device__ __forceinline__ void block_scatter(ScatterLds<K, RB>& s, K (&key)[kRadixItems],
-                                              int32_t (&val)[kRadixItems], int64_t base, int64_t P, int shift,
+template <typename K, int RB, bool RANGES, int ITEMS>
+__device__ __forceinline__ void block_scatter(ScatterLds<K, RB, kRadixThreads * ITEMS>& s, K (&key)[ITEMS],
+                                              int32_t (&val)[ITEMS], int64_t base, int64_t P, int shift,
                                               const uint32_t* __restrict__ offs, int nblk, K* __restrict__ kout,
                                               int32_t* __restrict__ vout, int32_t* __restrict__ start,
                                               int32_t* __restrict__ end, uint32_t n_tiles) {
@@ -89,11 +89,12 @@ __device__ __forceinline__ void block_scatter(ScatterLds<K, RB>& s, K (&key)[kRa
   __syncthreads();
 
   const uint64_t lt_mask = (1ull << lane) - 1ull;
-  uint32_t rank[kRadixItems];
-  // Wave w ranks its 1024 items 64 per round in position order, so ranks
-  // are stable.
+  constexpr int TILE = kRadixThreads * ITEMS;
+  uint32_t rank[ITEMS];
+  // Wave w ranks its 64 * ITEMS items 64 per round in position order, so
+  // ranks are stable.
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint32_t d = (uint32_t)(key[r] >> shift) & (BINS - 1);
     uint64_t peers = ~0ull;
 #pragma unroll
@@ -139,16 +140,16 @@ __device__ __forceinline__ void block_scatter(ScatterLds<K, RB>& s, K (&key)[kRa
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint32_t d = (uint32_t)(key[r] >> shift) & (BINS - 1);
     const uint32_t lp = s.base[w][d] + rank[r];
     s.keys[lp] = key[r];
     s.vals[lp] = val[r];
   }
   __syncthreads();
-  const int nvalid = (int)min((int64_t)kRadixTile, P - base);
+  const int nvalid = (int)min((int64_t)TILE, P - base);
 #pragma unroll 4
-  for (int i = t; i < kRadixTile; i += kRadixThreads) {
+  for (int i = t; i < TILE; i += kRadixThreads) {
     if (i < nvalid) {
       const K k = s.keys[i];
       const uint32_t d = (uint32_t)(k >> shift) & (BINS - 1);
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __rest
                                                                  const uint32_t* __restrict__ offs, int nblk,
                                                                  int32_t* __restrict__ start,
                                                                  int32_t* __restrict__ end, uint32_t n_tiles) {
-  __shared__ ScatterLds<K, RB> s;
+  __shared__ ScatterLds<K, RB, kRadixTile> s;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int64_t base = (int64_t)blockIdx.x * kRadixTile;
   K key[kRadixItems];
@@ -185,7 +186,8 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __rest
     key[r] = valid ? kin[p] : (K)~(K)0;
     val[r] = valid ? vin[p] : 0;
   }
-  block_scatter<K, RB, RANGES>(s, key, val, base, P, shift, offs, nblk, kout, vout, start, end, n_tiles);
+  block_scatter<K, RB, RANGES, kRadixItems>(s, key, val, base, P, shift, offs, nblk, kout, vout, start, end,
+                                            n_tiles);
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -239,70 +241,84 @@ __global__ void k_sorted_rects(const int32_t* __restrict__ dval, const uint32_t*
   rect[r] = make_int4(base + a.y * ntw + a.x, ntw, b.y - a.y + 1, v);
 }
 
-// View of visible entry dval[r]: the largest b with bases[2b] <= dval[r].
-__global__ void k_view_keys(const int32_t* __restrict__ dval, int64_t V, const int64_t* __restrict__ bases, int B,
-                            uint32_t* __restrict__ view) {
+// View of visible entry dval[r]: the largest b with bases[2b] <= dval[r]
+// (record bases staged in LDS; at most kMaxViewsLds views).
+constexpr int kMaxViewsLds = 1024;
+
+__global__ __launch_bounds__(256) void k_view_keys(const int32_t* __restrict__ dval, int64_t V,
+                                                   const int64_t* __restrict__ bases, int B,
+                                                   uint32_t* __restrict__ view) {
+  __shared__ int32_t s_base[kMaxViewsLds];
+  for (int b = threadIdx.x; b < B && b < kMaxViewsLds; b += blockDim.x) s_base[b] = (int32_t)bases[2 * b];
+  __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= V) return;
-  const int64_t v = dval[r];
-  int lo = 0, hi = B - 1;
+  const int32_t v = dval[r];
+  int lo = 0, hi = min(B, kMaxViewsLds) - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (bases[2 * mid] <= v) lo = mid;
+    if (s_base[mid] <= v) lo = mid;
     else hi = mid - 1;
   }
   view[r] = (uint32_t)lo;
 }
 
 // Expansion of the depth-ordered Gaussians into (tile, index) pairs.  Block b
-// owns output slots [b*kRadixTile, (b+1)*kRadixTile), the same partition as
-// the radix passes.  The offsets of the Gaussians overlapping that range are
-// staged in LDS (every visible Gaussian has at least one pair, so at most
-// kRadixTile + 1 of them); thread t takes the kRadixItems consecutive slots
-// starting at t*kRadixItems, finds the Gaussian of the first by binary search
-// and walks forward.
+// owns output slots [b*kExpandTile, (b+1)*kExpandTile) (2048: half the radix
+// tile, so that the fused scatter needs 27 KB of LDS and 5 workgroups fit a
+// CU).  The offsets of the Gaussians overlapping that range are staged in LDS
+// (every visible Gaussian has at least one pair, so at most kExpandTile + 1
+// of them); thread t takes the kExpandItems consecutive slots starting at
+// t*kExpandItems, finds the Gaussian of the first by binary search and walks
+// forward.
 constexpr int kExpandDigitBits = 7;
+constexpr int kExpandItems = 8;
+constexpr int kExpandTile = kRadixThreads * kExpandItems;
 
 struct ExpandLds {
-  int32_t off[kRadixTile + 2];
-  int32_t range[2];
+  int32_t off[kExpandTile + 2];
 };
 
-// Stages the block's Gaussian offsets; returns (r0, n) = first depth rank and
-// count of staged Gaussians.
-__device__ __forceinline__ int2 expand_stage(ExpandLds& s, const int32_t* __restrict__ off, int64_t V, int64_t P) {
+// first[b] = depth rank of the Gaussian whose pair range holds slot
+// b * kExpandTile.  Every visible Gaussian has at least one pair, so every
+// block start lies in exactly one range; entries a malformed count leaves
+// unset keep the fill value -1 and are clamped by the reader.
+__global__ void k_block_first(const int32_t* __restrict__ off, int64_t V, int64_t P, int32_t* __restrict__ first,
+                              int nblk) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= V) return;
+  const int64_t lo = off[r];
+  const int64_t hi = r + 1 < V ? (int64_t)off[r + 1] : P;
+  for (int64_t b = (lo + kExpandTile - 1) / kExpandTile; b * kExpandTile < hi && b < nblk; ++b)
+    if (b >= 0) first[b] = (int32_t)r;
+}
+
+// Stages the offsets of the Gaussians from first[b] to first[b + 1]
+// (inclusive: the last may start exactly at the next block); returns
+// (r0, n) = first depth rank and count of staged Gaussians.
+__device__ __forceinline__ int2 expand_stage(ExpandLds& s, const int32_t* __restrict__ off,
+                                             const int32_t* __restrict__ first, int64_t V, int nblk) {
   const int t = threadIdx.x;
-  const int64_t p0 = (int64_t)blockIdx.x * kRadixTile;
-  const int64_t p1 = min(p0 + kRadixTile, P);  // exclusive
-  if (t < 2) {
-    // largest r with off[r] <= target
-    const int64_t target = t == 0 ? p0 : p1 - 1;
-    int64_t lo = 0, hi = V - 1;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi + 1) >> 1;
-      if ((int64_t)off[mid] <= target) lo = mid;
-      else hi = mid - 1;
-    }
-    s.range[t] = (int32_t)lo;
-  }
-  __syncthreads();
-  const int r0 = s.range[0];
-  // <= kRadixTile + 1 when every Gaussian has at least one pair; clamped so
+  const int b = blockIdx.x;
+  const int hiV = (int)(V - 1);
+  const int r0 = min(max(first[b], 0), hiV);
+  const int r1 = b + 1 < nblk ? min(max(first[b + 1], r0), hiV) : hiV;
+  // <= kExpandTile + 2 when every Gaussian has at least one pair; clamped so
   // inconsistent bounds cannot index past the LDS array.
-  const int n = max(1, min(s.range[1] - r0 + 1, kRadixTile + 1));
+  const int n = max(1, min(r1 - r0 + 1, kExpandTile + 2));
   for (int i = t; i < n; i += kRadixThreads) s.off[i] = off[r0 + i];
   __syncthreads();
   return make_int2(r0, n);
 }
 
-// Tiles and indices of this thread's slots p0 + t*kRadixItems + k; invalid
+// Tiles and indices of this thread's slots p0 + t*kExpandItems + k; invalid
 // slots (past P) get tile 0xffffffff.
 __device__ __forceinline__ void expand_slots(const ExpandLds& s, int2 rn, const int4* __restrict__ rect, int64_t P,
-                                             uint32_t (&tile)[kRadixItems],
-                                             int32_t (&val)[kRadixItems]) {
+                                             uint32_t (&tile)[kExpandItems],
+                                             int32_t (&val)[kExpandItems]) {
   const int t = threadIdx.x;
   const int r0 = rn.x, n = rn.y;
-  int64_t p = (int64_t)blockIdx.x * kRadixTile + (int64_t)t * kRadixItems;
+  int64_t p = (int64_t)blockIdx.x * kExpandTile + (int64_t)t * kExpandItems;
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -312,7 +328,7 @@ __device__ __forceinline__ void expand_slots(const ExpandLds& s, int2 rn, const 
   int j = lo;
   int4 rc = rect[r0 + j];
 #pragma unroll
-  for (int k = 0; k < kRadixItems; ++k, ++p) {
+  for (int k = 0; k < kExpandItems; ++k, ++p) {
     tile[k] = 0xffffffffu;
     val[k] = 0;
     if (p < P) {
@@ -332,6 +348,7 @@ __device__ __forceinline__ void expand_slots(const ExpandLds& s, int2 rn, const 
 // Pass-1 histogram of the expansion: counts of the low kExpandDigitBits
 // tile-id bits per block, hist[digit][block].
 __global__ __launch_bounds__(kRadixThreads) void k_expand_hist(const int32_t* __restrict__ off,
+                                                               const int32_t* __restrict__ first,
                                                                const int4* __restrict__ rect, int64_t V, int64_t P,
                                                                uint32_t* __restrict__ hist,
                                                                int nblk) {
@@ -340,12 +357,12 @@ __global__ __launch_bounds__(kRadixThreads) void k_expand_hist(const int32_t* __
   __shared__ uint32_t s_cnt[4][BINS];
   const int t = threadIdx.x, w = t >> 6;
   for (int i = t; i < 4 * BINS; i += kRadixThreads) (&s_cnt[0][0])[i] = 0;
-  const int2 rn = expand_stage(s, off, V, P);
-  uint32_t tile[kRadixItems];
-  int32_t val[kRadixItems];
+  const int2 rn = expand_stage(s, off, first, V, nblk);
+  uint32_t tile[kExpandItems];
+  int32_t val[kExpandItems];
   expand_slots(s, rn, rect, P, tile, val);
 #pragma unroll
-  for (int k = 0; k < kRadixItems; ++k)
+  for (int k = 0; k < kExpandItems; ++k)
     if (tile[k] != 0xffffffffu) atomicAdd(&s_cnt[w][tile[k] & (BINS - 1)], 1u);
   __syncthreads();
   if (t < BINS) hist[(int64_t)t * nblk + blockIdx.x] = s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
@@ -361,6 +378,7 @@ __device__ __forceinline__ int swz(int q) { return q ^ ((q >> 6) & 63); }
 // low kExpandDigitBits tile-id bits (offsets from k_expand_hist, scanned).
 template <bool RANGES>
 __global__ __launch_bounds__(kRadixThreads) void k_expand_scatter(const int32_t* __restrict__ off,
+                                                                  const int32_t* __restrict__ first,
                                                                   const int4* __restrict__ rect, int64_t V,
                                                                   int64_t P,
                                                                   const uint32_t* __restrict__ offs, int nblk,
@@ -369,32 +387,32 @@ __global__ __launch_bounds__(kRadixThreads) void k_expand_scatter(const int32_t*
                                                                   int32_t* __restrict__ start,
                                                                   int32_t* __restrict__ end, uint32_t n_tiles) {
   __shared__ ExpandLds se;
-  __shared__ ScatterLds<uint32_t, kExpandDigitBits> s;
+  __shared__ ScatterLds<uint32_t, kExpandDigitBits, kExpandTile> s;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int2 rn = expand_stage(se, off, V, P);
+  const int2 rn = expand_stage(se, off, first, V, nblk);
   {
-    uint32_t tile[kRadixItems];
-    int32_t val[kRadixItems];
+    uint32_t tile[kExpandItems];
+    int32_t val[kExpandItems];
     expand_slots(se, rn, rect, P, tile, val);
 #pragma unroll
-    for (int k = 0; k < kRadixItems; ++k) {
-      const int q = swz(t * kRadixItems + k);
+    for (int k = 0; k < kExpandItems; ++k) {
+      const int q = swz(t * kExpandItems + k);
       s.keys[q] = tile[k];
       s.vals[q] = val[k];
     }
   }
   __syncthreads();
-  uint32_t key[kRadixItems];
-  int32_t val[kRadixItems];
+  uint32_t key[kExpandItems];
+  int32_t val[kExpandItems];
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
-    const int q = swz(w * (64 * kRadixItems) + r * 64 + lane);
+  for (int r = 0; r < kExpandItems; ++r) {
+    const int q = swz(w * (64 * kExpandItems) + r * 64 + lane);
     key[r] = s.keys[q];
     val[r] = s.vals[q];
   }
   // block_scatter's first barrier orders these reads before its LDS writes
-  block_scatter<uint32_t, kExpandDigitBits, RANGES>(s, key, val, (int64_t)blockIdx.x * kRadixTile, P, 0, offs, nblk,
-                                                    kout, vout, start, end, n_tiles);
+  block_scatter<uint32_t, kExpandDigitBits, RANGES, kExpandItems>(s, key, val, (int64_t)blockIdx.x * kExpandTile, P,
+                                                                  0, offs, nblk, kout, vout, start, end, n_tiles);
 }
 
 // start == INT32_MAX (no pair seen by the atomic-min) -> -1.
@@ -492,15 +510,19 @@ struct BinWs {
   uint32_t *ka, *kb;  // [P] tile ids, ping-pong
   int32_t *va, *vb;   // [P] indices, ping-pong
   uint32_t* hist;     // [nblk * 256]
+  int32_t* first;     // [expansion blocks] first Gaussian of each block
   void* scan_ws;
   void* sort_ws;  // depth sort
 };
 
 static int64_t pair_blocks(int64_t P) { return (P + kRadixTile - 1) / kRadixTile; }
 
+static int64_t expand_blocks(int64_t P) { return (P + kExpandTile - 1) / kExpandTile; }
+
 static size_t bin_ws_bytes(int64_t V, int64_t P) {
   const int64_t nh = pair_blocks(P) * 256;
   return align_up(V * 4) * 4 + align_up(V * 16) + 256 + align_up(P * 4) * 4 + align_up(nh * 4) +
+         align_up((expand_blocks(P) + 1) * 4) +
          align_up(scan_workspace_bytes(std::max(V, nh), 4)) + radix_ws_bytes<uint32_t, 8>(V);
 }
 
@@ -523,6 +545,7 @@ static BinWs carve_bin_ws(void* ws, int64_t V, int64_t P) {
   b.va = reinterpret_cast<int32_t*>(take(P * 4));
   b.vb = reinterpret_cast<int32_t*>(take(P * 4));
   b.hist = reinterpret_cast<uint32_t*>(take(pair_blocks(P) * 256 * 4));
+  b.first = reinterpret_cast<int32_t*>(take((expand_blocks(P) + 1) * 4));
   b.scan_ws = take(scan_workspace_bytes(std::max(V, pair_blocks(P) * 256), 4));
   b.sort_ws = p;
   return b;
@@ -534,24 +557,29 @@ static BinWs carve_bin_ws(void* ws, int64_t V, int64_t P) {
 static int pair_sort(const BinWs& w, int64_t V, int64_t P, int n_tiles, int32_t* ids,
                      int32_t* start, int32_t* end, bool want_tiles, uint32_t** tiles, hipStream_t st) {
   const int nblk = (int)pair_blocks(P);
+  const int nblk1 = (int)expand_blocks(P);  // expansion blocks (pass 1)
   const int bits = std::max(bits_for(n_tiles), 1);
   const unsigned nt = (unsigned)n_tiles;
   *tiles = nullptr;
-  hipLaunchKernelGGL(k_expand_hist, dim3(nblk), dim3(kRadixThreads), 0, st, w.cnt, w.rect, V, P, w.hist, nblk);
+  static_assert((kRadixTile / kExpandTile) * (1 << kExpandDigitBits) <= 256, "pass-1 histogram exceeds the buffer");
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nblk1 + 255) / 256)), dim3(256), 0, st, w.first, (int64_t)nblk1, -1);
+  hipLaunchKernelGGL(k_block_first, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, w.cnt, V, P, w.first, nblk1);
+  hipLaunchKernelGGL(k_expand_hist, dim3(nblk1), dim3(kRadixThreads), 0, st, w.cnt, w.first, w.rect, V, P, w.hist,
+                     nblk1);
   GSMI_CHECK_LAUNCH();
   int rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(w.hist), reinterpret_cast<int32_t*>(w.hist),
-                                   (int64_t)nblk << kExpandDigitBits, nullptr, w.scan_ws, st);
+                                   (int64_t)nblk1 << kExpandDigitBits, nullptr, w.scan_ws, st);
   if (rc) return rc;
   if (bits <= kExpandDigitBits) {
     uint32_t* ko = want_tiles ? w.ka : nullptr;
-    hipLaunchKernelGGL((k_expand_scatter<true>), dim3(nblk), dim3(kRadixThreads), 0, st, w.cnt, w.rect, V, P,
-                       w.hist, nblk, ko, ids, start, end, nt);
+    hipLaunchKernelGGL((k_expand_scatter<true>), dim3(nblk1), dim3(kRadixThreads), 0, st, w.cnt, w.first, w.rect, V,
+                       P, w.hist, nblk1, ko, ids, start, end, nt);
     GSMI_CHECK_LAUNCH();
     *tiles = ko;
     return GSMI_OK;
   }
-  hipLaunchKernelGGL((k_expand_scatter<false>), dim3(nblk), dim3(kRadixThreads), 0, st, w.cnt, w.rect, V, P,
-                     w.hist, nblk, w.ka, w.va, nullptr, nullptr, nt);
+  hipLaunchKernelGGL((k_expand_scatter<false>), dim3(nblk1), dim3(kRadixThreads), 0, st, w.cnt, w.first, w.rect, V,
+                     P, w.hist, nblk1, w.ka, w.va, nullptr, nullptr, nt);
   GSMI_CHECK_LAUNCH();
   uint32_t *kin = w.ka, *kout = w.kb;
   int32_t *vin = w.va, *vout = w.vb;
@@ -693,7 +721,7 @@ extern "C" int gsmi_bin_views(int64_t V, int64_t P, int B, const gsmi_view* view
   if (((uintptr_t)tl & 7) || ((uintptr_t)br & 7) || ((uintptr_t)views & 7)) return GSMI_ERR_ARG;
   if (P > 0 && !gaussian_ids) return GSMI_ERR_ARG;
   if (P >= (int64_t(1) << 31) || V >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
-  if (n_tiles_total > (1 << 24) || B > (1 << 16)) return GSMI_ERR_UNSUPPORTED;
+  if (n_tiles_total > (1 << 24) || B > kMaxViewsLds) return GSMI_ERR_UNSUPPORTED;
   if (!workspace || workspace_bytes < bin_ws_bytes(V, P)) return GSMI_ERR_WORKSPACE;
   return bin_impl(V, P, tl, br, depth, views, bases, B, n_tiles_total, 0, gaussian_ids, start, end, nullptr,
                   workspace, 0, (hipStream_t)stream);
