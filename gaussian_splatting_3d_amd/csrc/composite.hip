@@ -427,6 +427,35 @@ constexpr int next_pow2(int x) {
 // Gradient component layout per Gaussian in the LDS accumulator:
 // 0,1 mean; 2 cov00; 3 cov01 (= cov10); 4 cov11; 5 alpha; 6.. colour.
 template <int NCOL>
+__device__ __forceinline__ void emit_grad(const RasterArgs& a, int g, int c, float v) {
+  if (c < 2) {
+    unsafeAtomicAdd(a.grad_mean + 2 * (int64_t)g + c, v);
+  } else if (c < 5) {
+    if (c == 2) unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 0, v);
+    if (c == 3) {
+      unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 1, v);
+      unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 2, v);
+    }
+    if (c == 4) unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 3, v);
+  } else if (c == 5) {
+    unsafeAtomicAdd(a.grad_alpha + g, v);
+  } else {
+    const int r = c - 6;
+    if (a.sh_sc == 0) {  // RGB colour [V,3]
+      unsafeAtomicAdd(a.grad_sh + 3 * (int64_t)g + r, v);
+    } else {
+      const int CCl = (int)(NCOL / 3);
+      const int ch = r / CCl;
+      const int k = r - ch * CCl;
+      const int64_t row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
+      unsafeAtomicAdd(a.grad_sh + row * a.sh_sg + ch * a.sh_sc + k, v);
+    }
+  }
+}
+
+// Adds a batch's per-entry gradient sums (s_acc[entry][6 + NCOL]) to the
+// global gradients.
+template <int NCOL>
 __device__ __forceinline__ void flush_grads(const RasterArgs& a, const int* s_id, float* s_acc, int cnt) {
   constexpr int NCOMP = 6 + NCOL;
   const int t = threadIdx.x;
@@ -434,31 +463,7 @@ __device__ __forceinline__ void flush_grads(const RasterArgs& a, const int* s_id
     const float v = s_acc[e];
     if (v == 0.0f) continue;
     const int j = e / NCOMP;
-    const int c = e - j * NCOMP;
-    const int g = s_id[j];
-    if (c < 2) {
-      unsafeAtomicAdd(a.grad_mean + 2 * (int64_t)g + c, v);
-    } else if (c < 5) {
-      if (c == 2) unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 0, v);
-      if (c == 3) {
-        unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 1, v);
-        unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 2, v);
-      }
-      if (c == 4) unsafeAtomicAdd(a.grad_cov + 4 * (int64_t)g + 3, v);
-    } else if (c == 5) {
-      unsafeAtomicAdd(a.grad_alpha + g, v);
-    } else {
-      const int r = c - 6;
-      if (a.sh_sc == 0) {  // RGB colour [V,3]
-        unsafeAtomicAdd(a.grad_sh + 3 * (int64_t)g + r, v);
-      } else {
-        const int CCl = (int)(NCOL / 3);
-        const int ch = r / CCl;
-        const int k = r - ch * CCl;
-        const int64_t row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
-        unsafeAtomicAdd(a.grad_sh + row * a.sh_sg + ch * a.sh_sc + k, v);
-      }
-    }
+    emit_grad<NCOL>(a, s_id[j], e - j * NCOMP, v);
   }
 }
 
@@ -625,32 +630,71 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
 
 // ---------------------------------------------------------------------------
 // Backward for SH orders 3 and 4 with the SH-gradient reduction on the matrix
-// cores.  Per wave and per chunk of 8 list entries, the per-pixel colour
-// coefficients cc[ch](p) = w y (1-y) g_ch are staged in LDS as a 12 x 64
-// matrix (rows = (channel, entry) for 4 entries, columns = the wave's 64
-// pixels) and
+// cores.  Per wave, the per-pixel colour coefficients
+// cc[ch](p) = w y (1-y) g_ch of the list entries that have a contributing
+// pixel are staged in LDS, 5 entries at a time, as a 15 x 64 matrix
+// (row = ch * 5 + entry slot, columns = the wave's 64 pixels), and
 //   grad_sh[e][ch][k] += sum_p cc[ch,e](p) * Y_k(p)
-// is one 12x64 by 64x16 product: one M-tile (rows 12..15 zero) x 16 K-steps
-// of v_mfma_f32_16x16x4_f32 (the Y operand is constant per wave and lives in
-// 16 VGPRs).  The six scalar gradient components (mean, cov, alpha) are reduced
-// with an 8-wide wave reduce-scatter.  Operand layout of the 16x16x4 MFMA:
-// A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane
-// j + 16 (i / 4), accumulator register i % 4.
+// is one 16x64 by 64x16 product (row 15 unused) on bf16 matrix cores with
+// fp32 accumulation, in three terms: each fp32 operand x is split into
+// hi = x with the low 16 bits cleared (exactly a bf16) and lo = x - hi (exact
+// in fp32, then truncated to bf16), and
+//   A B ~= Ah Bh + Ah Bl + Al Bh,
+// dropping Al Bl and the truncation of lo: relative error below 2^-15 per
+// product, accumulated in fp32.  That is 6 v_mfma_f32_16x16x32_bf16 (2
+// K-steps of 32 pixels x 3 terms, 16 cycles each) per chunk instead of 16
+// fp32 16x16x4 steps of 32 cycles.  The Y operand is constant per wave and
+// lives in 16 VGPRs (hi and lo, 2 K-steps, 8 bf16 each).  Entries without a
+// contributing pixel take no slot.  The six scalar gradient components (mean,
+// cov, alpha) are reduced with an 8-wide wave reduce-scatter.
+//
+// Operand layout of the 16x16x32 bf16 MFMA: lane i + 16g holds 8 values of
+// row i of A (and of column i of B) for the same 8 K indices; D[r][c] is in
+// lane c + 16 (r / 4), accumulator register r % 4.  Both operands here give
+// lane group g pixels 32t + 8g .. 32t + 8g + 7 for K-step t, so the product
+// sums every pixel once whatever order the hardware assigns to the 8 K
+// indices of a lane.  A staged row holds the 64 pixels in 8 blocks of 8
+// (bf16), block b at position b ^ (row & 7), so each lane's A values of one
+// K-step are one 16-byte load and the 8 rows a load touches fall on distinct
+// banks.
+//
+// The per-entry sums of the 48 SH components go to per-wave LDS accumulators
+// with plain stores: an entry takes at most one slot per wave and batch, so
+// each (entry, component) is written at most once per wave, and the four
+// waves' values are added when the batch is flushed to the global gradients.
+// (LDS float atomics from every lane of every flush were the kernel's largest
+// cost.)  The six scalar components keep LDS atomics (6 lanes per entry).
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-// List entries staged per LDS batch of the MFMA backward: 32 keeps LDS at
-// 27.5 KB (5 workgroups per CU instead of 3); measured 4.33 -> 3.42 ms at
-// 3M Gaussians / 1080p against 64.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// List entries staged per LDS batch of the MFMA backward: 16 keeps LDS under
+// 32 KB with the per-wave accumulators (5 workgroups per CU).
 #ifndef GSMI_BWD_BATCH
-#define GSMI_BWD_BATCH 32
+#define GSMI_BWD_BATCH 16
 #endif
 constexpr int kMfmaBatch = GSMI_BWD_BATCH;
-constexpr int kChunk = 4;        // entries per MFMA reduction
-constexpr int kCcStride = 68;    // padded LDS row (64 pixels + 4)
+constexpr int kChunk = 5;        // entry slots per MFMA reduction (3 * 5 = 15 of 16 rows)
 constexpr int kCcRows = 3 * kChunk;
+constexpr int kCcStride = 64;    // bf16 per staged row (128 B)
+constexpr int kCcPlane = kCcRows * kCcStride;  // bf16 per plane (hi or lo)
+constexpr int kSlotBits = 6;     // batch positions of the chunk's slots, packed in one word
+static_assert(kMfmaBatch <= (1 << kSlotBits) && kChunk * kSlotBits <= 32, "slot packing");
+
+// Position (bf16 units) of pixel p in staged row `row`: 8-pixel blocks XOR-swizzled by row.
+__device__ __forceinline__ int cc_pos(int row, int p) {
+  return row * kCcStride + ((((p >> 3) ^ row) & 7) << 3) + (p & 7);
+}
+
+// Upper 16 bits of x (x truncated to bf16) and of the remainder x - hi.
+__device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) {
+  const uint32_t b = __float_as_uint(x);
+  hi = b >> 16;
+  lo = __float_as_uint(x - __uint_as_float(b & 0xffff0000u)) >> 16;
+}
 
 template <int C, bool BG>
-__global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bwd_sh_mfma(RasterArgs a) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
   constexpr int NCOMP = 6 + SHN;
@@ -660,8 +704,9 @@ __global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
   __shared__ float2 s_gc[kMfmaBatch];
   __shared__ int s_id[kMfmaBatch];
   __shared__ float s_sh[kMfmaBatch * SHN];
-  __shared__ float s_acc[kMfmaBatch * NCOMP];
-  __shared__ float s_cc[4][kCcRows * kCcStride];
+  __shared__ float s_acc[kMfmaBatch * 6];                      // scalar components, shared
+  __shared__ __align__(16) float s_shacc[4][kMfmaBatch * SHN];  // SH components, per wave
+  __shared__ __align__(16) uint16_t s_cc[4][2 * kCcPlane];     // per wave: hi plane, lo plane
   __shared__ int s_tmp[4];
 
   TileCtx tc;
@@ -686,21 +731,32 @@ __global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
   float Y[CC];
   sh_basis<C>(dir[0], dir[1], dir[2], Y);
 
-  // B operand: bY[s] = Y_{lane%16}(pixel 4s + lane/16 of this wave), via an
-  // LDS transpose through this wave's staging area, 8 basis columns at a
-  // time (64 x 8 floats fit in the area).
-  static_assert(kCcRows * kCcStride >= 64 * 8, "transpose does not fit the staging area");
-  float* cc_w = s_cc[wv];
-  float bY[16];
+  // B operand: lane j + 16g holds Y_j at pixels 32t + 8g + e (e = 0..7) of
+  // this wave for K-step t, split into hi and lo bf16.  Transposed through
+  // this wave's staging area as fp32, 8 basis functions at a time.
+  static_assert(2 * kCcPlane * 2 >= 64 * 8 * 4, "transpose does not fit the staging area");
+  uint16_t* cc_w = s_cc[wv];
+  float* tr = reinterpret_cast<float*>(cc_w);
+  u32x4 bh[2], bl[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) cc_w[lane * 8 + k] = (8 * h + k) < CC ? Y[(8 * h + k) < CC ? 8 * h + k : 0] : 0.0f;
+    for (int k = 0; k < 8; ++k) tr[lane * 8 + k] = (8 * h + k) < CC ? Y[(8 * h + k) < CC ? 8 * h + k : 0] : 0.0f;
     __builtin_amdgcn_wave_barrier();
     const int col = (lane & 15) - 8 * h;
     if (col >= 0 && col < 8) {
 #pragma unroll
-      for (int s = 0; s < 16; ++s) bY[s] = cc_w[(4 * s + (lane >> 4)) * 8 + col];
+      for (int st = 0; st < 2; ++st) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int p = 32 * st + 8 * (lane >> 4) + 2 * q;
+          uint32_t h0, l0, h1, l1;
+          split_bf16(tr[p * 8 + col], h0, l0);
+          split_bf16(tr[(p + 1) * 8 + col], h1, l1);
+          bh[st][q] = h0 | (h1 << 16);
+          bl[st][q] = l0 | (l1 << 16);
+        }
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -722,18 +778,26 @@ __global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
   const int nmax = block_max(nd, s_tmp);
   const int my_comp = lane >> 3;                      // scalar reduce-scatter (8 wide)
   const bool writer = (lane & 7) == 0 && my_comp < 6;
-  const int arow = lane & 15, acol = lane >> 4;       // MFMA A-operand coordinates
+  // MFMA A operand: lane i + 16g reads row i (row 15 repeats row 0; its
+  // output row is discarded) at pixels 32t + 8g .. +7 of each plane.
+  const int arow = (lane & 15) < kCcRows ? (lane & 15) : 0;
+  const u32x4* a_src0 = reinterpret_cast<const u32x4*>(cc_w + cc_pos(arow, 8 * (lane >> 4)));
+  const u32x4* a_src1 = reinterpret_cast<const u32x4*>(cc_w + cc_pos(arow, 32 + 8 * (lane >> 4)));
+  float* shacc_w = s_shacc[wv];
+  const int d_ch = lane >> 4;                          // channel of this lane's MFMA output rows
+  const int d_k = lane & 15;                           // basis function of this lane's output column
 
   for (int base = ((nmax - 1) / kMfmaBatch) * kMfmaBatch; nmax > 0 && base >= 0; base -= kMfmaBatch) {
     const int cnt = min(kMfmaBatch, nmax - base);
     __syncthreads();
-    for (int e = t; e < cnt * NCOMP; e += kTilePixels) s_acc[e] = 0.0f;
+    for (int e = t; e < cnt * 6; e += kTilePixels) s_acc[e] = 0.0f;
+    for (int e = t; e < 4 * kMfmaBatch * SHN / 4; e += kTilePixels)
+      reinterpret_cast<float4*>(&s_shacc[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
     stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
     __syncthreads();
-    int chunk_top = cnt - 1;  // list position (within the batch) of chunk row 0
-    bool chunk_any = false;
+    int n_slots = 0;      // filled slots of the current chunk (wave-uniform)
+    uint32_t slots = 0;   // batch position of slot m in bits [kSlotBits m, kSlotBits (m + 1))
     for (int j = cnt - 1; j >= 0; --j) {
-      const int e = chunk_top - j;  // row of this entry within the chunk
       const int idx = base + j;
       float cc0 = 0.0f, cc1 = 0.0f, cc2 = 0.0f;
       float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -784,42 +848,71 @@ __global__ __launch_bounds__(256) void k_bwd_sh_mfma(RasterArgs a) {
           }
         }
       }
-      const bool any = __any(contrib);
-      chunk_any = chunk_any || any;
-      cc_w[(0 * kChunk + e) * kCcStride + lane] = cc0;
-      cc_w[(1 * kChunk + e) * kCcStride + lane] = cc1;
-      cc_w[(2 * kChunk + e) * kCcStride + lane] = cc2;
-      if (any) {
+      if (__any(contrib)) {
+        const int m = n_slots;
+        uint32_t h, l;
+        split_bf16(cc0, h, l);
+        cc_w[cc_pos(0 * kChunk + m, lane)] = h;
+        cc_w[kCcPlane + cc_pos(0 * kChunk + m, lane)] = l;
+        split_bf16(cc1, h, l);
+        cc_w[cc_pos(1 * kChunk + m, lane)] = h;
+        cc_w[kCcPlane + cc_pos(1 * kChunk + m, lane)] = l;
+        split_bf16(cc2, h, l);
+        cc_w[cc_pos(2 * kChunk + m, lane)] = h;
+        cc_w[kCcPlane + cc_pos(2 * kChunk + m, lane)] = l;
+        slots |= static_cast<uint32_t>(j) << (kSlotBits * m);
+        n_slots = m + 1;
         const float r = wave_reduce_scatter8_swap(v, lane);
-        if (writer && r != 0.0f) atomicAdd(s_acc + j * NCOMP + my_comp, r);
+        if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r);
       }
-      if (e == kChunk - 1 || j == 0) {
-        if (chunk_any) {
-          __builtin_amdgcn_wave_barrier();
-          floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      if (n_slots == kChunk || (j == 0 && n_slots > 0)) {
+        // LDS operations of one wave complete in order; the barrier keeps the
+        // compiler from moving the loads above the stores.
+        __builtin_amdgcn_wave_barrier();
+        const u32x4 ah0 = a_src0[0], ah1 = a_src1[0];  // pixels 8g.., 32 + 8g..
+        const u32x4 al0 = a_src0[kCcPlane / 8], al1 = a_src1[kCcPlane / 8];
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah0),
+                                                      __builtin_bit_cast(bf16x8, bh[0]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah0),
+                                                      __builtin_bit_cast(bf16x8, bl[0]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al0),
+                                                      __builtin_bit_cast(bf16x8, bh[0]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah1),
+                                                      __builtin_bit_cast(bf16x8, bh[1]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah1),
+                                                      __builtin_bit_cast(bf16x8, bl[1]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al1),
+                                                      __builtin_bit_cast(bf16x8, bh[1]), acc, 0, 0, 0);
+        if (d_k < CC) {
 #pragma unroll
-          for (int s = 0; s < 16; ++s) {
-            const float a0 = arow < kCcRows ? cc_w[arow * kCcStride + 4 * s + acol] : 0.0f;
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bY[s], acc, 0, 0, 0);
-          }
-          const int k = lane & 15;
-          if (k < CC) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int row = 4 * (lane >> 4) + r;  // channel row / kChunk, entry row % kChunk
-              const int ee = row & (kChunk - 1);
-              if (row < kCcRows && ee <= e && acc[r] != 0.0f)
-                atomicAdd(s_acc + (chunk_top - ee) * NCOMP + 6 + (row / kChunk) * CC + k, acc[r]);
+          for (int r = 0; r < 4; ++r) {
+            const int row = 4 * d_ch + r;  // = ch * kChunk + slot
+            const int ch = row / kChunk, m = row - ch * kChunk;
+            if (row < kCcRows && m < n_slots) {
+              const int jj = (slots >> (kSlotBits * m)) & ((1u << kSlotBits) - 1);
+              shacc_w[jj * SHN + ch * CC + d_k] = acc[r];
             }
           }
-          __builtin_amdgcn_wave_barrier();
         }
-        chunk_top = j - 1;
-        chunk_any = false;
+        __builtin_amdgcn_wave_barrier();
+        n_slots = 0;
+        slots = 0;
       }
     }
     __syncthreads();
-    flush_grads<SHN>(a, s_id, s_acc, cnt);
+    for (int e = t; e < cnt * NCOMP; e += kTilePixels) {
+      const int jj = e / NCOMP;
+      const int c = e - jj * NCOMP;
+      float v;
+      if (c < 6) {
+        v = s_acc[jj * 6 + c];
+      } else {
+        const int o = jj * SHN + c - 6;
+        v = ((s_shacc[0][o] + s_shacc[1][o]) + s_shacc[2][o]) + s_shacc[3][o];
+      }
+      if (v != 0.0f) emit_grad<SHN>(a, s_id[jj], c, v);
+    }
   }
 }
 
