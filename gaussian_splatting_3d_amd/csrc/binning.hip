@@ -11,9 +11,9 @@
 //   3. the pairs are never written in depth order: the expansion (each output
 //      slot -> its Gaussian and tile, Gaussians in depth order, each
 //      Gaussian's tiles x-outer, y-inner) runs twice, once to count the low
-//      7 tile-id bits per 4096-slot block and once to rank the block's pairs
+//      7 tile-id bits per 2048-slot block and once to rank the block's pairs
 //      on those bits and scatter them as the first radix pass;
-//   4. the remaining tile-id bits in 8-bit passes; the last pass also
+//   4. the remaining tile-id bits in passes of at most 8; the last pass also
 //      records each tile's first and last position (atomic min / max of the
 //      block-local run ends), so no separate range pass reads the keys.
 // Within a tile the entries are then ordered by (depth bits, index), i.e.
@@ -551,9 +551,32 @@ static BinWs carve_bin_ws(void* ws, int64_t V, int64_t P) {
   return b;
 }
 
+// One radix pass over the pair list on tile-id bits [shift, shift + RB).
+template <int RB>
+static int pair_pass(const BinWs& w, int64_t P, int nblk, int shift, bool last, uint32_t* kin, int32_t* vin,
+                     uint32_t* kout, int32_t* vout, int32_t* start, int32_t* end, unsigned nt, hipStream_t st) {
+  hipLaunchKernelGGL((k_radix_hist<uint32_t, RB>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, P, shift, w.hist,
+                     nblk);
+  GSMI_CHECK_LAUNCH();
+  int rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(w.hist), reinterpret_cast<int32_t*>(w.hist),
+                                   (int64_t)nblk << RB, nullptr, w.scan_ws, st);
+  if (rc) return rc;
+  if (last) {
+    hipLaunchKernelGGL((k_radix_scatter<uint32_t, RB, true>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin,
+                       kout, vout, P, shift, w.hist, nblk, start, end, nt);
+  } else {
+    hipLaunchKernelGGL((k_radix_scatter<uint32_t, RB, false>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin,
+                       kout, vout, P, shift, w.hist, nblk, nullptr, nullptr, nt);
+  }
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
 // Pair sort on the tile id: pass 1 fused with the expansion (low 7 bits),
-// then 8-bit passes; the last pass writes ids (and tiles when asked) and the
-// ranges.  Returns the sorted tile ids in *tiles (null unless want_tiles).
+// then the remaining bits in passes of at most 8, split evenly (13 bits at
+// 1920x1080: one 6-bit pass, 64 runs per block instead of 256); the last
+// pass writes ids (and tiles when asked) and the ranges.  Returns the sorted
+// tile ids in *tiles (null unless want_tiles).
 static int pair_sort(const BinWs& w, int64_t V, int64_t P, int n_tiles, int32_t* ids,
                      int32_t* start, int32_t* end, bool want_tiles, uint32_t** tiles, hipStream_t st) {
   const int nblk = (int)pair_blocks(P);
@@ -583,27 +606,32 @@ static int pair_sort(const BinWs& w, int64_t V, int64_t P, int n_tiles, int32_t*
   GSMI_CHECK_LAUNCH();
   uint32_t *kin = w.ka, *kout = w.kb;
   int32_t *vin = w.va, *vout = w.vb;
-  for (int shift = kExpandDigitBits; shift < bits; shift += 8) {
-    const bool last = shift + 8 >= bits;
-    hipLaunchKernelGGL((k_radix_hist<uint32_t, 8>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, P, shift, w.hist,
-                       nblk);
-    GSMI_CHECK_LAUNCH();
-    rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(w.hist), reinterpret_cast<int32_t*>(w.hist),
-                                 (int64_t)nblk * 256, nullptr, w.scan_ws, st);
+  const int rem = bits - kExpandDigitBits;
+  const int npass = (rem + 7) / 8;
+  int shift = kExpandDigitBits;
+  for (int pass = 0; pass < npass; ++pass) {
+    const bool last = pass == npass - 1;
+    const int left = bits - shift;
+    const int rb = (left + (npass - pass) - 1) / (npass - pass);  // ceil(left / passes left), <= 8
+    uint32_t* ko = last ? (want_tiles ? kout : nullptr) : kout;
+    int32_t* vo = last ? ids : vout;
+    // Fewer than 4 bits run as 4: the extra digit bits are above the tile
+    // id's width (zero) or belong to the next pass, which re-sorts them stably.
+    switch (rb) {
+      case 5: rc = pair_pass<5>(w, P, nblk, shift, last, kin, vin, ko, vo, start, end, nt, st); break;
+      case 6: rc = pair_pass<6>(w, P, nblk, shift, last, kin, vin, ko, vo, start, end, nt, st); break;
+      case 7: rc = pair_pass<7>(w, P, nblk, shift, last, kin, vin, ko, vo, start, end, nt, st); break;
+      case 8: rc = pair_pass<8>(w, P, nblk, shift, last, kin, vin, ko, vo, start, end, nt, st); break;
+      default: rc = pair_pass<4>(w, P, nblk, shift, last, kin, vin, ko, vo, start, end, nt, st); break;
+    }
     if (rc) return rc;
     if (last) {
-      uint32_t* ko = want_tiles ? kout : nullptr;
-      hipLaunchKernelGGL((k_radix_scatter<uint32_t, 8, true>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin, ko,
-                         ids, P, shift, w.hist, nblk, start, end, nt);
-      GSMI_CHECK_LAUNCH();
       *tiles = ko;
     } else {
-      hipLaunchKernelGGL((k_radix_scatter<uint32_t, 8, false>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin,
-                         kout, vout, P, shift, w.hist, nblk, nullptr, nullptr, nt);
-      GSMI_CHECK_LAUNCH();
       std::swap(kin, kout);
       std::swap(vin, vout);
     }
+    shift += rb;
   }
   return GSMI_OK;
 }

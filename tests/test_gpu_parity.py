@@ -199,6 +199,31 @@ def test_forward_background(orc):
     assert np.array_equal(st.pre_bg.cpu().numpy(), ref["pre_bg"])
 
 
+@pytest.mark.parametrize("nth,ntw", [(64, 128), (181, 181), (300, 300)])
+def test_binning_wide_tile_grids(orc, nth, ntw):
+    """Tile ids of 13, 15 and 17 bits: one, one and two radix passes after
+    the fused first pass, with digit widths 6, 8 and 5 + 5."""
+    rng = np.random.default_rng(nth)
+    V = 20000
+    lo = rng.integers(0, [ntw, nth], size=(V, 2)).astype(np.int32)
+    hi = np.minimum(lo + rng.integers(0, 5, size=(V, 2)), [ntw - 1, nth - 1]).astype(np.int32)
+    lo[0], hi[0] = (0, 0), (min(ntw - 1, 40), min(nth - 1, 40))
+    depth = rng.choice(np.array([0.5, 1.0, 2.0, 3.25], np.float32), size=V).astype(np.float32)
+    depth[::7] = rng.random(depth[::7].size).astype(np.float32)
+    want = orc.bin_gaussians(lo, hi, depth, nth, ntw)
+    P = int(want["ids"].size)
+    ids = torch.zeros(P, dtype=torch.int32, device=DEV)
+    start = torch.empty(nth * ntw, dtype=torch.int32, device=DEV)
+    end = torch.empty(nth * ntw, dtype=torch.int32, device=DEV)
+    keys = torch.empty(P, dtype=torch.int64, device=DEV)
+    _backend.tile_culling_aabb_start_end(torch.from_numpy(lo).to(DEV), torch.from_numpy(hi).to(DEV), ids, start,
+                                         end, torch.from_numpy(depth).reshape(-1, 1).to(DEV), nth, ntw,
+                                         sorted_keys=keys)
+    assert np.array_equal(keys.cpu().numpy().view(np.uint64), want["sorted_keys"])
+    assert np.array_equal(ids.cpu().numpy(), want["ids"])
+    assert np.array_equal(start.cpu().numpy(), want["start"]) and np.array_equal(end.cpu().numpy(), want["end"])
+
+
 def test_binning_ties_and_ragged_counts(orc):
     """Equal depths (ties broken by index), one-tile and image-wide boxes,
     zero Gaussians, through the drop-in binning entry point."""
