@@ -34,16 +34,22 @@ namespace gsmi {
 constexpr int kRadixThreads = 256;
 constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kRadixThreads * kRadixItems;  // 2048 keys per block
+// Radix and expansion kernels map blockIdx to data blocks with xcd_tile, so
+// each XCD works on a contiguous range: the [digit][block] histogram entries
+// and the digit runs that neighbouring blocks write are then adjacent in one
+// L2 (measured at 58.7M pairs: final scatter 284 -> 171 us).
 
 template <typename K, int RB>
 __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const K* __restrict__ keys, int64_t P, int shift,
                                                               uint32_t* __restrict__ hist, int nblk) {
   constexpr int BINS = 1 << RB;
   __shared__ uint32_t s_cnt[4][BINS];
+  const int blk = xcd_tile(blockIdx.x, nblk);
+  if (blk >= nblk) return;
   const int t = threadIdx.x, w = t >> 6;
   for (int i = t; i < 4 * BINS; i += kRadixThreads) (&s_cnt[0][0])[i] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+  const int64_t base = (int64_t)blk * kRadixTile;
 #pragma unroll 4
   for (int i = 0; i < kRadixItems; ++i) {
     const int64_t p = base + t + (int64_t)i * kRadixThreads;
@@ -53,7 +59,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const K* __restric
     }
   }
   __syncthreads();
-  if (t < BINS) hist[(int64_t)t * nblk + blockIdx.x] = s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
+  if (t < BINS) hist[(int64_t)t * nblk + blk] = s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
 }
 
 template <typename K, int RB, int TILE>
@@ -80,12 +86,12 @@ __device__ __forceinline__ void block_scatter(ScatterLds<K, RB, kRadixThreads * 
                                               int32_t (&val)[ITEMS], int64_t base, int64_t P, int shift,
                                               const uint32_t* __restrict__ offs, int nblk, K* __restrict__ kout,
                                               int32_t* __restrict__ vout, int32_t* __restrict__ start,
-                                              int32_t* __restrict__ end, uint32_t n_tiles) {
+                                              int32_t* __restrict__ end, uint32_t n_tiles, int blk) {
   constexpr int BINS = 1 << RB;
   static_assert(BINS <= kRadixThreads, "one digit per thread in the block prefix");
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   for (int i = lane; i < BINS; i += 64) s.cnt[w][i] = 0;
-  if (t < BINS) s.goff[t] = offs[(int64_t)t * nblk + blockIdx.x];
+  if (t < BINS) s.goff[t] = offs[(int64_t)t * nblk + blk];
   __syncthreads();
 
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -175,8 +181,10 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __rest
                                                                  int32_t* __restrict__ start,
                                                                  int32_t* __restrict__ end, uint32_t n_tiles) {
   __shared__ ScatterLds<K, RB, kRadixTile> s;
+  const int blk = xcd_tile(blockIdx.x, nblk);
+  if (blk >= nblk) return;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+  const int64_t base = (int64_t)blk * kRadixTile;
   K key[kRadixItems];
   int32_t val[kRadixItems];
 #pragma unroll
@@ -187,7 +195,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __rest
     val[r] = valid ? vin[p] : 0;
   }
   block_scatter<K, RB, RANGES, kRadixItems>(s, key, val, base, P, shift, offs, nblk, kout, vout, start, end,
-                                            n_tiles);
+                                            n_tiles, blk);
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -297,9 +305,8 @@ __global__ void k_block_first(const int32_t* __restrict__ off, int64_t V, int64_
 // (inclusive: the last may start exactly at the next block); returns
 // (r0, n) = first depth rank and count of staged Gaussians.
 __device__ __forceinline__ int2 expand_stage(ExpandLds& s, const int32_t* __restrict__ off,
-                                             const int32_t* __restrict__ first, int64_t V, int nblk) {
+                                             const int32_t* __restrict__ first, int64_t V, int nblk, int b) {
   const int t = threadIdx.x;
-  const int b = blockIdx.x;
   const int hiV = (int)(V - 1);
   const int r0 = min(max(first[b], 0), hiV);
   const int r1 = b + 1 < nblk ? min(max(first[b + 1], r0), hiV) : hiV;
@@ -314,11 +321,11 @@ __device__ __forceinline__ int2 expand_stage(ExpandLds& s, const int32_t* __rest
 // Tiles and indices of this thread's slots p0 + t*kExpandItems + k; invalid
 // slots (past P) get tile 0xffffffff.
 __device__ __forceinline__ void expand_slots(const ExpandLds& s, int2 rn, const int4* __restrict__ rect, int64_t P,
-                                             uint32_t (&tile)[kExpandItems],
+                                             int blk, uint32_t (&tile)[kExpandItems],
                                              int32_t (&val)[kExpandItems]) {
   const int t = threadIdx.x;
   const int r0 = rn.x, n = rn.y;
-  int64_t p = (int64_t)blockIdx.x * kExpandTile + (int64_t)t * kExpandItems;
+  int64_t p = (int64_t)blk * kExpandTile + (int64_t)t * kExpandItems;
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -355,17 +362,19 @@ __global__ __launch_bounds__(kRadixThreads) void k_expand_hist(const int32_t* __
   constexpr int BINS = 1 << kExpandDigitBits;
   __shared__ ExpandLds s;
   __shared__ uint32_t s_cnt[4][BINS];
+  const int blk = xcd_tile(blockIdx.x, nblk);
+  if (blk >= nblk) return;
   const int t = threadIdx.x, w = t >> 6;
   for (int i = t; i < 4 * BINS; i += kRadixThreads) (&s_cnt[0][0])[i] = 0;
-  const int2 rn = expand_stage(s, off, first, V, nblk);
+  const int2 rn = expand_stage(s, off, first, V, nblk, blk);
   uint32_t tile[kExpandItems];
   int32_t val[kExpandItems];
-  expand_slots(s, rn, rect, P, tile, val);
+  expand_slots(s, rn, rect, P, blk, tile, val);
 #pragma unroll
   for (int k = 0; k < kExpandItems; ++k)
     if (tile[k] != 0xffffffffu) atomicAdd(&s_cnt[w][tile[k] & (BINS - 1)], 1u);
   __syncthreads();
-  if (t < BINS) hist[(int64_t)t * nblk + blockIdx.x] = s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
+  if (t < BINS) hist[(int64_t)t * nblk + blk] = s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
 }
 
 // Block position q of the staged expansion -> LDS word (64-word XOR
@@ -388,12 +397,14 @@ __global__ __launch_bounds__(kRadixThreads) void k_expand_scatter(const int32_t*
                                                                   int32_t* __restrict__ end, uint32_t n_tiles) {
   __shared__ ExpandLds se;
   __shared__ ScatterLds<uint32_t, kExpandDigitBits, kExpandTile> s;
+  const int blk = xcd_tile(blockIdx.x, nblk);
+  if (blk >= nblk) return;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int2 rn = expand_stage(se, off, first, V, nblk);
+  const int2 rn = expand_stage(se, off, first, V, nblk, blk);
   {
     uint32_t tile[kExpandItems];
     int32_t val[kExpandItems];
-    expand_slots(se, rn, rect, P, tile, val);
+    expand_slots(se, rn, rect, P, blk, tile, val);
 #pragma unroll
     for (int k = 0; k < kExpandItems; ++k) {
       const int q = swz(t * kExpandItems + k);
@@ -411,8 +422,8 @@ __global__ __launch_bounds__(kRadixThreads) void k_expand_scatter(const int32_t*
     val[r] = s.vals[q];
   }
   // block_scatter's first barrier orders these reads before its LDS writes
-  block_scatter<uint32_t, kExpandDigitBits, RANGES, kExpandItems>(s, key, val, (int64_t)blockIdx.x * kExpandTile, P,
-                                                                  0, offs, nblk, kout, vout, start, end, n_tiles);
+  block_scatter<uint32_t, kExpandDigitBits, RANGES, kExpandItems>(s, key, val, (int64_t)blk * kExpandTile, P, 0,
+                                                                  offs, nblk, kout, vout, start, end, n_tiles, blk);
 }
 
 // start == INT32_MAX (no pair seen by the atomic-min) -> -1.
@@ -467,13 +478,14 @@ static int radix_sort(int64_t P, K* keys, int32_t* values, int end_bit, void* ws
   int32_t* vout = valt;
   for (int pass = 0; pass < npass; ++pass) {
     const int shift = RB * pass;
-    hipLaunchKernelGGL((k_radix_hist<K, RB>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, P, shift, hist, nblk);
+    hipLaunchKernelGGL((k_radix_hist<K, RB>), dim3(xcd_grid(nblk)), dim3(kRadixThreads), 0, st, kin, P, shift, hist,
+                       nblk);
     GSMI_CHECK_LAUNCH();
     int rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(hist), reinterpret_cast<int32_t*>(hist), nh,
                                      nullptr, scan_ws, st);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_radix_scatter<K, RB, false>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin, kout,
-                       vout, P, shift, hist, nblk, nullptr, nullptr, 0u);
+    hipLaunchKernelGGL((k_radix_scatter<K, RB, false>), dim3(xcd_grid(nblk)), dim3(kRadixThreads), 0, st, kin, vin,
+                       kout, vout, P, shift, hist, nblk, nullptr, nullptr, 0u);
     GSMI_CHECK_LAUNCH();
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -555,18 +567,18 @@ static BinWs carve_bin_ws(void* ws, int64_t V, int64_t P) {
 template <int RB>
 static int pair_pass(const BinWs& w, int64_t P, int nblk, int shift, bool last, uint32_t* kin, int32_t* vin,
                      uint32_t* kout, int32_t* vout, int32_t* start, int32_t* end, unsigned nt, hipStream_t st) {
-  hipLaunchKernelGGL((k_radix_hist<uint32_t, RB>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, P, shift, w.hist,
-                     nblk);
+  const dim3 grid(xcd_grid(nblk)), block(kRadixThreads);
+  hipLaunchKernelGGL((k_radix_hist<uint32_t, RB>), grid, block, 0, st, kin, P, shift, w.hist, nblk);
   GSMI_CHECK_LAUNCH();
   int rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(w.hist), reinterpret_cast<int32_t*>(w.hist),
                                    (int64_t)nblk << RB, nullptr, w.scan_ws, st);
   if (rc) return rc;
   if (last) {
-    hipLaunchKernelGGL((k_radix_scatter<uint32_t, RB, true>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin,
-                       kout, vout, P, shift, w.hist, nblk, start, end, nt);
+    hipLaunchKernelGGL((k_radix_scatter<uint32_t, RB, true>), grid, block, 0, st, kin, vin, kout, vout, P, shift,
+                       w.hist, nblk, start, end, nt);
   } else {
-    hipLaunchKernelGGL((k_radix_scatter<uint32_t, RB, false>), dim3(nblk), dim3(kRadixThreads), 0, st, kin, vin,
-                       kout, vout, P, shift, w.hist, nblk, nullptr, nullptr, nt);
+    hipLaunchKernelGGL((k_radix_scatter<uint32_t, RB, false>), grid, block, 0, st, kin, vin, kout, vout, P, shift,
+                       w.hist, nblk, nullptr, nullptr, nt);
   }
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
@@ -587,22 +599,22 @@ static int pair_sort(const BinWs& w, int64_t V, int64_t P, int n_tiles, int32_t*
   static_assert((kRadixTile / kExpandTile) * (1 << kExpandDigitBits) <= 256, "pass-1 histogram exceeds the buffer");
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nblk1 + 255) / 256)), dim3(256), 0, st, w.first, (int64_t)nblk1, -1);
   hipLaunchKernelGGL(k_block_first, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, w.cnt, V, P, w.first, nblk1);
-  hipLaunchKernelGGL(k_expand_hist, dim3(nblk1), dim3(kRadixThreads), 0, st, w.cnt, w.first, w.rect, V, P, w.hist,
-                     nblk1);
+  const dim3 grid1(xcd_grid(nblk1)), block1(kRadixThreads);
+  hipLaunchKernelGGL(k_expand_hist, grid1, block1, 0, st, w.cnt, w.first, w.rect, V, P, w.hist, nblk1);
   GSMI_CHECK_LAUNCH();
   int rc = exclusive_scan<int32_t>(reinterpret_cast<int32_t*>(w.hist), reinterpret_cast<int32_t*>(w.hist),
                                    (int64_t)nblk1 << kExpandDigitBits, nullptr, w.scan_ws, st);
   if (rc) return rc;
   if (bits <= kExpandDigitBits) {
     uint32_t* ko = want_tiles ? w.ka : nullptr;
-    hipLaunchKernelGGL((k_expand_scatter<true>), dim3(nblk1), dim3(kRadixThreads), 0, st, w.cnt, w.first, w.rect, V,
-                       P, w.hist, nblk1, ko, ids, start, end, nt);
+    hipLaunchKernelGGL((k_expand_scatter<true>), grid1, block1, 0, st, w.cnt, w.first, w.rect, V, P, w.hist, nblk1,
+                       ko, ids, start, end, nt);
     GSMI_CHECK_LAUNCH();
     *tiles = ko;
     return GSMI_OK;
   }
-  hipLaunchKernelGGL((k_expand_scatter<false>), dim3(nblk1), dim3(kRadixThreads), 0, st, w.cnt, w.first, w.rect, V,
-                     P, w.hist, nblk1, w.ka, w.va, nullptr, nullptr, nt);
+  hipLaunchKernelGGL((k_expand_scatter<false>), grid1, block1, 0, st, w.cnt, w.first, w.rect, V, P, w.hist, nblk1,
+                     w.ka, w.va, nullptr, nullptr, nt);
   GSMI_CHECK_LAUNCH();
   uint32_t *kin = w.ka, *kout = w.kb;
   int32_t *vin = w.va, *vout = w.vb;
