@@ -799,71 +799,86 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     uint32_t slots = 0;   // batch position of slot m in bits [kSlotBits m, kSlotBits (m + 1))
     for (int j = cnt - 1; j >= 0; --j) {
       const int idx = base + j;
-      float cc0 = 0.0f, cc1 = 0.0f, cc2 = 0.0f;
-      float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      bool contrib = false;
-      if (__any(idx < nd) && idx < nd) {
+      // Per-lane decisions are selects, not branches: the wave runs the
+      // entry when any lane needs it, and lanes that do not contribute keep
+      // their state (a divergent branch here cost more in exec-mask handling
+      // and re-zeroing than the masked arithmetic it skipped).
+      const bool live = idx < nd;
+      if (__any(live)) {
         const GeoA ga = s_ga[j];
         const GeoB gb = s_gb[j];
         const float2 gc = s_gc[j];
+        const float al = gc.x;
         const float dx = px - ga.mx, dy = py - ga.my;
         const float tmpx = dx * gb.c3 - dy * gb.c2;
         const float tmpy = -dx * ga.c1 + dy * ga.c0;
         const float q = tmpx * dx + tmpy * dy;
-        if (!(q * gb.inv_det > gc.y)) {
-          const float al = gc.x;
-          const float G = gauss_decided(q, gb.det, gb.inv_det, al);
-          const float aG = al * G;
-          if (!(aG < kMinRenderAlpha)) {
-            contrib = true;
-            const float om = 1.0f - aG;
-            const float inv_om = __builtin_amdgcn_rcpf(om);
-            const float Tj = T * inv_om;
-            float w = al * Tj * G;
-            if (isnan(w)) w = 0.0f;
-            const float* shj = s_sh + j * SHN;
-            // w is finite, a sigmoid NaN or in [0, 1]: isnan(y * w) == isnan(y)
-            const float y0 = fmaxf(sigmoid_fast(dot_fast<CC>(shj, Y)), 0.0f);
-            const float y1 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + CC, Y)), 0.0f);
-            const float y2 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y)), 0.0f);
-            cc0 = w * (y0 * (1.0f - y0)) * g0;
-            cc1 = w * (y1 * (1.0f - y1)) * g1;
-            cc2 = w * (y2 * (1.0f - y2)) * g2;
-            float P = g0 * (y0 * Tj - S0 * inv_om);
-            P = fmaf(g1, y1 * Tj - S1 * inv_om, P);
-            P = fmaf(g2, y2 * Tj - S2 * inv_om, P);
-            S0 = fmaf(w, y0, S0);
-            S1 = fmaf(w, y1, S1);
-            S2 = fmaf(w, y2, S2);
-            const float D = P * aG;
-            const float s = tmpx * gb.inv_det;
-            const float u = tmpy * gb.inv_det;
-            v[0] = D * s;
-            v[1] = D * u;
-            v[2] = 0.5f * v[0] * s;
-            v[3] = 0.5f * v[0] * u;
-            v[4] = 0.5f * v[1] * u;
-            v[5] = P * G;
-            T = Tj;
+        const float rq = q * gb.inv_det;
+        const bool near = live && !(rq > gc.y);
+        // gauss_decided: hardware exp, the exact sequence only where the skip
+        // decision is within 2^-12 of the threshold (or NaN, or q == 0).
+        float G = __builtin_amdgcn_exp2f((rq < 0.0f ? 1000.0f : rq) * -0.72134752044448170f);
+        {
+          const float aG0 = al * G;
+          const bool exact = near && ((aG0 > kMinRenderAlpha * (1.0f - 0x1p-12f) &&
+                                       aG0 < kMinRenderAlpha * (1.0f + 0x1p-12f)) ||
+                                      !(aG0 == aG0) || q == 0.0f);
+          if (__any(exact)) {
+            if (exact) G = gauss_exact(q, gb.det);
           }
         }
-      }
-      if (__any(contrib)) {
-        const int m = n_slots;
-        uint32_t h, l;
-        split_bf16(cc0, h, l);
-        cc_w[cc_pos(0 * kChunk + m, lane)] = h;
-        cc_w[kCcPlane + cc_pos(0 * kChunk + m, lane)] = l;
-        split_bf16(cc1, h, l);
-        cc_w[cc_pos(1 * kChunk + m, lane)] = h;
-        cc_w[kCcPlane + cc_pos(1 * kChunk + m, lane)] = l;
-        split_bf16(cc2, h, l);
-        cc_w[cc_pos(2 * kChunk + m, lane)] = h;
-        cc_w[kCcPlane + cc_pos(2 * kChunk + m, lane)] = l;
-        slots |= static_cast<uint32_t>(j) << (kSlotBits * m);
-        n_slots = m + 1;
-        const float r = wave_reduce_scatter8_swap(v, lane);
-        if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r);
+        const float aG = al * G;
+        const bool contrib = near && !(aG < kMinRenderAlpha);
+        if (__any(contrib)) {
+          const float om = 1.0f - aG;
+          const float inv_om = __builtin_amdgcn_rcpf(om);
+          const float Tj = T * inv_om;
+          float w = al * Tj * G;
+          w = isnan(w) ? 0.0f : w;
+          const float* shj = s_sh + j * SHN;
+          // w is finite, a sigmoid NaN or in [0, 1]: isnan(y * w) == isnan(y)
+          const float y0 = fmaxf(sigmoid_fast(dot_fast<CC>(shj, Y)), 0.0f);
+          const float y1 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + CC, Y)), 0.0f);
+          const float y2 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y)), 0.0f);
+          const float cc0 = contrib ? w * (y0 * (1.0f - y0)) * g0 : 0.0f;
+          const float cc1 = contrib ? w * (y1 * (1.0f - y1)) * g1 : 0.0f;
+          const float cc2 = contrib ? w * (y2 * (1.0f - y2)) * g2 : 0.0f;
+          float P = g0 * (y0 * Tj - S0 * inv_om);
+          P = fmaf(g1, y1 * Tj - S1 * inv_om, P);
+          P = fmaf(g2, y2 * Tj - S2 * inv_om, P);
+          S0 = contrib ? fmaf(w, y0, S0) : S0;
+          S1 = contrib ? fmaf(w, y1, S1) : S1;
+          S2 = contrib ? fmaf(w, y2, S2) : S2;
+          T = contrib ? Tj : T;
+          const float D = P * aG;
+          const float sx = tmpx * gb.inv_det;
+          const float uy = tmpy * gb.inv_det;
+          const float Ds = D * sx, Du = D * uy;
+          float v[8];
+          v[0] = contrib ? Ds : 0.0f;
+          v[1] = contrib ? Du : 0.0f;
+          v[2] = contrib ? 0.5f * Ds * sx : 0.0f;
+          v[3] = contrib ? 0.5f * Ds * uy : 0.0f;
+          v[4] = contrib ? 0.5f * Du * uy : 0.0f;
+          v[5] = contrib ? P * G : 0.0f;
+          v[6] = 0.0f;
+          v[7] = 0.0f;
+          const int m = n_slots;
+          uint32_t h, l;
+          split_bf16(cc0, h, l);
+          cc_w[cc_pos(0 * kChunk + m, lane)] = h;
+          cc_w[kCcPlane + cc_pos(0 * kChunk + m, lane)] = l;
+          split_bf16(cc1, h, l);
+          cc_w[cc_pos(1 * kChunk + m, lane)] = h;
+          cc_w[kCcPlane + cc_pos(1 * kChunk + m, lane)] = l;
+          split_bf16(cc2, h, l);
+          cc_w[cc_pos(2 * kChunk + m, lane)] = h;
+          cc_w[kCcPlane + cc_pos(2 * kChunk + m, lane)] = l;
+          slots |= static_cast<uint32_t>(j) << (kSlotBits * m);
+          n_slots = m + 1;
+          const float r = wave_reduce_scatter8_swap(v, lane);
+          if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r);
+        }
       }
       if (n_slots == kChunk || (j == 0 && n_slots > 0)) {
         // LDS operations of one wave complete in order; the barrier keeps the
