@@ -334,7 +334,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the dominant kernel's HBM bytes")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement of the default run")
-    ap.add_argument("--serve-levels", default="1,2,3,4,6",
+    ap.add_argument("--serve-levels", default="0.7,0.85,1,1.15,1.3,1.6,2",
                     help="server arrival rates as multiples of the plain per-call capacity")
     ap.add_argument("--serve-duration", type=float, default=2.0, help="seconds of arrivals per serving level")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
