@@ -683,7 +683,7 @@ static_assert(kMfmaBatch <= (1 << kSlotBits) && kChunk * kSlotBits <= 32, "slot 
 
 // Position (bf16 units) of pixel p in staged row `row`: 8-pixel blocks XOR-swizzled by row.
 __device__ __forceinline__ int cc_pos(int row, int p) {
-  return row * kCcStride + ((((p >> 3) ^ row) & 7) << 3) + (p & 7);
+  return row * kCcStride + (p ^ ((row & 7) << 3));  // p < 64: block p / 8 -> (p / 8) ^ (row & 7)
 }
 
 // Upper 16 bits of x (x truncated to bf16) and of the remainder x - hi.
