@@ -71,7 +71,7 @@ class Timed:
 
 
 def gpu_bench(scene, steps, warmup, exact=False):
-    from gaussian_splatting_3d_amd import _lib, rasterize, scenes
+    from gaussian_splatting_3d_amd import _lib, rasterize, scenes, training
     from gaussian_splatting_3d_amd.backend import _backend
 
     dev = torch.device("cuda:0")
@@ -97,11 +97,11 @@ def gpu_bench(scene, steps, warmup, exact=False):
             return orig_bwd(*a, **k)
 
     _backend.sh_forward, _backend.sh_backward = fwd_wrap, bwd_wrap
-    numel = target.numel()
 
     def step():
         out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg, exact=exact)
-        grad_out = torch.sign(out - target) / numel  # d/d out of mean |out - target|
+        # mean |out - target| and its gradient sign(out - target) / numel, one fused pass
+        _, grad_out = training.photometric_loss(out, target.view_as(out), "l1")
         grads = rasterize.backward_view(st, *params, grad_out)
         return st, grads
 
@@ -137,7 +137,7 @@ def gpu_bench_views(scene, steps, warmup):
     """One training step over all cameras of the scene in one batch: batched
     forward, summed L1 against seeded targets, batched adjoint.  HIP events
     bracket the two compositing launches (on the stream they run on)."""
-    from gaussian_splatting_3d_amd import _lib, multiview, scenes
+    from gaussian_splatting_3d_amd import _lib, multiview, scenes, training
 
     dev = torch.device("cuda:0")
     cfg = scenes.RasterConfig()
@@ -164,7 +164,7 @@ def gpu_bench_views(scene, steps, warmup):
 
     def step():
         out, st = multiview.forward_views(*params, scene.cameras, scene.C, cfg)
-        grad_out = torch.sign(out - target) / out.numel()
+        _, grad_out = training.photometric_loss(out, target.view_as(out), "l1")
         grads = multiview.backward_views(st, *params, grad_out)
         return st, grads
 

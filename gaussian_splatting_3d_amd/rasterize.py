@@ -167,9 +167,12 @@ def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g
     b, cfg, ic = s.b, s.cfg, s.b.consts
     dev = mean.device
     L = _lib.load()
-    g_m2d = torch.zeros(b.V, 2, dtype=torch.float32, device=dev)
-    g_cov = torch.zeros(b.V, 2, 2, dtype=torch.float32, device=dev)
-    g_al = torch.zeros(b.V, dtype=torch.float32, device=dev)
+    # one fill for the three boundary gradients; cov first so every part
+    # keeps the alignment of a separate allocation (16 / 16 / 8 bytes)
+    bnd = torch.zeros(b.V * 7, dtype=torch.float32, device=dev)
+    g_cov = bnd[: 4 * b.V].view(b.V, 2, 2)
+    g_m2d = bnd[4 * b.V: 6 * b.V].view(b.V, 2)
+    g_al = bnd[6 * b.V:]
     if g_sh is None:
         g_sh = torch.zeros_like(sh_coeffs)
     cmax2 = sh_coeffs.shape[-1]
