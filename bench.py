@@ -297,6 +297,25 @@ def _pmc_pass(counter, child_args, timeout_s=90):
     return {name: (float(v), int(n)) for name, v, n in rows}
 
 
+# Non-packed FP32 VALU issue peak of MI355X: 256 CUs x 4 SIMDs x one wave64
+# instruction per 4 cycles at the 2.4 GHz peak engine clock.
+VALU_PEAK_INSTR_S = 256 * 4 * 2.4e9 / 4
+
+
+def measure_valu(workload, exact, kernel, kernel_ms):
+    """Wave-level VALU instructions per launch of `kernel` (rocprofv3
+    SQ_INSTS_VALU, summed over the chip) and their issue rate over the
+    kernel's HIP-event time, against the non-packed issue peak."""
+    child = ["--pmc-child", "--workload", workload] + (["--exact"] if exact else [])
+    run = _pmc_pass("SQ_INSTS_VALU", child)
+    vals = [v for k, (v, n) in run.items() if k.split("void ", 1)[-1].startswith(KERNEL_PREFIX[kernel])]
+    if not vals:
+        raise RuntimeError(f"{kernel} not seen in the SQ_INSTS_VALU run")
+    rate = vals[0] / (kernel_ms / 1e3)
+    return dict(instructions_per_launch=int(vals[0]), achieved_g_instr_s=round(rate / 1e9, 2),
+                peak_g_instr_s=round(VALU_PEAK_INSTR_S / 1e9, 1), frac=round(rate / VALU_PEAK_INSTR_S, 4))
+
+
 def measure_traffic(workload, exact, kernel):
     """HBM bytes per launch of `kernel` from rocprofv3 FETCH_SIZE and
     WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
@@ -412,6 +431,12 @@ def main():
         except Exception as e:  # counters unavailable: traffic stays null, say why
             line["roofline"]["traffic_error"] = str(e)
         print(f"[bench] traffic: {line['roofline'].get('traffic')}", file=sys.stderr, flush=True)
+        # the compositing kernels are VALU-issue bound, not HBM bound: report that rate too
+        try:
+            line["roofline"]["valu_issue"] = measure_valu(args.workload, args.exact, dom, line["kernels_ms"][dom])
+        except Exception as e:
+            line["roofline"]["valu_issue"] = {"error": str(e)}
+        print(f"[bench] valu: {line['roofline']['valu_issue']}", file=sys.stderr, flush=True)
     if args.workload == "dense_3m" and not args.no_serving:
         from gaussian_splatting_3d_amd import loadgen
 
