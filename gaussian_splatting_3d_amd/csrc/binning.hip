@@ -70,7 +70,7 @@ struct ScatterLds {
   uint32_t goff[1 << RB];
   uint32_t wsum[4];
   K keys[TILE];
-  int32_t vals[TILE];
+  alignas(16) int32_t vals[TILE];
 };
 
 // Stable scatter of one block of 256 * ITEMS items.  Item r of lane `lane` in
@@ -283,6 +283,11 @@ constexpr int kExpandDigitBits = 7;
 constexpr int kExpandItems = 8;
 constexpr int kExpandTile = kRadixThreads * kExpandItems;
 
+// Tile rectangles of the first kRectCache staged Gaussians are staged in LDS
+// too (a block holds ~100 Gaussians at 21 pairs each), so the expansion
+// reads them from LDS instead of a dependent global load per Gaussian.
+constexpr int kRectCache = 512;
+
 struct ExpandLds {
   int32_t off[kExpandTile + 2];
 };
@@ -302,10 +307,12 @@ __global__ void k_block_first(const int32_t* __restrict__ off, int64_t V, int64_
 }
 
 // Stages the offsets of the Gaussians from first[b] to first[b + 1]
-// (inclusive: the last may start exactly at the next block); returns
-// (r0, n) = first depth rank and count of staged Gaussians.
-__device__ __forceinline__ int2 expand_stage(ExpandLds& s, const int32_t* __restrict__ off,
-                                             const int32_t* __restrict__ first, int64_t V, int nblk, int b) {
+// (inclusive: the last may start exactly at the next block) and the
+// rectangles of the first kRectCache of them; returns (r0, n) = first depth
+// rank and count of staged Gaussians.
+__device__ __forceinline__ int2 expand_stage(ExpandLds& s, int4* s_rect, const int32_t* __restrict__ off,
+                                             const int4* __restrict__ rect, const int32_t* __restrict__ first,
+                                             int64_t V, int nblk, int b) {
   const int t = threadIdx.x;
   const int hiV = (int)(V - 1);
   const int r0 = min(max(first[b], 0), hiV);
@@ -314,15 +321,16 @@ __device__ __forceinline__ int2 expand_stage(ExpandLds& s, const int32_t* __rest
   // inconsistent bounds cannot index past the LDS array.
   const int n = max(1, min(r1 - r0 + 1, kExpandTile + 2));
   for (int i = t; i < n; i += kRadixThreads) s.off[i] = off[r0 + i];
+  for (int i = t; i < min(n, kRectCache); i += kRadixThreads) s_rect[i] = rect[r0 + i];
   __syncthreads();
   return make_int2(r0, n);
 }
 
 // Tiles and indices of this thread's slots p0 + t*kExpandItems + k; invalid
 // slots (past P) get tile 0xffffffff.
-__device__ __forceinline__ void expand_slots(const ExpandLds& s, int2 rn, const int4* __restrict__ rect, int64_t P,
-                                             int blk, uint32_t (&tile)[kExpandItems],
-                                             int32_t (&val)[kExpandItems]) {
+__device__ __forceinline__ void expand_slots(const ExpandLds& s, const int4* s_rect, int2 rn,
+                                             const int4* __restrict__ rect, int64_t P, int blk,
+                                             uint32_t (&tile)[kExpandItems], int32_t (&val)[kExpandItems]) {
   const int t = threadIdx.x;
   const int r0 = rn.x, n = rn.y;
   int64_t p = (int64_t)blk * kExpandTile + (int64_t)t * kExpandItems;
@@ -333,7 +341,7 @@ __device__ __forceinline__ void expand_slots(const ExpandLds& s, int2 rn, const 
     else hi = mid - 1;
   }
   int j = lo;
-  int4 rc = rect[r0 + j];
+  int4 rc = j < kRectCache ? s_rect[j] : rect[r0 + j];
 #pragma unroll
   for (int k = 0; k < kExpandItems; ++k, ++p) {
     tile[k] = 0xffffffffu;
@@ -341,7 +349,7 @@ __device__ __forceinline__ void expand_slots(const ExpandLds& s, int2 rn, const 
     if (p < P) {
       while (j + 1 < n && (int64_t)s.off[j + 1] <= p) {
         ++j;
-        rc = rect[r0 + j];
+        rc = j < kRectCache ? s_rect[j] : rect[r0 + j];
       }
       const int local = (int)(p - s.off[j]);
       const int ny = max(rc.z, 1);
@@ -361,15 +369,16 @@ __global__ __launch_bounds__(kRadixThreads) void k_expand_hist(const int32_t* __
                                                                int nblk) {
   constexpr int BINS = 1 << kExpandDigitBits;
   __shared__ ExpandLds s;
+  __shared__ int4 s_rect[kRectCache];
   __shared__ uint32_t s_cnt[4][BINS];
   const int blk = xcd_tile(blockIdx.x, nblk);
   if (blk >= nblk) return;
   const int t = threadIdx.x, w = t >> 6;
   for (int i = t; i < 4 * BINS; i += kRadixThreads) (&s_cnt[0][0])[i] = 0;
-  const int2 rn = expand_stage(s, off, first, V, nblk, blk);
+  const int2 rn = expand_stage(s, s_rect, off, rect, first, V, nblk, blk);
   uint32_t tile[kExpandItems];
   int32_t val[kExpandItems];
-  expand_slots(s, rn, rect, P, blk, tile, val);
+  expand_slots(s, s_rect, rn, rect, P, blk, tile, val);
 #pragma unroll
   for (int k = 0; k < kExpandItems; ++k)
     if (tile[k] != 0xffffffffu) atomicAdd(&s_cnt[w][tile[k] & (BINS - 1)], 1u);
@@ -400,11 +409,14 @@ __global__ __launch_bounds__(kRadixThreads) void k_expand_scatter(const int32_t*
   const int blk = xcd_tile(blockIdx.x, nblk);
   if (blk >= nblk) return;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int2 rn = expand_stage(se, off, first, V, nblk, blk);
+  int4* s_rect = reinterpret_cast<int4*>(s.vals);  // free until the pairs are staged below
+  static_assert(sizeof(s.vals) >= kRectCache * sizeof(int4), "rect cache does not fit");
+  const int2 rn = expand_stage(se, s_rect, off, rect, first, V, nblk, blk);
   {
     uint32_t tile[kExpandItems];
     int32_t val[kExpandItems];
-    expand_slots(se, rn, rect, P, blk, tile, val);
+    expand_slots(se, s_rect, rn, rect, P, blk, tile, val);
+    __syncthreads();  // every rect read before s.vals is overwritten
 #pragma unroll
     for (int k = 0; k < kExpandItems; ++k) {
       const int q = swz(t * kExpandItems + k);
