@@ -70,6 +70,8 @@ SIGNATURES = {
                                            _P]),
     "gsmi_loss_workspace_size": (_SZ, [_I64]),
     "gsmi_photometric_loss": (_I32, [_I64, _P, _P, _I32, _P, _P, _P, _SZ, _P]),
+    "gsmi_ssim_loss_workspace_size": (_SZ, [_I32, _I32, _I32]),
+    "gsmi_photometric_ssim_loss": (_I32, [_I32, _I32, _I32, _P, _P, _I32, _F, _I32, _P, _P, _P, _SZ, _P]),
     "gsmi_adam_step": (_I32, [_I32, _P, _P, _P, _P, _P, _P, _I32, _D, _D, _D, _I32, _P]),
     "gsmi_grad_stats": (_I32, [_I64, _P, _I32, _P, _I32, _P, _P]),
     "gsmi_count_visible": (_I32, [_I64, _P, _P, _P]),

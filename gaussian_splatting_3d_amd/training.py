@@ -24,8 +24,13 @@ from ._lib import check, ptr, stream_of
 GROUPS = ("mean", "qvec", "svec", "sh_coeffs", "alpha")
 
 
-def photometric_loss(out, gt, kind="l1", want_grad=True):
-    """(loss scalar tensor, d loss / d out or None)."""
+def photometric_loss(out, gt, kind="l1", want_grad=True, ssim_mult=0.0, ssim_win=11):
+    """(loss scalar tensor, d loss / d out or None).
+
+    get_loss_fn (utils/loss.py:5-22): ssim_mult * ssim_loss(out, gt, ssim_win)
+    + (1 - ssim_mult) * base, base = l1_loss or mse_loss (mean).  With
+    ssim_mult > 0 the images are [H, W, C] (the rasteriser's HWC layout) and
+    the gradient is always computed."""
     L = _lib.load()
     if out.shape != gt.shape:
         raise RuntimeError(f"image shapes differ: {tuple(out.shape)} vs {tuple(gt.shape)}")
@@ -33,8 +38,18 @@ def photometric_loss(out, gt, kind="l1", want_grad=True):
         raise NotImplementedError(kind)
     o = out.contiguous().float()
     g_t = gt.contiguous().float()
-    n = o.numel()
     loss = torch.empty((), dtype=torch.float32, device=o.device)
+    if ssim_mult:
+        if o.dim() != 3:
+            raise RuntimeError("the SSIM term needs images of shape [H, W, C]")
+        H, W, CH = (int(x) for x in o.shape)
+        grad = torch.empty_like(o)
+        ws = torch.empty(int(L.gsmi_ssim_loss_workspace_size(H, W, CH)), dtype=torch.uint8, device=o.device)
+        check(L.gsmi_photometric_ssim_loss(H, W, CH, ptr(o), ptr(g_t), int(ssim_win), float(ssim_mult),
+                                           1 if kind == "l2" else 0, ptr(grad), ptr(loss), ptr(ws), ws.numel(),
+                                           stream_of(o.device)), "photometric_ssim_loss")
+        return loss, grad
+    n = o.numel()
     grad = torch.empty_like(o) if want_grad else None
     ws = torch.empty(int(L.gsmi_loss_workspace_size(n)), dtype=torch.uint8, device=o.device)
     check(L.gsmi_photometric_loss(n, ptr(o), ptr(g_t), 1 if kind == "l2" else 0, ptr(grad), ptr(loss), ptr(ws),

@@ -349,6 +349,18 @@ size_t gsmi_loss_workspace_size(int64_t n);
 int gsmi_photometric_loss(int64_t n, const float *out, const float *gt, int l2, float *grad, float *loss,
                           void *workspace, size_t workspace_bytes, void *stream);
 
+size_t gsmi_ssim_loss_workspace_size(int H, int W, int CH);
+
+/* The whole loss of get_loss_fn (utils/loss.py:14-21) for images [H, W, CH]
+ * (HWC, CH <= 4): *loss = ssim_mult * ssim_loss(out, gt, win) +
+ * (1 - ssim_mult) * base (base as above), grad = d loss / d out.  ssim_loss
+ * is kornia's: mean of (1 - SSIM) / 2 with a win x win Gaussian window of
+ * sigma 1.5 (win odd, win / 2 < min(H, W), win <= 21), reflect padding,
+ * C1 = 1e-4, C2 = 9e-4, 1e-12 added to the denominator. */
+int gsmi_photometric_ssim_loss(int H, int W, int CH, const float *out, const float *gt, int win, float ssim_mult,
+                               int l2, float *grad, float *loss, void *workspace, size_t workspace_bytes,
+                               void *stream);
+
 /* One torch.optim.Adam step (gs/sh_renderer.py:720-729: per-group lr, betas
  * (0.9, 0.99), eps) over up to 8 parameter groups in one launch; arrays of
  * n_groups device pointers (host arrays).  fresh = 1: moments start at zero

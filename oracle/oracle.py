@@ -683,3 +683,99 @@ def prune_alpha(mean, qvec, log_s, logit, sh, alpha_thresh):
     _, a = activate(np.zeros((lg.size, 3), np.float32), lg)
     m = a >= np.float32(alpha_thresh)
     return [np.asarray(x, np.float32)[m] for x in (mean, qvec, log_s, logit, sh)], int(m.sum())
+
+
+# --------------------------------------------------------------------------
+# Photometric loss with the SSIM term (utils/loss.py:14-21).  ssim_loss is
+# kornia.losses.ssim_loss (third party, unpinned in requirements.txt, absent
+# here), restated from kornia's published algorithm (kornia.metrics.ssim):
+# 1-D Gaussian weights exp(-x^2 / (2 * 1.5^2)) for x = -win//2 .. win//2 in
+# float32, normalised by their sum, 2-D window = outer product; filter2d =
+# reflect padding by win//2 + correlation; C1 = (0.01)^2, C2 = (0.03)^2;
+# SSIM = (2 mu1 mu2 + C1)(2 s12 + C2) / ((mu1^2 + mu2^2 + C1)(s11 + s22 + C2)
+# + 1e-12); ssim_loss = mean((1 - SSIM) / 2).  Two independent restatements:
+# torch float64 (pad + conv2d, autograd for the gradient) and numpy float64
+# with scipy.ndimage.correlate1d in "mirror" mode (= torch's "reflect").
+# --------------------------------------------------------------------------
+
+SSIM_C1 = 0.01 ** 2
+SSIM_C2 = 0.03 ** 2
+
+
+def ssim_window1d(win, sigma=1.5):
+    """kornia's get_gaussian_kernel1d(win, sigma), float32."""
+    x = np.arange(win, dtype=np.float32) - np.float32(win // 2)
+    g = np.exp(-(x * x) / np.float32(2.0 * sigma * sigma)).astype(np.float32)
+    return (g / g.sum(dtype=np.float32)).astype(np.float32)
+
+
+def ssim_map_scipy(img1, img2, win):
+    """SSIM per pixel and channel of [H, W, C] images, float64, scipy filters."""
+    from scipy import ndimage
+
+    w = ssim_window1d(win).astype(np.float64)
+    a = np.asarray(img1, np.float64)
+    b = np.asarray(img2, np.float64)
+
+    def filt(x):
+        x = ndimage.correlate1d(x, w, axis=0, mode="mirror")
+        return ndimage.correlate1d(x, w, axis=1, mode="mirror")
+
+    mu1, mu2 = filt(a), filt(b)
+    s11 = filt(a * a) - mu1 * mu1
+    s22 = filt(b * b) - mu2 * mu2
+    s12 = filt(a * b) - mu1 * mu2
+    num = (2 * mu1 * mu2 + SSIM_C1) * (2 * s12 + SSIM_C2)
+    den = (mu1 * mu1 + mu2 * mu2 + SSIM_C1) * (s11 + s22 + SSIM_C2)
+    return num / (den + 1e-12)
+
+
+def ssim_map_t(img1, img2, win):
+    """The same in torch float64 (differentiable): [H, W, C] -> [H, W, C]."""
+    import torch
+    import torch.nn.functional as F
+
+    R = win // 2
+    w1 = torch.from_numpy(ssim_window1d(win).astype(np.float64))
+    k2 = (w1[:, None] * w1[None, :])[None, None]  # [1, 1, win, win]
+    a = img1.permute(2, 0, 1)[:, None]             # [C, 1, H, W]
+    b = img2.permute(2, 0, 1)[:, None]
+
+    def filt(x):
+        return F.conv2d(F.pad(x, (R, R, R, R), mode="reflect"), k2)
+
+    mu1, mu2 = filt(a), filt(b)
+    s11 = filt(a * a) - mu1 * mu1
+    s22 = filt(b * b) - mu2 * mu2
+    s12 = filt(a * b) - mu1 * mu2
+    num = (2 * mu1 * mu2 + SSIM_C1) * (2 * s12 + SSIM_C2)
+    den = (mu1 * mu1 + mu2 * mu2 + SSIM_C1) * (s11 + s22 + SSIM_C2)
+    return (num / (den + 1e-12))[:, 0].permute(1, 2, 0)
+
+
+def photometric_ssim_loss(out, gt, win=11, ssim_mult=0.2, l2=False):
+    """(loss, d loss / d out) of get_loss_fn in float64 (autograd over
+    ssim_map_t); images [H, W, C]."""
+    import torch
+
+    o = torch.tensor(np.asarray(out, np.float64), requires_grad=True)
+    g = torch.tensor(np.asarray(gt, np.float64))
+    ssim_l = ((1.0 - ssim_map_t(o, g, win)) / 2.0).mean()
+    base = ((o - g) ** 2).mean() if l2 else (o - g).abs().mean()
+    loss = ssim_mult * ssim_l + (1.0 - ssim_mult) * base
+    loss.backward()
+    return float(loss.detach()), o.grad.numpy()
+
+
+def reflect_filter_matrix(n, win):
+    """Dense [n, n] matrix of one axis of "reflect pad by win//2, correlate
+    with the 1-D window": row p = weights of input q in output p."""
+    R = win // 2
+    w = ssim_window1d(win).astype(np.float64)
+    M = np.zeros((n, n))
+    for p in range(n):
+        for d in range(-R, R + 1):
+            r = p + d
+            q = -r if r < 0 else (2 * (n - 1) - r if r > n - 1 else r)
+            M[p, q] += w[d + R]
+    return M
