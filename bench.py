@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -28,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
-WORKLOADS = ("dense_3m", "fwd_bwd_500k", "plain_forward_10k", "multiview_6m", "serve")
+WORKLOADS = ("dense_3m", "fwd_bwd_500k", "plain_forward_10k", "multiview_6m", "serve", "train_step_3m")
 
 
 def make_scene(name):
@@ -381,6 +382,9 @@ def main():
     if args.workload == "serve":
         serve_main(args)
         return
+    if args.workload == "train_step_3m":
+        train_main(args)
+        return
     scene = make_scene(args.workload)
     print(f"[bench] workload {scene.name}: N={scene.N}", file=sys.stderr, flush=True)
     views = len(scene.cameras)
@@ -449,6 +453,68 @@ def main():
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
     else:
         line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+def train_main(args):
+    """One whole training iteration of main_sh.py:141-240 at 3M Gaussians,
+    1920x1080: render, loss 0.8 L1 + 0.2 SSIM (window 11, the conf/*.yaml
+    values), adjoint, the visibility count and gradient statistics of
+    adaptive control (sh_renderer.py:215-216, 602-623), and the Adam step of
+    the optimizer the reference rebuilds every iteration (fresh moments,
+    per-group learning rates of conf/garden_sh.yaml:72-76)."""
+    from gaussian_splatting_3d_amd import densify, rasterize, scenes, training
+
+    scene = scenes.dense_3m()
+    dev = torch.device("cuda:0")
+    c2w, cam = scene.cameras[0]
+    cfg = scenes.RasterConfig()
+    params = [t.to(dev).contiguous() for t in (scene.mean, scene.qvec, scene.svec_before_activation,
+                                               scene.alpha_before_activation, scene.sh_coeffs)]
+    target = scenes.target_image(cam.h, cam.w, 3).to(dev).reshape(cam.h, cam.w, 3)
+    adam = training.FusedAdam(list(zip(params, (1e-3, 1e-2, 1e-2, 1e-2, 1e-2))), betas=(0.9, 0.99), fresh=True)
+    dc = densify.DensityControl(scene.N, device=dev)
+
+    def step():
+        out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg)
+        loss, grad_out = training.photometric_loss(out.view(cam.h, cam.w, 3), target, "l1", ssim_mult=0.2,
+                                                   ssim_win=11)
+        g = rasterize.backward_view(st, *params, grad_out.view(-1))
+        dc.count_visible(st.b.vis)
+        dc.update_grads(g[0])
+        adam.step(list(g[:5]))
+        return loss, st
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, st = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    assert math.isfinite(float(loss))
+    line = {
+        "metric": "training iterations/s @3M Gaussians 1080p (render, L1+SSIM loss, adjoint, density stats, Adam)",
+        "value": round(1e3 / ms, 3),
+        "unit": "it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene and target, random-init Gaussians)",
+        "config": {"workload": "train_step_3000k_1920x1080_sh3", "gaussians": scene.N, "visible": st.b.V,
+                   "pairs": st.b.P, "width": cam.w, "height": cam.h, "sh_degree": scene.C - 1,
+                   "ssim_mult": 0.2, "ssim_win": 11},
+        "mpix_s": round(cam.h * cam.w / (ms / 1e3) / 1e6, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
     print(json.dumps(line), flush=True)
 
 
