@@ -136,19 +136,20 @@ __device__ __forceinline__ float gauss_exact(float q, float det) {
   return exp_spec(-0.5f * radial);
 }
 
-// Gaussian value for the backward pass: hardware exponential of
-// -q/(2 det), except when a*G lies within a relative 2^-12 of the 1/255
-// skip threshold (or is not finite), where the exact value decides, so the
-// skip decision always equals the forward's.  Values feed gradients only.
-__device__ __forceinline__ float gauss_decided(float q, float det, float inv_det, float al) {
-  float r = q * inv_det;
-  if (r < 0.0f) r = 1000.0f;
-  const float G = __builtin_amdgcn_exp2f(r * -0.72134752044448170f);  // exp(-r/2)
-  const float aG = al * G;
-  const float lo = kMinRenderAlpha * (1.0f - 0x1p-12f), hi = kMinRenderAlpha * (1.0f + 0x1p-12f);
-  if (aG > lo && aG < hi) return gauss_exact(q, det);
-  if (!(aG == aG) || q == 0.0f) return gauss_exact(q, det);
-  return G;
+// Gaussian value for the backward pass: the hardware exponential of -r/2 with
+// r = q/det from the staged reciprocal (1000 for r < 0, as the reference).
+// Values feed gradients only; decisions use gauss_needs_exact.
+__device__ __forceinline__ float gauss_fast(float rq) {
+  return __builtin_amdgcn_exp2f((rq < 0.0f ? 1000.0f : rq) * -0.72134752044448170f);  // exp(-r/2)
+}
+
+// Where the fast value cannot take the forward's 1/255 skip decision: a*G
+// within a relative 2^-12 of the threshold, not a number, or q == 0; there
+// the exact sequence (gauss_exact) decides, so the decision always equals
+// the forward's.
+__device__ __forceinline__ bool gauss_needs_exact(float aG, float q) {
+  return (aG > kMinRenderAlpha * (1.0f - 0x1p-12f) && aG < kMinRenderAlpha * (1.0f + 0x1p-12f)) || !(aG == aG) ||
+         q == 0.0f;
 }
 
 // The reference's kernel_gaussian_2d (double) from float inputs.
@@ -815,17 +816,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         const float q = tmpx * dx + tmpy * dy;
         const float rq = q * gb.inv_det;
         const bool near = live && !(rq > gc.y);
-        // gauss_decided: hardware exp, the exact sequence only where the skip
-        // decision is within 2^-12 of the threshold (or NaN, or q == 0).
-        float G = __builtin_amdgcn_exp2f((rq < 0.0f ? 1000.0f : rq) * -0.72134752044448170f);
-        {
-          const float aG0 = al * G;
-          const bool exact = near && ((aG0 > kMinRenderAlpha * (1.0f - 0x1p-12f) &&
-                                       aG0 < kMinRenderAlpha * (1.0f + 0x1p-12f)) ||
-                                      !(aG0 == aG0) || q == 0.0f);
-          if (__any(exact)) {
-            if (exact) G = gauss_exact(q, gb.det);
-          }
+        float G = gauss_fast(rq);
+        const bool exact = near && gauss_needs_exact(al * G, q);
+        if (__any(exact)) {
+          if (exact) G = gauss_exact(q, gb.det);
         }
         const float aG = al * G;
         const bool contrib = near && !(aG < kMinRenderAlpha);
