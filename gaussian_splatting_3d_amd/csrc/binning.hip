@@ -94,25 +94,29 @@ __device__ __forceinline__ void block_scatter(ScatterLds<K, RB, kRadixThreads * 
   if (t < BINS) s.goff[t] = offs[(int64_t)t * nblk + blk];
   __syncthreads();
 
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
   constexpr int TILE = kRadixThreads * ITEMS;
   uint32_t rank[ITEMS];
   // Wave w ranks its 64 * ITEMS items 64 per round in position order, so
-  // ranks are stable.
+  // ranks are stable.  Lanes with the same digit ("peers") are found one
+  // digit bit at a time: m is the lane's bit as an all-zeros/all-ones word,
+  // and a lane keeps the ballot bits that agree with it, ~(ballot ^ m), in
+  // each 32-lane half.
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const uint32_t d = (uint32_t)(key[r] >> shift) & (BINS - 1);
-    uint64_t peers = ~0ull;
+    uint32_t plo = ~0u, phi = ~0u;
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
+      const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+      const uint64_t bal = __ballot(m != 0u);
+      plo &= ~((uint32_t)bal ^ m);
+      phi &= ~((uint32_t)(bal >> 32) ^ m);
     }
-    const uint32_t lower = __popcll(peers & lt_mask);
+    // peers in lanes below this one, and in the whole wave
+    const uint32_t lower = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
     const uint32_t old = s.cnt[w][d];
     __builtin_amdgcn_wave_barrier();
-    if (lower == 0) s.cnt[w][d] = old + (uint32_t)__popcll(peers);
+    if (lower == 0) s.cnt[w][d] = old + (uint32_t)(__popc(plo) + __popc(phi));
     __builtin_amdgcn_wave_barrier();
     rank[r] = old + lower;
   }
@@ -342,20 +346,38 @@ __device__ __forceinline__ void expand_slots(const ExpandLds& s, const int4* s_r
   }
   int j = lo;
   int4 rc = j < kRectCache ? s_rect[j] : rect[r0 + j];
+  // Slot `local` of a Gaussian is tile (row, col) = (local % ny, local / ny)
+  // of its rectangle.  One division for the first slot; later slots of the
+  // same Gaussian step the row.  A Gaussian the loop moves to starts at
+  // slot p exactly (offsets are non-decreasing and the previous slot lay
+  // before off[j]), so it restarts at row = col = 0; with inconsistent
+  // counts the tiles are wrong but the same in both expansion kernels.
+  int ny = max(rc.z, 1);
+  const int local0 = (int)(p - s.off[j]);
+  int col = local0 / ny;
+  int row = local0 - col * ny;
 #pragma unroll
   for (int k = 0; k < kExpandItems; ++k, ++p) {
     tile[k] = 0xffffffffu;
     val[k] = 0;
     if (p < P) {
+      bool moved = false;
       while (j + 1 < n && (int64_t)s.off[j + 1] <= p) {
         ++j;
-        rc = j < kRectCache ? s_rect[j] : rect[r0 + j];
+        moved = true;
       }
-      const int local = (int)(p - s.off[j]);
-      const int ny = max(rc.z, 1);
-      const int col = local / ny;
-      tile[k] = (uint32_t)(rc.x + (local - col * ny) * rc.y + col);
+      if (moved) {
+        rc = j < kRectCache ? s_rect[j] : rect[r0 + j];
+        ny = max(rc.z, 1);
+        row = (int)(p - s.off[j]);
+        col = 0;
+      }
+      tile[k] = (uint32_t)(rc.x + row * rc.y + col);
       val[k] = rc.w;
+      if (++row == ny) {
+        row = 0;
+        ++col;
+      }
     }
   }
 }
