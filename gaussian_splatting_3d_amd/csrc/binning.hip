@@ -50,14 +50,16 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const K* __restric
   for (int i = t; i < 4 * BINS; i += kRadixThreads) (&s_cnt[0][0])[i] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blk * kRadixTile;
-#pragma unroll 4
+  // all loads in flight before the first counter update
+  K k[kRadixItems];
+#pragma unroll
   for (int i = 0; i < kRadixItems; ++i) {
     const int64_t p = base + t + (int64_t)i * kRadixThreads;
-    if (p < P) {
-      const uint32_t d = (uint32_t)(keys[p] >> shift) & (BINS - 1);
-      atomicAdd(&s_cnt[w][d], 1u);
-    }
+    k[i] = p < P ? keys[p] : (K)0;
   }
+#pragma unroll
+  for (int i = 0; i < kRadixItems; ++i)
+    if (base + t + (int64_t)i * kRadixThreads < P) atomicAdd(&s_cnt[w][(uint32_t)(k[i] >> shift) & (BINS - 1)], 1u);
   __syncthreads();
   if (t < BINS) hist[(int64_t)t * nblk + blk] = s_cnt[0][t] + s_cnt[1][t] + s_cnt[2][t] + s_cnt[3][t];
 }
