@@ -1,7 +1,12 @@
-// Device-wide exclusive prefix sums (reduce, scan-of-partials, down-sweep).
-// 256 threads x 16 items = 4096 elements per block.  Element type is int32
-// or Int2 (two independent int32 sums scanned together).
+// Device-wide exclusive prefix sums in two launches: per-block sums, then a
+// down-sweep in which every block first adds up the sums of the blocks
+// before it (at most a few thousand values, read from L2) and scans its own
+// elements.  256 threads x 16 items = 4096 elements per block.  Element type
+// is int32 or Int2 (two independent int32 sums scanned together).
 #pragma once
+
+#include <algorithm>
+#include <cstdint>
 
 #include "gsmi_device.h"
 
@@ -70,10 +75,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const T* __restric
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// Single block: exclusive scan of the partials in place, total to *total.
+// Single block: exclusive scan of the partials in place (for launches of
+// more than kScanSumBlocks blocks, where per-block summing would be slow).
 template <typename T>
-__global__ __launch_bounds__(kScanThreads) void k_scan_partials(T* __restrict__ partial, int64_t nb,
-                                                                T* __restrict__ total) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(T* __restrict__ partial, int64_t nb) {
   __shared__ T s_wave[kScanThreads / 64];
   T carry = zero_of<T>();
   for (int64_t c = 0; c < nb; c += kScanThreads) {
@@ -84,28 +89,64 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(T* __restrict__ 
     if (p < nb) partial[p] = carry + ex;
     carry = carry + tot;
   }
-  if (threadIdx.x == 0 && total) *total = carry;
 }
 
-// Each thread owns kScanItems consecutive elements of the block's tile.
+constexpr int64_t kScanSumBlocks = 4096;
+
+// Each thread owns kScanItems consecutive elements of the block's tile, moved
+// as 16-byte vectors when the tile is whole and both arrays are 16-byte
+// aligned.  Block b's offset is the sum of partial[0..b-1] (partial[b] when
+// the partials are already scanned); the last block writes the grand total.
 template <typename T>
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict__ in, int64_t n,
-                                                            const T* __restrict__ partial, T* __restrict__ out) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* in, int64_t n, const T* __restrict__ partial,
+                                                            bool scanned, T* out, T* __restrict__ total) {
+  static_assert(sizeof(T) % 4 == 0 && (kScanItems * sizeof(T)) % 16 == 0, "vector width");
+  constexpr int kVec = kScanItems * (int)sizeof(T) / 16;
   __shared__ T s_wave[kScanThreads / 64];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  const int64_t b = blockIdx.x;
+  T offset;
+  if (scanned) {
+    offset = partial[b];
+  } else {
+    T pre = zero_of<T>();
+    for (int64_t i = threadIdx.x; i < b; i += kScanThreads) pre = pre + partial[i];
+    block_exclusive_scan<T>(pre, s_wave, &offset);
+  }
+
+  const int64_t base = b * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  const bool vec =
+      (b + 1) * kScanTile <= n && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
   T v[kScanItems];
+  if (vec) {
+    uint4 raw[kVec];
+#pragma unroll
+    for (int k = 0; k < kVec; ++k) raw[k] = reinterpret_cast<const uint4*>(in + base)[k];
+    __builtin_memcpy(v, raw, sizeof(raw));
+  } else {
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) v[i] = (base + i < n) ? in[base + i] : zero_of<T>();
+  }
   T s = zero_of<T>();
 #pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    v[i] = (base + i < n) ? in[base + i] : zero_of<T>();
-    s = s + v[i];
-  }
+  for (int i = 0; i < kScanItems; ++i) s = s + v[i];
   T tot;
-  T run = block_exclusive_scan<T>(s, s_wave, &tot) + partial[blockIdx.x];
+  T run = block_exclusive_scan<T>(s, s_wave, &tot) + offset;
+  if (total && b == (int64_t)gridDim.x - 1 && threadIdx.x == 0) *total = offset + tot;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    if (base + i < n) out[base + i] = run;
-    run = run + v[i];
+    const T x = v[i];
+    v[i] = run;
+    run = run + x;
+  }
+  if (vec) {
+    uint4 raw[kVec];
+    __builtin_memcpy(raw, v, sizeof(raw));
+#pragma unroll
+    for (int k = 0; k < kVec; ++k) reinterpret_cast<uint4*>(out + base)[k] = raw[k];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+      if (base + i < n) out[base + i] = v[i];
   }
 }
 
@@ -124,12 +165,15 @@ inline int exclusive_scan(const T* in, T* out, int64_t n, T* total, void* ws, hi
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n, partial);
     GSMI_CHECK_LAUNCH();
   }
-  hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kScanThreads), 0, st, partial, nb, total);
-  GSMI_CHECK_LAUNCH();
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scan_down<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n, partial, out);
+  const bool scanned = nb > kScanSumBlocks;
+  if (scanned) {
+    hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kScanThreads), 0, st, partial, nb);
     GSMI_CHECK_LAUNCH();
   }
+  // n == 0: one block with no elements writes the zero total
+  hipLaunchKernelGGL(k_scan_down<T>, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(kScanThreads), 0, st, in, n,
+                     partial, scanned, out, total);
+  GSMI_CHECK_LAUNCH();
   return GSMI_OK;
 }
 
