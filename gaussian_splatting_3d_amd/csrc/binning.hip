@@ -222,13 +222,19 @@ __global__ void k_tile_ranges(const K* __restrict__ keys, int64_t P, int32_t* __
   if (i == P - 1 || (uint32_t)(keys[i + 1] >> SHIFT) != tile) end[tile] = (int32_t)(i + 1);
 }
 
-// Depth keys and identity values for the V visible Gaussians.
-__global__ void k_depth_keys(const float* __restrict__ depth, int64_t V, uint32_t* __restrict__ dkey,
-                             int32_t* __restrict__ dval) {
+// Depth keys and identity values for the V visible Gaussians, and their
+// tile bounds packed as one 16-byte record {tl.x, tl.y, br.x, br.y}, so the
+// gather in depth order below touches one cache line per Gaussian.
+__global__ void k_depth_keys(const float* __restrict__ depth, const int32_t* __restrict__ tl,
+                             const int32_t* __restrict__ br, int64_t V, uint32_t* __restrict__ dkey,
+                             int32_t* __restrict__ dval, int4* __restrict__ tlbr) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
   dkey[v] = __float_as_uint(depth[v]);
   dval[v] = (int32_t)v;
+  const int2 a = reinterpret_cast<const int2*>(tl)[v];
+  const int2 b = reinterpret_cast<const int2*>(br)[v];
+  tlbr[v] = make_int4(a.x, a.y, b.x, b.y);
 }
 
 // Pair count and tile rectangle of the Gaussian at depth rank r
@@ -237,9 +243,8 @@ __global__ void k_depth_keys(const float* __restrict__ depth, int64_t V, uint32_
 // (x outer, y inner).  Multi-view: view[r] selects the record's camera and
 // its tile_base (first tile id of the view in the batch numbering).
 __global__ void k_sorted_rects(const int32_t* __restrict__ dval, const uint32_t* __restrict__ view,
-                               const gsmi_view* __restrict__ views, int n_tiles_w, const int32_t* __restrict__ tl,
-                               const int32_t* __restrict__ br, int64_t V, int32_t* __restrict__ cnt,
-                               int4* __restrict__ rect) {
+                               const gsmi_view* __restrict__ views, int n_tiles_w, const int4* __restrict__ tlbr,
+                               int64_t V, int32_t* __restrict__ cnt, int4* __restrict__ rect) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= V) return;
   const int v = dval[r];
@@ -249,8 +254,8 @@ __global__ void k_sorted_rects(const int32_t* __restrict__ dval, const uint32_t*
     base = cam.tile_base;
     ntw = cam.n_tiles_w;
   }
-  const int2 a = reinterpret_cast<const int2*>(tl)[v];
-  const int2 b = reinterpret_cast<const int2*>(br)[v];
+  const int4 q = tlbr[v];
+  const int2 a = make_int2(q.x, q.y), b = make_int2(q.z, q.w);
   cnt[r] = (b.x - a.x + 1) * (b.y - a.y + 1);
   rect[r] = make_int4(base + a.y * ntw + a.x, ntw, b.y - a.y + 1, v);
 }
@@ -553,6 +558,7 @@ struct BinWs {
   int32_t* dval;   // [V] indices in depth order
   int32_t* cnt;    // [V] pair counts in depth order -> exclusive offsets
   int4* rect;      // [V] tile rectangles in depth order
+  int4* tlbr;      // [V] {tl, br} in index order
   uint32_t* view;  // [V] view of each entry in depth order (multi-view)
   int32_t* total;
   uint32_t *ka, *kb;  // [P] tile ids, ping-pong
@@ -569,7 +575,7 @@ static int64_t expand_blocks(int64_t P) { return (P + kExpandTile - 1) / kExpand
 
 static size_t bin_ws_bytes(int64_t V, int64_t P) {
   const int64_t nh = pair_blocks(P) * 256;
-  return align_up(V * 4) * 4 + align_up(V * 16) + 256 + align_up(P * 4) * 4 + align_up(nh * 4) +
+  return align_up(V * 4) * 4 + align_up(V * 16) * 2 + 256 + align_up(P * 4) * 4 + align_up(nh * 4) +
          align_up((expand_blocks(P) + 1) * 4) +
          align_up(scan_workspace_bytes(std::max(V, nh), 4)) + radix_ws_bytes<uint32_t, 8>(V);
 }
@@ -586,6 +592,7 @@ static BinWs carve_bin_ws(void* ws, int64_t V, int64_t P) {
   b.dval = reinterpret_cast<int32_t*>(take(V * 4));
   b.cnt = reinterpret_cast<int32_t*>(take(V * 4));
   b.rect = reinterpret_cast<int4*>(take(V * 16));
+  b.tlbr = reinterpret_cast<int4*>(take(V * 16));
   b.view = reinterpret_cast<uint32_t*>(take(V * 4));
   b.total = reinterpret_cast<int32_t*>(take(256));
   b.ka = reinterpret_cast<uint32_t*>(take(P * 4));
@@ -724,7 +731,7 @@ static int bin_impl(int64_t V, int64_t P, const int32_t* tl, const int32_t* br, 
   int rc;
   if (V > 0) {
     const unsigned g = (unsigned)((V + 255) / 256);
-    hipLaunchKernelGGL(k_depth_keys, dim3(g), dim3(256), 0, st, depth, V, w.dkey, w.dval);
+    hipLaunchKernelGGL(k_depth_keys, dim3(g), dim3(256), 0, st, depth, tl, br, V, w.dkey, w.dval, w.tlbr);
     GSMI_CHECK_LAUNCH();
     rc = radix_sort<uint32_t, 8>(V, w.dkey, w.dval, 32, w.sort_ws, st);
     if (rc) return rc;
@@ -737,7 +744,7 @@ static int bin_impl(int64_t V, int64_t P, const int32_t* tl, const int32_t* br, 
       if (rc) return rc;
       view = w.view;
     }
-    hipLaunchKernelGGL(k_sorted_rects, dim3(g), dim3(256), 0, st, w.dval, view, views, n_tiles_w, tl, br, V, w.cnt,
+    hipLaunchKernelGGL(k_sorted_rects, dim3(g), dim3(256), 0, st, w.dval, view, views, n_tiles_w, w.tlbr, V, w.cnt,
                        w.rect);
     GSMI_CHECK_LAUNCH();
   }

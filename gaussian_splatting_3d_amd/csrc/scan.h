@@ -60,15 +60,30 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* s_wave, T* total) {
   return pre + excl;
 }
 
+// Per-block sums; whole, 16-byte aligned tiles are read as one 16-byte
+// vector per thread and step.
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const T* __restrict__ in, int64_t n,
                                                               T* __restrict__ partial) {
+  constexpr int kPer = 16 / (int)sizeof(T);  // elements per vector
   __shared__ T s_wave[kScanThreads / 64];
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
   T s = zero_of<T>();
-  for (int i = threadIdx.x; i < kScanTile; i += kScanThreads) {
-    const int64_t p = base + i;
-    if (p < n) s = s + in[p];
+  if (base + kScanTile <= n && (reinterpret_cast<uintptr_t>(in) & 15) == 0) {
+    const uint4* in4 = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+    for (int k = 0; k < kScanTile / kPer / kScanThreads; ++k) {
+      const uint4 raw = in4[threadIdx.x + k * kScanThreads];
+      T v[kPer];
+      __builtin_memcpy(v, &raw, sizeof(raw));
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) s = s + v[i];
+    }
+  } else {
+    for (int i = threadIdx.x; i < kScanTile; i += kScanThreads) {
+      const int64_t p = base + i;
+      if (p < n) s = s + in[p];
+    }
   }
   T tot;
   block_exclusive_scan<T>(s, s_wave, &tot);
