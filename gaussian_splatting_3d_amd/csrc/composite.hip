@@ -129,6 +129,56 @@ __device__ __forceinline__ float reject_radius(float a) {
   return r + 1e-3f * (1.0f + fabsf(r));
 }
 
+// Whether some pixel of the rectangle [x0,x1] x [y0,y1] (one wave's pixel
+// positions) can pass the rejection test rq <= rlim of a staged Gaussian.
+// With q = A dx^2 + B dx dy + C dy^2 (A = c3, B = -(c1 + c2), C = c0), the
+// minimum of q over the rectangle is 0 when it holds the centre, else the
+// least of the four edge minima (each edge's minimiser clamped to the
+// edge).  The answer is false only when that minimum exceeds rlim by 2% +
+// 1e-3 and the form is positive definite with condition number below 2000
+// and off-diagonals bounded by sqrt(A C), so the rounding of a pixel's q
+// (relative error below 2e-3 then) cannot bring it under rlim.  Non-finite
+// positions or NaN covariances keep it true.  A wave that skips such an
+// entry skips only pixels that the reference skips too.
+__device__ __forceinline__ bool may_reach(const GeoA& ga, const GeoB& gb, float rlim, float x0, float x1, float y0,
+                                          float y1) {
+  const float A = gb.c3, C = ga.c0, B = -(ga.c1 + gb.c2);
+  const float dq = A * C - 0.25f * (B * B);
+  const float tr = A + C;
+  const float off = fabsf(ga.c1) + fabsf(gb.c2);
+  const float dx0 = x0 - ga.mx, dx1 = x1 - ga.mx, dy0 = y0 - ga.my, dy1 = y1 - ga.my;
+  const float kx = -0.5f * B / C, ky = -0.5f * B / A;  // minimisers: y = kx x on x-edges, x = ky y on y-edges
+  const float ya = fminf(fmaxf(kx * dx0, dy0), dy1), yb = fminf(fmaxf(kx * dx1, dy0), dy1);
+  const float xa = fminf(fmaxf(ky * dy0, dx0), dx1), xb = fminf(fmaxf(ky * dy1, dx0), dx1);
+  const float fa = (A * dx0) * dx0 + (B * dx0) * ya + (C * ya) * ya;
+  const float fb = (A * dx1) * dx1 + (B * dx1) * yb + (C * yb) * yb;
+  const float fc = (A * xa) * xa + (B * xa) * dy0 + (C * dy0) * dy0;
+  const float fd = (A * xb) * xb + (B * xb) * dy1 + (C * dy1) * dy1;
+  const bool centre = dx0 <= 0.0f && dx1 >= 0.0f && dy0 <= 0.0f && dy1 >= 0.0f;
+  const float m = centre ? 0.0f : fminf(fminf(fa, fb), fminf(fc, fd));
+  const float big = 0x1p100f;
+  const bool well = gb.det > 0.0f && A > 0.0f && C > 0.0f && dq > 0.0f && tr * tr < 2000.0f * dq &&
+                    off * off <= 4.0f * (A * C) && fabsf(dx0) < big && fabsf(dx1) < big && fabsf(dy0) < big &&
+                    fabsf(dy1) < big;
+  const float thr = rlim + 0.02f * fabsf(rlim) + 1e-3f;
+  return !(well && m * gb.inv_det > thr);
+}
+
+// This wave's pixel rectangle: positions are monotone in the pixel index, so
+// lanes 0, 15 (first row) and 48 (last row) hold the extremes.  The values
+// are wave-uniform and kept in scalar registers.
+__device__ __forceinline__ float uniform_f(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ float4 wave_pixel_rect(float px, float py) {
+  const float xa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px), 0));
+  const float xb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px), 15));
+  const float ya = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), 0));
+  const float yb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), 48));
+  return make_float4(uniform_f(fminf(xa, xb)), uniform_f(fmaxf(xa, xb)), uniform_f(fminf(ya, yb)),
+                     uniform_f(fmaxf(ya, yb)));
+}
+
 // Exact Gaussian value from the shared quadratic form (reference order).
 __device__ __forceinline__ float gauss_exact(float q, float det) {
   float radial = q / det;
@@ -213,7 +263,7 @@ __device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int bas
 // Forward, SH colour.  vol_render_sh.h:171-248 (+ vol_render_bg.h:12-100).
 // ---------------------------------------------------------------------------
 template <int C, bool BG, bool EXACT>
-__global__ __launch_bounds__(256) void k_fwd_sh(RasterArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_fwd_sh(RasterArgs a) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
   __shared__ GeoA s_ga[kBatch];
@@ -265,12 +315,20 @@ __global__ __launch_bounds__(256) void k_fwd_sh(RasterArgs a) {
     done = true;
     ndone = 0;
   }
+  static_assert(kBatch <= 64, "one batch entry per lane in the relevance mask");
+  const float4 wr = wave_pixel_rect(px, py);
+  const int lane = threadIdx.x & 63;
   for (int base = 0; base < n; base += kBatch) {
     const int cnt = min(kBatch, n - base);
     __syncthreads();
     stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
     __syncthreads();
-    for (int j = 0; j < cnt; ++j) {
+    // entries of the batch that can reach a pixel of this wave, in order
+    const int le = lane < cnt ? lane : 0;
+    uint64_t rel = __ballot(lane < cnt && may_reach(s_ga[le], s_gb[le], s_gc[le].y, wr.x, wr.y, wr.z, wr.w));
+    while (rel) {
+      const int j = __builtin_ctzll(rel);
+      rel &= rel - 1;
       if (__all(done)) break;
       if (done) continue;
       const GeoA ga = s_ga[j];
@@ -731,6 +789,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   pixel_direction(cw, px, py, dir);
   float Y[CC];
   sh_basis<C>(dir[0], dir[1], dir[2], Y);
+  static_assert(kMfmaBatch <= 32, "one batch entry per lane in the relevance mask");
+  const float4 wr = wave_pixel_rect(px, py);
 
   // B operand: lane j + 16g holds Y_j at pixels 32t + 8g + e (e = 0..7) of
   // this wave for K-step t, split into hi and lo bf16.  Transposed through
@@ -798,7 +858,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     __syncthreads();
     int n_slots = 0;      // filled slots of the current chunk (wave-uniform)
     uint32_t slots = 0;   // batch position of slot m in bits [kSlotBits m, kSlotBits (m + 1))
-    for (int j = cnt - 1; j >= 0; --j) {
+    // entries of the batch that can reach a pixel of this wave, walked from
+    // the last one down
+    const int le = lane < cnt ? lane : 0;
+    uint32_t rel =
+        (uint32_t)__ballot(lane < cnt && may_reach(s_ga[le], s_gb[le], s_gc[le].y, wr.x, wr.y, wr.z, wr.w));
+    while (rel) {
+      const int j = 31 - __builtin_clz(rel);
+      rel &= ~(1u << j);
       const int idx = base + j;
       // Per-lane decisions are selects, not branches: the wave runs the
       // entry when any lane needs it, and lanes that do not contribute keep
@@ -874,7 +941,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
           if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r);
         }
       }
-      if (n_slots == kChunk || (j == 0 && n_slots > 0)) {
+      if (n_slots == kChunk || (rel == 0 && n_slots > 0)) {
         // LDS operations of one wave complete in order; the barrier keeps the
         // compiler from moving the loads above the stores.
         __builtin_amdgcn_wave_barrier();
