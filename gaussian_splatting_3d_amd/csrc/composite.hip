@@ -2,9 +2,11 @@
 //
 // One 256-thread workgroup (4 wave64) per 16x16 tile; wave w owns pixel rows
 // 4w..4w+3.  Gaussians of the tile's sorted list are staged in LDS in
-// batches; every wave walks the batch in lockstep (wave-uniform loop) and
-// leaves it as soon as a ballot shows all 64 of its pixels saturated; the
-// workgroup stops loading batches once no pixel is alive.
+// batches; every wave walks the entries of the batch that can reach one of
+// its pixels (a relevance mask built once per batch, one entry per lane) in
+// lockstep (wave-uniform loop) and leaves the batch as soon as a ballot
+// shows all 64 of its pixels saturated; the workgroup stops loading batches
+// once no pixel is alive.
 //
 // Decision exactness.  The skip test a*G < 1/255 and the transmittance
 // break use the reference's operation sequence (kernels.h:172-193,
