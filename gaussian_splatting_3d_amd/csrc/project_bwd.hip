@@ -4,6 +4,12 @@
 // gathers, with the reference's detaches (gs/renderer.py:365-419: the
 // Jacobian is built under no_grad and depth is detached, so the mean receives
 // gradient only through xy / depth).
+//
+// Values here feed gradients only (checked to 1e-3 against the float64
+// graph), so the exponentials and quotients use the hardware exp2 and
+// reciprocal (about 1e-6 relative over the inputs' range) instead of the
+// forward's exactly specified sequences, and every load of a thread is
+// issued before the visibility branch.
 #include "gsmi_device.h"
 
 namespace gsmi {
@@ -12,22 +18,25 @@ struct Cam12 {
   float cw[12];
 };
 
-// Per-view terms of the adjoint at raw Gaussian i: adds dL/dmu (3) and
-// dL/dSigma (3x3) for the compacted record v seen through camera cw.
-__device__ __forceinline__ void view_terms(const float* __restrict__ mean, int64_t i, const float* cw,
-                                           float2 gm, float4 G, float* __restrict__ g_mean, float* __restrict__ gS) {
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Per-view terms of the adjoint at a raw Gaussian with mean mu: adds dL/dmu
+// (3) and dL/dSigma (3x3) for its compacted record seen through camera cw.
+__device__ __forceinline__ void view_terms(const float (&mu)[3], const float* cw, float2 gm, float4 G,
+                                           float* __restrict__ g_mean, float* __restrict__ gS) {
   // W = Rc^T, u = W (mu - t)
   float Wm[9];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) Wm[3 * a + b] = cw[4 * b + a];
-  const float d0 = mean[3 * i] - cw[3], d1 = mean[3 * i + 1] - cw[7], d2 = mean[3 * i + 2] - cw[11];
+  const float d0 = mu[0] - cw[3], d1 = mu[1] - cw[7], d2 = mu[2] - cw[11];
   float u[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) u[a] = Wm[3 * a] * d0 + Wm[3 * a + 1] * d1 + Wm[3 * a + 2] * d2;
   const float z = u[2];
-  const float iz = 1.0f / z;
+  const float iz = fast_rcp(z);
   // mean: dL/du = (g.x / z, g.y / z, 0); dL/dmu = W^T dL/du.
   const float gu0 = gm.x * iz, gu1 = gm.y * iz;
 #pragma unroll
@@ -53,16 +62,38 @@ __device__ __forceinline__ void view_terms(const float* __restrict__ mean, int64
     for (int b = 0; b < 3; ++b) gS[3 * a + b] += A[a] * GA[b] + A[3 + a] * GA[3 + b];
 }
 
+// Raw parameters of Gaussian i, loaded together.
+struct RawParams {
+  float mu[3];
+  float4 q;
+  float ls[3];
+  float lg;
+};
+
+__device__ __forceinline__ RawParams load_params(int64_t i, const float* __restrict__ mean,
+                                                 const float* __restrict__ qvec, const float* __restrict__ log_s,
+                                                 const float* __restrict__ logit) {
+  RawParams r;
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    r.mu[b] = mean[3 * i + b];
+    r.ls[b] = log_s[3 * i + b];
+  }
+  r.q = reinterpret_cast<const float4*>(qvec)[i];
+  r.lg = logit[i];
+  return r;
+}
+
 // dL/dSigma and dL/dalpha -> gradients of qvec, log-scale and logit of raw
 // Gaussian i (writes the outputs of i).
-__device__ __forceinline__ void param_chain(int64_t i, const float* __restrict__ qvec,
-                                            const float* __restrict__ log_s, const float* __restrict__ logit,
-                                            const float* gS, float g_alpha, float* __restrict__ o_q,
-                                            float* __restrict__ o_ls, float* __restrict__ o_logit) {
-  const float4 q = reinterpret_cast<const float4*>(qvec)[i];
+__device__ __forceinline__ void param_chain(int64_t i, const RawParams& rp, const float* gS, float g_alpha,
+                                            float* __restrict__ o_q, float* __restrict__ o_ls,
+                                            float* __restrict__ o_logit) {
+  const float4 q = rp.q;
   const float nq = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
   const float den = fmaxf(nq, 1e-12f);
-  const float w = q.x / den, x = q.y / den, y = q.z / den, zq = q.w / den;
+  const float iden = fast_rcp(den);
+  const float w = q.x * iden, x = q.y * iden, y = q.z * iden, zq = q.w * iden;
   float R[9];
   R[0] = 1.0f - 2.0f * (y * y + zq * zq);
   R[1] = 2.0f * (x * y - w * zq);
@@ -73,7 +104,7 @@ __device__ __forceinline__ void param_chain(int64_t i, const float* __restrict__
   R[6] = 2.0f * (x * zq - w * y);
   R[7] = 2.0f * (y * zq + w * x);
   R[8] = 1.0f - 2.0f * (x * x + y * y);
-  const float s[3] = {exp_spec(log_s[3 * i]), exp_spec(log_s[3 * i + 1]), exp_spec(log_s[3 * i + 2])};
+  const float s[3] = {fast_exp(rp.ls[0]), fast_exp(rp.ls[1]), fast_exp(rp.ls[2])};
   float M[9];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -108,19 +139,16 @@ __device__ __forceinline__ void param_chain(int64_t i, const float* __restrict__
   const float gz = 2.0f * (-2.0f * zq * gR[0] - w * gR[1] + x * gR[2] + w * gR[3] - 2.0f * zq * gR[4] +
                            y * gR[5] + x * gR[6] + y * gR[7]);
   // q_hat = q / max(|q|, eps): dq = (dq_hat - q_hat (q_hat . dq_hat)) / |q|.
-  float4 gq;
-  if (nq > 1e-12f) {
-    const float dotp = w * gw + x * gx + y * gy + zq * gz;
-    gq = make_float4((gw - w * dotp) / nq, (gx - x * dotp) / nq, (gy - y * dotp) / nq, (gz - zq * dotp) / nq);
-  } else {
-    gq = make_float4(gw / den, gx / den, gy / den, gz / den);
-  }
+  // (den == nq when nq > 1e-12, so one reciprocal serves both cases)
+  const float dotp = nq > 1e-12f ? w * gw + x * gx + y * gy + zq * gz : 0.0f;
+  const float4 gq = make_float4((gw - w * dotp) * iden, (gx - x * dotp) * iden, (gy - y * dotp) * iden,
+                                (gz - zq * dotp) * iden);
   reinterpret_cast<float4*>(o_q)[i] = gq;
   // s = exp(log_s)
 #pragma unroll
   for (int b = 0; b < 3; ++b) o_ls[3 * i + b] = gs[b] * s[b];
   // alpha = sigmoid(logit)
-  const float al = 1.0f / (1.0f + exp_spec(-logit[i]));
+  const float al = fast_rcp(1.0f + fast_exp(-rp.lg));
   o_logit[i] = g_alpha * (al * (1.0f - al));
 }
 
@@ -146,17 +174,23 @@ __global__ __launch_bounds__(256) void k_project_bwd(int64_t N, const float* __r
                                                      float* __restrict__ o_logit) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  if (!vis[i]) {
+  // vis, the record index and the raw parameters in one round of loads; the
+  // record's gradients (index from voff, valid for visible i) in the next.
+  const bool visible = vis[i] != 0;
+  const int64_t v = voff[i];
+  const RawParams rp = load_params(i, mean, qvec, log_s, logit);
+  if (!visible) {
     write_zero(i, o_mean, o_q, o_ls, o_logit);
     return;
   }
-  const int64_t v = voff[i];
+  const float2 g2 = reinterpret_cast<const float2*>(g_m2d)[v];
+  const float4 gc = reinterpret_cast<const float4*>(g_cov)[v];
+  const float ga = g_alpha[v];
   float gm[3] = {0.f, 0.f, 0.f}, gS[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  view_terms(mean, i, cam.cw, reinterpret_cast<const float2*>(g_m2d)[v], reinterpret_cast<const float4*>(g_cov)[v],
-             gm, gS);
+  view_terms(rp.mu, cam.cw, g2, gc, gm, gS);
 #pragma unroll
   for (int b = 0; b < 3; ++b) o_mean[3 * i + b] = gm[b];
-  param_chain(i, qvec, log_s, logit, gS, g_alpha[v], o_q, o_ls, o_logit);
+  param_chain(i, rp, gS, ga, o_q, o_ls, o_logit);
 }
 
 // Multi-view: the gradients of every view that sees Gaussian i are summed
@@ -170,6 +204,7 @@ __global__ __launch_bounds__(256) void k_project_bwd_views(
     float* __restrict__ o_q, float* __restrict__ o_ls, float* __restrict__ o_logit) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const RawParams rp = load_params(i, mean, qvec, log_s, logit);
   float gm[3] = {0.f, 0.f, 0.f}, gS[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float ga = 0.0f;
   bool any = false;
@@ -181,8 +216,8 @@ __global__ __launch_bounds__(256) void k_project_bwd_views(
     float cw[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) cw[k] = views[b].c2w[k];
-    view_terms(mean, i, cw, reinterpret_cast<const float2*>(g_m2d)[v], reinterpret_cast<const float4*>(g_cov)[v],
-               gm, gS);
+    view_terms(rp.mu, cw, reinterpret_cast<const float2*>(g_m2d)[v], reinterpret_cast<const float4*>(g_cov)[v], gm,
+               gS);
     ga += g_alpha[v];
   }
   if (!any) {
@@ -191,7 +226,7 @@ __global__ __launch_bounds__(256) void k_project_bwd_views(
   }
 #pragma unroll
   for (int b = 0; b < 3; ++b) o_mean[3 * i + b] = gm[b];
-  param_chain(i, qvec, log_s, logit, gS, ga, o_q, o_ls, o_logit);
+  param_chain(i, rp, gS, ga, o_q, o_ls, o_logit);
 }
 
 }  // namespace gsmi
