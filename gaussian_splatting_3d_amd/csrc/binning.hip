@@ -14,8 +14,9 @@
 //      7 tile-id bits per 2048-slot block and once to rank the block's pairs
 //      on those bits and scatter them as the first radix pass;
 //   4. the remaining tile-id bits in passes of at most 8; the last pass also
-//      records each tile's first and last position (atomic min / max of the
-//      block-local run ends), so no separate range pass reads the keys.
+//      writes each tile's first and last position from the digit offsets at
+//      the boundaries of the low-bit buckets (block_ranges), so no separate
+//      range pass reads the keys.
 // Within a tile the entries are then ordered by (depth bits, index), i.e.
 // exactly the order of a stable sort of the 64-bit keys emitted in index
 // order.  The 64-bit keys are materialised only when a caller asks for them.
@@ -70,33 +71,131 @@ struct ScatterLds {
   uint32_t base[4][1 << RB];
   uint32_t dstart[1 << RB];
   uint32_t goff[1 << RB];
+  uint32_t pref[1 << RB];  // ranges: digit counts before a bucket boundary
   uint32_t wsum[4];
+  K wlast[4];              // ranges: last key of each wave's items
+  K kfirst, klast;         // ranges: keys at block positions 0 and nvalid - 1
+  int nextb;
+  uint32_t lnew, lold;
   K keys[TILE];
   alignas(16) int32_t vals[TILE];
 };
+
+constexpr uint32_t kNoBucket = 0xffffffffu;
+
+// Tile ranges of the last pass without atomics.  Before the last pass (shift
+// = its first bit) the pairs are sorted by the low tile-id bits, so a block's
+// "buckets" (runs of equal low bits) start and end at known positions, and
+// for digit h the number of pairs with tile < (h, l) is
+//   X(h) = offs[h][block] + (digit-h pairs of the block before the position
+//          where bucket l starts),
+// since earlier blocks and earlier positions hold only lower buckets.  So at
+// every bucket boundary (low bits change between positions pb - 1 and pb;
+// also the array's first position and its end) X(h) is the start of tile
+// (h, l_new) and the end of tile (h, l_old), for every digit h.  Only blocks
+// holding a boundary (~1% at 1080p) do this; each tile's start and end are
+// written once with plain stores.  Tiles whose two values are equal (empty)
+// or never written are set to -1 afterwards (k_fix_ranges).
+template <typename K, int RB, int ITEMS>
+__device__ __forceinline__ void block_ranges(ScatterLds<K, RB, kRadixThreads * ITEMS>& s, const K (&key)[ITEMS],
+                                             int nvalid, bool is_last, int shift, K prev_key, bool has_prev,
+                                             int32_t* __restrict__ start, int32_t* __restrict__ end,
+                                             uint32_t n_tiles) {
+  static_assert(sizeof(K) == 4, "ranges are built on 32-bit tile ids");
+  constexpr int BINS = 1 << RB;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const uint32_t lowmask = shift >= 32 ? ~0u : (1u << shift) - 1u;
+  const uint32_t lf = (uint32_t)s.kfirst & lowmask, ll = (uint32_t)s.klast & lowmask;
+  const uint32_t lprev = (uint32_t)prev_key & lowmask;
+  const bool starts = !has_prev || lprev != lf;
+  if (!(starts || lf != ll || is_last)) return;  // block-uniform
+  // boundary flags: the low bits differ from the previous position's
+  bool isb[ITEMS];
+#pragma unroll
+  for (int r = 0; r < ITEMS; ++r) {
+    const int p = w * (64 * ITEMS) + r * 64 + lane;
+    const K up = (K)__shfl_up((uint32_t)key[r], 1);
+    K wrap;
+    if (r > 0) wrap = (K)__shfl((uint32_t)key[r - 1], 63);
+    else wrap = w > 0 ? s.wlast[w - 1] : prev_key;
+    const K pk = lane > 0 ? up : wrap;
+    const bool has_pred = p > 0 || has_prev;
+    isb[r] = p < nvalid && (!has_pred || (((uint32_t)pk & lowmask) != ((uint32_t)key[r] & lowmask)));
+  }
+  int cur = -1;
+  for (;;) {
+    if (t == 0) {
+      s.nextb = 0x7fffffff;
+      s.lnew = kNoBucket;
+      s.lold = kNoBucket;
+    }
+    for (int i = t; i < BINS; i += kRadixThreads) s.pref[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const int p = w * (64 * ITEMS) + r * 64 + lane;
+      if (isb[r] && p > cur) atomicMin(&s.nextb, p);
+    }
+    __syncthreads();
+    int pb = s.nextb;
+    if (pb == 0x7fffffff) {
+      if (!(is_last && cur < nvalid)) break;  // block-uniform
+      pb = nvalid;                            // the end of the array
+    }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const int p = w * (64 * ITEMS) + r * 64 + lane;
+      const uint32_t lo = (uint32_t)key[r] & lowmask;
+      if (p < pb) atomicAdd(&s.pref[(uint32_t)(key[r] >> shift) & (BINS - 1)], 1u);
+      if (p == pb && p < nvalid) s.lnew = lo;  // none at the end of the array
+      if (p == pb - 1) s.lold = lo;
+    }
+    if (pb == 0 && has_prev && t == 0) s.lold = lprev;
+    __syncthreads();
+    const uint32_t lnew = s.lnew, lold = s.lold;
+    for (int h = t; h < BINS; h += kRadixThreads) {
+      const int32_t X = (int32_t)(s.goff[h] + s.pref[h]);
+      const uint32_t hi = (uint32_t)h << shift;
+      if (lnew != kNoBucket && (hi | lnew) < n_tiles) start[hi | lnew] = X;
+      if (lold != kNoBucket && (hi | lold) < n_tiles) end[hi | lold] = X;
+    }
+    cur = pb;
+    __syncthreads();
+  }
+}
 
 // Stable scatter of one block of 256 * ITEMS items.  Item r of lane `lane` in
 // wave w sits at block position w*64*ITEMS + r*64 + lane (key[r], val[r]; keys of
 // invalid positions are all-ones).  Ranks by digit (key >> shift) & (2^RB-1)
 // with wave64 ballot matching, permutes through LDS, writes each digit's run
 // at offs[digit][block] (exclusive-scanned histogram).  kout may be null
-// (keys not needed).  With RANGES the keys are final tile ids in sorted
-// order: every block-local run boundary is folded into start (atomic min) and
-// end (atomic max of position + 1).
+// (keys not needed).  With RANGES this is the last pass over tile ids and
+// block_ranges writes the tile ranges; prev_key is the key at position
+// base - 1 (when has_prev).
 template <typename K, int RB, bool RANGES, int ITEMS>
 __device__ __forceinline__ void block_scatter(ScatterLds<K, RB, kRadixThreads * ITEMS>& s, K (&key)[ITEMS],
                                               int32_t (&val)[ITEMS], int64_t base, int64_t P, int shift,
                                               const uint32_t* __restrict__ offs, int nblk, K* __restrict__ kout,
                                               int32_t* __restrict__ vout, int32_t* __restrict__ start,
-                                              int32_t* __restrict__ end, uint32_t n_tiles, int blk) {
+                                              int32_t* __restrict__ end, uint32_t n_tiles, int blk, K prev_key,
+                                              bool has_prev) {
   constexpr int BINS = 1 << RB;
+  constexpr int TILE = kRadixThreads * ITEMS;
   static_assert(BINS <= kRadixThreads, "one digit per thread in the block prefix");
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int nvalid = (int)min((int64_t)TILE, P - base);
   for (int i = lane; i < BINS; i += 64) s.cnt[w][i] = 0;
   if (t < BINS) s.goff[t] = offs[(int64_t)t * nblk + blk];
+  if constexpr (RANGES) {
+    if (lane == 63) s.wlast[w] = key[ITEMS - 1];
+    if (t == 0) s.kfirst = key[0];
+    const int pl = nvalid - 1;  // the last valid position's owner publishes its key
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      if (w * (64 * ITEMS) + r * 64 + lane == pl) s.klast = key[r];
+  }
   __syncthreads();
 
-  constexpr int TILE = kRadixThreads * ITEMS;
   uint32_t rank[ITEMS];
   // Wave w ranks its 64 * ITEMS items 64 per round in position order, so
   // ranks are stable.  Lanes with the same digit ("peers") are found one
@@ -159,7 +258,8 @@ __device__ __forceinline__ void block_scatter(ScatterLds<K, RB, kRadixThreads * 
     s.vals[lp] = val[r];
   }
   __syncthreads();
-  const int nvalid = (int)min((int64_t)TILE, P - base);
+  if constexpr (RANGES)
+    block_ranges<K, RB, ITEMS>(s, key, nvalid, base + nvalid >= P, shift, prev_key, has_prev, start, end, n_tiles);
 #pragma unroll 4
   for (int i = t; i < TILE; i += kRadixThreads) {
     if (i < nvalid) {
@@ -168,12 +268,6 @@ __device__ __forceinline__ void block_scatter(ScatterLds<K, RB, kRadixThreads * 
       const int64_t gp = (int64_t)s.goff[d] + (i - (int64_t)s.dstart[d]);
       if (kout) kout[gp] = k;
       vout[gp] = s.vals[i];
-      if constexpr (RANGES) {
-        const uint32_t tile = (uint32_t)k;
-        if (tile >= n_tiles) continue;  // only from inconsistent caller bounds
-        if (i == 0 || (uint32_t)s.keys[i - 1] != tile) atomicMin(start + tile, (int32_t)gp);
-        if (i == nvalid - 1 || (uint32_t)s.keys[i + 1] != tile) atomicMax(end + tile, (int32_t)(gp + 1));
-      }
     }
   }
 }
@@ -200,8 +294,9 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const K* __rest
     key[r] = valid ? kin[p] : (K)~(K)0;
     val[r] = valid ? vin[p] : 0;
   }
+  const K prev_key = RANGES && base > 0 ? kin[base - 1] : (K)0;
   block_scatter<K, RB, RANGES, kRadixItems>(s, key, val, base, P, shift, offs, nblk, kout, vout, start, end,
-                                            n_tiles, blk);
+                                            n_tiles, blk, prev_key, base > 0);
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -462,15 +557,21 @@ __global__ __launch_bounds__(kRadixThreads) void k_expand_scatter(const int32_t*
     key[r] = s.keys[q];
     val[r] = s.vals[q];
   }
-  // block_scatter's first barrier orders these reads before its LDS writes
+  // block_scatter's first barrier orders these reads before its LDS writes;
+  // as the only pass (shift 0) every pair is in one bucket, so the key
+  // before the block is irrelevant
   block_scatter<uint32_t, kExpandDigitBits, RANGES, kExpandItems>(s, key, val, (int64_t)blk * kExpandTile, P, 0,
-                                                                  offs, nblk, kout, vout, start, end, n_tiles, blk);
+                                                                  offs, nblk, kout, vout, start, end, n_tiles, blk,
+                                                                  0u, blk > 0);
 }
 
-// start == INT32_MAX (no pair seen by the atomic-min) -> -1.
-__global__ void k_fix_start(int32_t* start, int n) {
+// Tiles with start == end (empty, or never written: both -1) -> -1, -1.
+__global__ void k_fix_ranges(int32_t* start, int32_t* end, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && start[i] == 0x7fffffff) start[i] = -1;
+  if (i < n && start[i] >= end[i]) {
+    start[i] = -1;
+    end[i] = -1;
+  }
 }
 
 // 64-bit keys of the sorted pairs: (tile << 32) | depth bits of the Gaussian.
@@ -759,7 +860,7 @@ static int bin_impl(int64_t V, int64_t P, const int32_t* tl, const int32_t* br, 
     if ((int64_t)h_total != P) return GSMI_ERR_COUNT;
   }
   const int nb = (n_tiles + 255) / 256;
-  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, start, (int64_t)n_tiles, 0x7fffffff);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, start, (int64_t)n_tiles, -1);
   hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, end, (int64_t)n_tiles, -1);
   GSMI_CHECK_LAUNCH();
   uint32_t* tiles = nullptr;
@@ -767,7 +868,7 @@ static int bin_impl(int64_t V, int64_t P, const int32_t* tl, const int32_t* br, 
     rc = pair_sort(w, V, P, n_tiles, gaussian_ids, start, end, sorted_keys != nullptr, &tiles, st);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_fix_start, dim3(nb), dim3(256), 0, st, start, n_tiles);
+  hipLaunchKernelGGL(k_fix_ranges, dim3(nb), dim3(256), 0, st, start, end, n_tiles);
   GSMI_CHECK_LAUNCH();
   if (sorted_keys && P > 0 && tiles) {
     hipLaunchKernelGGL(k_make_keys, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, tiles, gaussian_ids, depth,
