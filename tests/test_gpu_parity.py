@@ -259,6 +259,63 @@ def test_forward_deterministic():
     assert torch.equal(a, b)
 
 
+def extreme_shapes_scene(seed=23, n=3000, W=96, H=80):
+    """Shapes at the edges of the per-wave relevance test: 100:1 needles (past
+    its conditioning bound) and 30:1 needles (within it) at random
+    orientations, sub-pixel dots, Gaussians wider than the image, opacities
+    below 1/255 (negative rejection radius) and at the 0.99 clamp."""
+    g = torch.Generator().manual_seed(seed)
+    kind = torch.arange(n) % 5
+    mean = torch.randn(n, 3, generator=g) * 0.9
+    q = torch.randn(n, 4, generator=g)
+    q = q / q.norm(dim=-1, keepdim=True)
+    log_s = torch.randn(n, 3, generator=g) * 0.3 - 3.0
+    thin = (kind == 0) & (torch.arange(n) % 2 == 0)
+    log_s[thin] = torch.tensor([np.log(0.2), np.log(0.002), np.log(0.002)], dtype=torch.float32)
+    log_s[(kind == 0) & ~thin] = torch.tensor([np.log(0.2), np.log(0.0067), np.log(0.0067)], dtype=torch.float32)
+    log_s[kind == 1] = float(np.log(2e-4))
+    log_s[kind == 2] = float(np.log(1.5))
+    logit = torch.randn(n, generator=g)
+    logit[kind == 3] = -7.0   # alpha ~ 9e-4 < 1/255
+    logit[kind == 4] = 8.0    # alpha clamped at 0.99
+    sh = torch.randn(n, 3, 16, generator=g) * 0.1
+    sh[:, :, 0] = torch.randn(n, 3, generator=g) * 0.6
+    c2w, cam = scenes.orbit_camera(4.0, 0.4, 90.0, 90.0, W, H, near=0.01, far=100.0, height=0.5)
+    return scenes.Scene(mean, q, log_s, logit, sh, 4, [(c2w, cam)], "extreme_shapes")
+
+
+def test_extreme_shapes_exact_and_adjoint(orc):
+    """Exact colour mode stays bit-identical to the oracle (image, final
+    transmittance, entries examined) on shapes where a wave's relevance test
+    is tight or must give up; the adjoint stays within 1e-3."""
+    sc = extreme_shapes_scene()
+    c2w, cam = sc.cameras[0]
+    ref = _oracle_view(orc, sc)
+    assert ref["P"] > 0
+    cfg = scenes.RasterConfig()
+    p = dev_params(sc)
+    out_e, st = rasterize.forward_view(*p, c2w, cam, sc.C, cfg, exact=True)
+    got = out_e.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), ref["out"].view(np.uint32)), np.abs(got - ref["out"]).max()
+    assert np.array_equal(st.T.cpu().numpy().view(np.uint32), ref["T"].view(np.uint32))
+    assert np.array_equal(st.n_done.cpu().numpy(), ref["n_done"])
+    out_f, st_f = rasterize.forward_view(*p, c2w, cam, sc.C, cfg)
+    assert np.abs(out_f.cpu().numpy() - ref["out"]).max() <= IMG_TOL
+    assert np.array_equal(st_f.n_done.cpu().numpy(), ref["n_done"])
+    gout = torch.rand(out_f.shape, generator=torch.Generator().manual_seed(5)).to(DEV) - 0.5
+    _, _, _, _, g_sh, bd = rasterize.backward_view(st_f, *p, gout)
+    rg, mism = orc.render_sh_bwd(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]], ref["start"],
+                                 ref["end"], ref["ids"], ref["out"], gout.cpu().numpy(), ref["topleft"], c2w, 16,
+                                 ref["n_tiles_h"], ref["n_tiles_w"], ref["psx"], ref["psy"], ref["H"], ref["W"],
+                                 sc.C, 1e-4)
+    assert mism == 0
+    assert rel_err(bd["mean2d"].cpu(), rg["mean"]) <= GRAD_RTOL
+    assert rel_err(bd["cov"].cpu(), rg["cov"]) <= GRAD_RTOL
+    assert rel_err(bd["alpha"].cpu(), rg["alpha"]) <= GRAD_RTOL
+    gsh = g_sh.cpu().numpy()[ref["idx"]][..., : sc.C * sc.C]
+    assert rel_err(gsh, rg["sh"]) <= GRAD_RTOL
+
+
 @pytest.mark.parametrize("which,bg", [("c1", False), ("small2", False), ("small3", False), ("small4", False),
                                       ("small4", True), ("small3", True)])
 def test_backward_boundary_gradients(orc, which, bg):
