@@ -769,6 +769,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   __shared__ __align__(16) float s_shacc[4][kMfmaBatch * SHN];  // SH components, per wave
   __shared__ __align__(16) uint16_t s_cc[4][2 * kCcPlane];     // per wave: hi plane, lo plane
   __shared__ int s_tmp[4];
+  __shared__ uint32_t s_touch;          // batch entries with a contribution in some wave
+  __shared__ int s_tlist[kMfmaBatch];   // their positions, ascending
 
   TileCtx tc;
   if (!tile_setup(a, tc)) return;
@@ -850,12 +852,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   const int d_ch = lane >> 4;                          // channel of this lane's MFMA output rows
   const int d_k = lane & 15;                           // basis function of this lane's output column
 
+  // The accumulators are zeroed once here; each batch's flush zeroes what it
+  // reads, and only entries of the touched list are ever written.
+  for (int e = t; e < kMfmaBatch * 6; e += kTilePixels) s_acc[e] = 0.0f;
+  for (int e = t; e < 4 * kMfmaBatch * SHN / 4; e += kTilePixels)
+    reinterpret_cast<float4*>(&s_shacc[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int base = ((nmax - 1) / kMfmaBatch) * kMfmaBatch; nmax > 0 && base >= 0; base -= kMfmaBatch) {
     const int cnt = min(kMfmaBatch, nmax - base);
     __syncthreads();
-    for (int e = t; e < cnt * 6; e += kTilePixels) s_acc[e] = 0.0f;
-    for (int e = t; e < 4 * kMfmaBatch * SHN / 4; e += kTilePixels)
-      reinterpret_cast<float4*>(&s_shacc[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t == 0) s_touch = 0u;
     stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
     __syncthreads();
     int n_slots = 0;      // filled slots of the current chunk (wave-uniform)
@@ -939,6 +944,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
           cc_w[kCcPlane + cc_pos(2 * kChunk + m, lane)] = l;
           slots |= static_cast<uint32_t>(j) << (kSlotBits * m);
           n_slots = m + 1;
+          if (lane == 0) atomicOr(&s_touch, 1u << j);
           const float r = wave_reduce_scatter8_swap(v, lane);
           if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r);
         }
@@ -979,15 +985,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
       }
     }
     __syncthreads();
-    for (int e = t; e < cnt * NCOMP; e += kTilePixels) {
-      const int jj = e / NCOMP;
-      const int c = e - jj * NCOMP;
+    const uint32_t touch = s_touch;
+    if (t < kMfmaBatch && ((touch >> t) & 1u)) s_tlist[__popc(touch & ((1u << t) - 1u))] = t;
+    __syncthreads();
+    // one thread per (touched entry, component): read, zero, emit
+    for (int e = t; e < __popc(touch) * NCOMP; e += kTilePixels) {
+      const int k = e / NCOMP;
+      const int c = e - k * NCOMP;
+      const int jj = s_tlist[k];
       float v;
       if (c < 6) {
         v = s_acc[jj * 6 + c];
+        s_acc[jj * 6 + c] = 0.0f;
       } else {
         const int o = jj * SHN + c - 6;
         v = ((s_shacc[0][o] + s_shacc[1][o]) + s_shacc[2][o]) + s_shacc[3][o];
+        s_shacc[0][o] = s_shacc[1][o] = s_shacc[2][o] = s_shacc[3][o] = 0.0f;
       }
       if (v != 0.0f) emit_grad<SHN>(a, s_id[jj], c, v);
     }
