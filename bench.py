@@ -298,8 +298,10 @@ def _pmc_pass(counter, child_args, timeout_s=90):
     return {name: (float(v), int(n)) for name, v, n in rows}
 
 
-# Non-packed FP32 VALU issue peak of MI355X: 256 CUs x 4 SIMDs x one wave64
-# instruction per 4 cycles at the 2.4 GHz peak engine clock.
+# Nominal non-packed FP32 VALU issue rate of MI355X: 256 CUs x 4 SIMDs x one
+# wave64 instruction per 4 cycles at the 2.4 GHz peak engine clock.  Some
+# VALU instructions (moves, lane reads, compares) take fewer cycles, so a
+# kernel rich in them can measure above it.
 VALU_PEAK_INSTR_S = 256 * 4 * 2.4e9 / 4
 
 
