@@ -51,6 +51,10 @@ SIGNATURES = {
     "gsmi_tile_based_vol_rendering_backward_sh": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P,
                                                          _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32,
                                                          _F, _F, _I32, _I32, _I32, _F, _P, _P]),
+    "gsmi_backward_order_workspace_size": (_SZ, [_I32]),
+    "gsmi_tile_based_vol_rendering_backward_sh_ws": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P,
+                                                            _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
+                                                            _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P, _SZ, _P]),
     "gsmi_tile_based_vol_rendering_start_end": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
                                                        _I32, _F, _F, _I32, _I32, _F, _P, _P, _I32, _P]),
     "gsmi_tile_based_vol_rendering_backward_start_end": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
@@ -65,7 +69,7 @@ SIGNATURES = {
     "gsmi_render_views_sh": (_I32, [_I64, _I64, _I32, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P,
                                     _I32, _F, _P, _P, _P, _P, _I32, _I32, _P]),
     "gsmi_render_views_backward_sh": (_I32, [_I64, _I64, _I32, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P, _P,
-                                             _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _P, _P]),
+                                             _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _P, _P, _SZ, _P]),
     "gsmi_project_backward_views": (_I32, [_I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                            _P]),
     "gsmi_loss_workspace_size": (_SZ, [_I64]),

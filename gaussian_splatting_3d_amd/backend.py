@@ -120,17 +120,21 @@ class _Backend:
                     grad_out, grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, topleft, c2w, tile_size, n_tiles_h,
                     n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb=None, sh_stride_g=None,
                     sh_stride_c=None, sh_index=None):
-        """Reverse-order adjoint from saved per-pixel state (accumulates)."""
+        """Reverse-order adjoint from saved per-pixel state (accumulates).
+        The tiles are launched most expensive first (a small workspace holds
+        the order)."""
         L = _lib.load()
         C = int(C)
         sg = 3 * C * C if sh_stride_g is None else int(sh_stride_g)
         sc = C * C if sh_stride_c is None else int(sh_stride_c)
-        check(L.gsmi_tile_based_vol_rendering_backward_sh(
+        nws = int(L.gsmi_backward_order_workspace_size(int(n_tiles_h) * int(n_tiles_w)))
+        ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=grad_out.device)
+        check(L.gsmi_tile_based_vol_rendering_backward_sh_ws(
             mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(sh_coeffs), sg, sc, ptr(sh_index),
             ptr(alpha), ptr(start), ptr(end), ptr(gaussian_ids), ptr(out), ptr(T_final), ptr(n_done), ptr(pre_bg),
             ptr(grad_out), ptr(grad_mean), ptr(grad_cov), ptr(grad_sh_coeffs), ptr(grad_alpha), ptr(topleft),
             ptr(c2w), int(tile_size), int(n_tiles_h), int(n_tiles_w), float(pixel_size_x), float(pixel_size_y),
-            int(H), int(W), C, float(thresh), ptr(bg_rgb), stream_of(grad_out.device)),
+            int(H), int(W), C, float(thresh), ptr(bg_rgb), ptr(ws), nws, stream_of(grad_out.device)),
             "tile_based_vol_rendering_backward_sh")
 
     # gs/src/render.cu:483-544

@@ -62,6 +62,9 @@ struct RasterArgs {
   const gsmi_view* views;
   int n_views;
   int n_tiles_total;
+  // Optional launch order: block b works on tile tile_order[b] (a permutation
+  // of the tiles); null: the XCD-contiguous mapping.
+  const int32_t* tile_order;
 };
 
 // Pixel of this thread in the block's tile.
@@ -75,8 +78,14 @@ struct TileCtx {
 };
 
 __device__ __forceinline__ bool tile_setup(const RasterArgs& a, TileCtx& c) {
-  const int tile = xcd_tile(blockIdx.x, a.n_tiles_total);
-  if (tile >= a.n_tiles_total) return false;
+  int tile;
+  if (a.tile_order) {
+    if ((int)blockIdx.x >= a.n_tiles_total) return false;
+    tile = a.tile_order[blockIdx.x];
+  } else {
+    tile = xcd_tile(blockIdx.x, a.n_tiles_total);
+  }
+  if (tile < 0 || tile >= a.n_tiles_total) return false;
   int local = tile, ntw = a.n_tiles_w, W = a.W, H = a.H;
   float tlx, tly, psx = a.psx, psy = a.psy;
   int64_t pix_base = 0;
@@ -1204,6 +1213,84 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Backward launch order.  A tile's backward cost is set by how far its waves
+// walk the list: the sum over its four waves of the largest n_done (+ a fixed
+// per-tile cost).  Costs vary ~3x between tiles, and tiles in image order put
+// expensive ones late; launching the tiles most expensive first (longest
+// processing time first) shortens the tail of the launch.  The order is a
+// counting sort on the cost quantised to 1024 levels, in one workgroup.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tile_cost(RasterArgs a, uint32_t* __restrict__ cost) {
+  __shared__ int s_w[4];
+  TileCtx tc;
+  if (!tile_setup(a, tc)) return;
+  int nd = tc.inside ? a.n_done_saved[tc.pix] : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nd = max(nd, __shfl_xor(nd, o));
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = nd;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int s0 = a.start[tc.tile], e0 = a.end[tc.tile];
+    const bool empty = s0 == -1 || e0 <= s0;
+    cost[tc.tile] = empty ? 0u : (uint32_t)(s_w[0] + s_w[1] + s_w[2] + s_w[3]) + 32u;
+  }
+}
+
+constexpr int kOrderBins = 1024;
+
+__global__ __launch_bounds__(kOrderBins) void k_tile_order(const uint32_t* __restrict__ cost, int n,
+                                                           int32_t* __restrict__ order) {
+  __shared__ uint32_t s_bin[kOrderBins];
+  __shared__ uint32_t s_wsum[kOrderBins / 64];
+  __shared__ uint32_t s_max;
+  const int t = threadIdx.x;
+  if (t == 0) s_max = 1u;
+  s_bin[t] = 0u;
+  __syncthreads();
+  uint32_t m = 0u;
+  for (int i = t; i < n; i += kOrderBins) m = max(m, cost[i]);
+  atomicMax(&s_max, m);
+  __syncthreads();
+  const uint64_t mx = s_max;
+  // bin 0 holds the most expensive tiles
+  auto bin_of = [mx](uint32_t c) {
+    return (kOrderBins - 1) - (int)(((uint64_t)c * (uint64_t)(kOrderBins - 1)) / mx);
+  };
+  for (int i = t; i < n; i += kOrderBins) atomicAdd(&s_bin[bin_of(cost[i])], 1u);
+  __syncthreads();
+  // exclusive prefix over the bins, one bin per thread
+  const uint32_t v = s_bin[t];
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d);
+    if ((t & 63) >= d) inc += o;
+  }
+  if ((t & 63) == 63) s_wsum[t >> 6] = inc;
+  __syncthreads();
+  uint32_t pre = 0u;
+  for (int w = 0; w < (t >> 6); ++w) pre += s_wsum[w];
+  s_bin[t] = pre + inc - v;  // first slot of bin t (each thread rewrites only its own bin)
+  __syncthreads();
+  for (int i = t; i < n; i += kOrderBins) order[atomicAdd(&s_bin[bin_of(cost[i])], 1u)] = i;
+}
+
+static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+static size_t order_ws_bytes(int n_tiles) { return 2 * align_up((size_t)(n_tiles > 0 ? n_tiles : 0) * 4); }
+
+// Fills a.tile_order from the saved n_done (workspace: costs, then order).
+static int make_tile_order(RasterArgs& a, void* ws, hipStream_t st) {
+  uint32_t* cost = reinterpret_cast<uint32_t*>(ws);
+  int32_t* order = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ws) + align_up((size_t)a.n_tiles_total * 4));
+  a.tile_order = nullptr;
+  hipLaunchKernelGGL(k_tile_cost, dim3(xcd_grid(a.n_tiles_total)), dim3(256), 0, st, a, cost);
+  hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderBins), 0, st, cost, a.n_tiles_total, order);
+  GSMI_CHECK_LAUNCH();
+  a.tile_order = order;
+  return GSMI_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 template <int C, bool BG>
@@ -1293,13 +1380,14 @@ extern "C" int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const floa
   }
 }
 
-extern "C" int gsmi_tile_based_vol_rendering_backward_sh(
-    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
-    int64_t sh_stride_c, const int32_t* sh_index, const float* alpha, const int32_t* start, const int32_t* end,
-    const int32_t* gaussian_ids, const float* out, const float* T_final, const int32_t* n_done,
-    const float* pre_bg, const float* grad_out, float* grad_mean, float* grad_cov, float* grad_sh,
-    float* grad_alpha, const float* topleft, const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w,
-    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float* bg, void* stream) {
+static int bwd_sh_impl(int64_t V, int64_t P, const float* mean, const float* cov, const float* sh,
+                       int64_t sh_stride_g, int64_t sh_stride_c, const int32_t* sh_index, const float* alpha,
+                       const int32_t* start, const int32_t* end, const int32_t* gaussian_ids, const float* out,
+                       const float* T_final, const int32_t* n_done, const float* pre_bg, const float* grad_out,
+                       float* grad_mean, float* grad_cov, float* grad_sh, float* grad_alpha, const float* topleft,
+                       const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x,
+                       float pixel_size_y, int H, int W, int C, float thresh, const float* bg, void* workspace,
+                       size_t workspace_bytes, void* stream) {
   int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
   if (rc) return rc;
   if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
@@ -1342,6 +1430,11 @@ extern "C" int gsmi_tile_based_vol_rendering_backward_sh(
   a.H = H;
   a.W = W;
   hipStream_t st = (hipStream_t)stream;
+  if (workspace) {
+    if (workspace_bytes < order_ws_bytes(a.n_tiles_total)) return GSMI_ERR_WORKSPACE;
+    rc = make_tile_order(a, workspace, st);
+    if (rc) return rc;
+  }
   const bool B = bg != nullptr;
   switch (C) {
     case 1: return B ? launch_bwd_sh<1, true>(a, st) : launch_bwd_sh<1, false>(a, st);
@@ -1349,6 +1442,35 @@ extern "C" int gsmi_tile_based_vol_rendering_backward_sh(
     case 3: return B ? launch_bwd_sh<3, true>(a, st) : launch_bwd_sh<3, false>(a, st);
     default: return B ? launch_bwd_sh<4, true>(a, st) : launch_bwd_sh<4, false>(a, st);
   }
+}
+
+extern "C" int gsmi_tile_based_vol_rendering_backward_sh(
+    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
+    int64_t sh_stride_c, const int32_t* sh_index, const float* alpha, const int32_t* start, const int32_t* end,
+    const int32_t* gaussian_ids, const float* out, const float* T_final, const int32_t* n_done,
+    const float* pre_bg, const float* grad_out, float* grad_mean, float* grad_cov, float* grad_sh,
+    float* grad_alpha, const float* topleft, const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float* bg, void* stream) {
+  return bwd_sh_impl(V, P, mean, cov, sh, sh_stride_g, sh_stride_c, sh_index, alpha, start, end, gaussian_ids, out,
+                     T_final, n_done, pre_bg, grad_out, grad_mean, grad_cov, grad_sh, grad_alpha, topleft, c2w,
+                     tile_size, n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg, nullptr, 0,
+                     stream);
+}
+
+extern "C" size_t gsmi_backward_order_workspace_size(int n_tiles_total) { return order_ws_bytes(n_tiles_total); }
+
+extern "C" int gsmi_tile_based_vol_rendering_backward_sh_ws(
+    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
+    int64_t sh_stride_c, const int32_t* sh_index, const float* alpha, const int32_t* start, const int32_t* end,
+    const int32_t* gaussian_ids, const float* out, const float* T_final, const int32_t* n_done,
+    const float* pre_bg, const float* grad_out, float* grad_mean, float* grad_cov, float* grad_sh,
+    float* grad_alpha, const float* topleft, const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float* bg, void* workspace,
+    size_t workspace_bytes, void* stream) {
+  return bwd_sh_impl(V, P, mean, cov, sh, sh_stride_g, sh_stride_c, sh_index, alpha, start, end, gaussian_ids, out,
+                     T_final, n_done, pre_bg, grad_out, grad_mean, grad_cov, grad_sh, grad_alpha, topleft, c2w,
+                     tile_size, n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg, workspace,
+                     workspace_bytes, stream);
 }
 
 extern "C" int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, const float* mean, const float* cov,
@@ -1502,7 +1624,8 @@ extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const 
                                              const float* T_final, const int32_t* n_done, const float* pre_bg,
                                              const float* grad_out, float* grad_mean, float* grad_cov,
                                              float* grad_sh, float* grad_alpha, int C, float thresh,
-                                             const float* bg, void* stream) {
+                                             const float* bg, void* workspace, size_t workspace_bytes,
+                                             void* stream) {
   int rc = check_views(V, P, B, views, n_tiles_total);
   if (rc) return rc;
   if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
@@ -1539,6 +1662,11 @@ extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const 
   a.n_views = B;
   a.n_tiles_total = n_tiles_total;
   hipStream_t st = (hipStream_t)stream;
+  if (workspace) {
+    if (workspace_bytes < order_ws_bytes(n_tiles_total)) return GSMI_ERR_WORKSPACE;
+    rc = make_tile_order(a, workspace, st);
+    if (rc) return rc;
+  }
   const bool G = bg != nullptr;
   switch (C) {
     case 1: return G ? launch_bwd_sh<1, true>(a, st) : launch_bwd_sh<1, false>(a, st);

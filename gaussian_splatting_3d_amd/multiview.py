@@ -195,11 +195,14 @@ def backward_views(s: ViewsState, mean, qvec, log_s, logit, sh_coeffs, grad_out)
     go = grad_out.contiguous().view(-1)
     if go.numel() != b.n_pix * 3:
         raise RuntimeError(f"grad_out has {go.numel()} values, expected {b.n_pix * 3}")
+    nws = int(L.gsmi_backward_order_workspace_size(b.n_tiles))  # launch order, most expensive tiles first
+    ows = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
     check(L.gsmi_render_views_backward_sh(b.V, b.P, b.B, ptr(b.views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d),
                                           ptr(sh_coeffs), 3 * cmax2, cmax2, ptr(b.orig), ptr(b.alpha), ptr(b.start),
                                           ptr(b.end), ptr(b.ids), ptr(s.out), ptr(s.T), ptr(s.n_done),
                                           ptr(s.pre_bg), ptr(go), ptr(g_m2d), ptr(g_cov), ptr(g_sh), ptr(g_al), s.C,
-                                          float(s.cfg.T_thresh), ptr(s.bg), st), "render_views_backward_sh")
+                                          float(s.cfg.T_thresh), ptr(s.bg), ptr(ows), nws, st),
+          "render_views_backward_sh")
     g_mean = torch.empty_like(mean)
     g_q = torch.empty_like(qvec)
     g_ls = torch.empty_like(log_s)

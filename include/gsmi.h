@@ -228,6 +228,23 @@ int gsmi_tile_based_vol_rendering_backward_sh(
     float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float *bg, void *stream);
 
 /*
+ * Same as gsmi_tile_based_vol_rendering_backward_sh, with a workspace of
+ * gsmi_backward_order_workspace_size(n_tiles_h * n_tiles_w) bytes: the tiles
+ * are launched most expensive first (cost from n_done), which shortens the
+ * launch's tail.  No counterpart in the reference (a scheduling variant of
+ * the entry above; the gradients are the same).
+ */
+size_t gsmi_backward_order_workspace_size(int n_tiles_total);
+int gsmi_tile_based_vol_rendering_backward_sh_ws(
+    int64_t V, int64_t P, const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
+    int64_t sh_stride_c, const int32_t *sh_index, const float *alpha, const int32_t *start, const int32_t *end,
+    const int32_t *gaussian_ids, const float *out, const float *T_final, const int32_t *n_done,
+    const float *pre_bg, const float *grad_out, float *grad_mean, float *grad_cov, float *grad_sh,
+    float *grad_alpha, const float *topleft, const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float *bg, void *workspace,
+    size_t workspace_bytes, void *stream);
+
+/*
  * Replaces tile_based_vol_rendering_start_end (render.cu:399-441,
  * vol_render.h:716-798; bound at bindings.cpp:31): RGB compositing with the
  * Gaussian evaluated in double (kernels.h:195-224).  color [V,3].
@@ -317,7 +334,9 @@ int gsmi_render_views_sh(int64_t V, int64_t P, int B, const gsmi_view *views, in
                          void *stream);
 
 /* Reverse-order adjoint of gsmi_render_views_sh; gradients of records seen
- * in several views accumulate (SH rows through sh_index). */
+ * in several views accumulate (SH rows through sh_index).  workspace (may be
+ * null): gsmi_backward_order_workspace_size(n_tiles_total) bytes to launch
+ * the tiles most expensive first. */
 int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view *views, int n_tiles_total,
                                   const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
                                   int64_t sh_stride_c, const int32_t *sh_index, const float *alpha,
@@ -325,7 +344,7 @@ int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view *
                                   const float *out, const float *T_final, const int32_t *n_done,
                                   const float *pre_bg, const float *grad_out, float *grad_mean, float *grad_cov,
                                   float *grad_sh, float *grad_alpha, int C, float thresh, const float *bg,
-                                  void *stream);
+                                  void *workspace, size_t workspace_bytes, void *stream);
 
 /* Projection adjoint over all views: per Gaussian the gradients of every
  * view that sees it are summed in view order, then chained to the raw
