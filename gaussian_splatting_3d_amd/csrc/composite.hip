@@ -1218,7 +1218,9 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
 // per-tile cost).  Costs vary ~3x between tiles, and tiles in image order put
 // expensive ones late; launching the tiles most expensive first (longest
 // processing time first) shortens the tail of the launch.  The order is a
-// counting sort on the cost quantised to 1024 levels, in one workgroup.
+// stable counting sort on the cost quantised to 16 levels, in one workgroup:
+// within a level the tiles keep image order, so neighbours (which share
+// Gaussians) still run close together.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tile_cost(RasterArgs a, uint32_t* __restrict__ cost) {
   __shared__ int s_w[4];
@@ -1236,43 +1238,72 @@ __global__ __launch_bounds__(256) void k_tile_cost(RasterArgs a, uint32_t* __res
   }
 }
 
-constexpr int kOrderBins = 1024;
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderLevels = 16;  // cost levels; within a level tiles keep their index order
 
-__global__ __launch_bounds__(kOrderBins) void k_tile_order(const uint32_t* __restrict__ cost, int n,
-                                                           int32_t* __restrict__ order) {
-  __shared__ uint32_t s_bin[kOrderBins];
-  __shared__ uint32_t s_wsum[kOrderBins / 64];
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __restrict__ cost, int n,
+                                                              int32_t* __restrict__ order) {
+  constexpr int NW = kOrderThreads / 64;
+  __shared__ uint32_t s_cursor[kOrderLevels];
+  __shared__ uint32_t s_wc[NW][kOrderLevels];
   __shared__ uint32_t s_max;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (t == 0) s_max = 1u;
-  s_bin[t] = 0u;
+  if (t < kOrderLevels) s_cursor[t] = 0u;
   __syncthreads();
   uint32_t m = 0u;
-  for (int i = t; i < n; i += kOrderBins) m = max(m, cost[i]);
+  for (int i = t; i < n; i += kOrderThreads) m = max(m, cost[i]);
   atomicMax(&s_max, m);
   __syncthreads();
   const uint64_t mx = s_max;
-  // bin 0 holds the most expensive tiles
-  auto bin_of = [mx](uint32_t c) {
-    return (kOrderBins - 1) - (int)(((uint64_t)c * (uint64_t)(kOrderBins - 1)) / mx);
+  // level 0 holds the most expensive tiles
+  auto level_of = [mx](uint32_t c) {
+    return (kOrderLevels - 1) - (int)(((uint64_t)c * (uint64_t)(kOrderLevels - 1)) / mx);
   };
-  for (int i = t; i < n; i += kOrderBins) atomicAdd(&s_bin[bin_of(cost[i])], 1u);
+  for (int i = t; i < n; i += kOrderThreads) atomicAdd(&s_cursor[level_of(cost[i])], 1u);
   __syncthreads();
-  // exclusive prefix over the bins, one bin per thread
-  const uint32_t v = s_bin[t];
-  uint32_t inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(inc, d);
-    if ((t & 63) >= d) inc += o;
+  if (t == 0) {  // exclusive prefix over the levels
+    uint32_t run = 0u;
+    for (int l = 0; l < kOrderLevels; ++l) {
+      const uint32_t c = s_cursor[l];
+      s_cursor[l] = run;
+      run += c;
+    }
   }
-  if ((t & 63) == 63) s_wsum[t >> 6] = inc;
   __syncthreads();
-  uint32_t pre = 0u;
-  for (int w = 0; w < (t >> 6); ++w) pre += s_wsum[w];
-  s_bin[t] = pre + inc - v;  // first slot of bin t (each thread rewrites only its own bin)
-  __syncthreads();
-  for (int i = t; i < n; i += kOrderBins) order[atomicAdd(&s_bin[bin_of(cost[i])], 1u)] = i;
+  // Stable placement, kOrderThreads tiles per round: rank among the lanes of
+  // the wave with the same level (ballot match on the 4 level bits), then the
+  // counts of the waves before it, then the level's cursor.
+  for (int c0 = 0; c0 < n; c0 += kOrderThreads) {
+    const int i = c0 + t;
+    const bool valid = i < n;
+    const uint32_t lv = valid ? (uint32_t)level_of(cost[i]) : (uint32_t)kOrderLevels;  // 16: no level
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int bit = 0; bit < 5; ++bit) {
+      const bool on = (lv >> bit) & 1u;
+      const uint64_t bal = __ballot(on);
+      peers &= on ? bal : ~bal;
+    }
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+    if (lane < kOrderLevels) s_wc[w][lane] = 0u;
+    __syncthreads();
+    if (valid && below == 0) s_wc[w][lv] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = s_cursor[lv] + below;
+      for (int v = 0; v < w; ++v) pos += s_wc[v][lv];
+      order[pos] = i;
+    }
+    __syncthreads();
+    if (t < kOrderLevels) {
+      uint32_t add = 0u;
+      for (int v = 0; v < NW; ++v) add += s_wc[v][t];
+      s_cursor[t] += add;
+    }
+    __syncthreads();
+  }
 }
 
 static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
@@ -1284,7 +1315,7 @@ static int make_tile_order(RasterArgs& a, void* ws, hipStream_t st) {
   int32_t* order = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ws) + align_up((size_t)a.n_tiles_total * 4));
   a.tile_order = nullptr;
   hipLaunchKernelGGL(k_tile_cost, dim3(xcd_grid(a.n_tiles_total)), dim3(256), 0, st, a, cost);
-  hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderBins), 0, st, cost, a.n_tiles_total, order);
+  hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, st, cost, a.n_tiles_total, order);
   GSMI_CHECK_LAUNCH();
   a.tile_order = order;
   return GSMI_OK;
