@@ -82,7 +82,7 @@ def gpu_bench(scene, steps, warmup, exact=False):
                                                scene.alpha_before_activation, scene.sh_coeffs)]
     target = scenes.target_image(cam.h, cam.w, 3).to(dev).reshape(-1)
     t_fwd, t_bwd = Timed(), Timed()
-    orig_fwd, orig_bwd = _backend.sh_forward, _backend.sh_backward
+    orig_fwd, orig_bwd, orig_fwd2 = _backend.sh_forward, _backend.sh_backward, _backend.sh_forward_two_level
     state = {"on": False}
 
     def fwd_wrap(*a, **k):
@@ -91,13 +91,19 @@ def gpu_bench(scene, steps, warmup, exact=False):
         with t_fwd:
             return orig_fwd(*a, **k)
 
+    def fwd2_wrap(*a, **k):
+        if not state["on"]:
+            return orig_fwd2(*a, **k)
+        with t_fwd:
+            return orig_fwd2(*a, **k)
+
     def bwd_wrap(*a, **k):
         if not state["on"]:
             return orig_bwd(*a, **k)
         with t_bwd:
             return orig_bwd(*a, **k)
 
-    _backend.sh_forward, _backend.sh_backward = fwd_wrap, bwd_wrap
+    _backend.sh_forward, _backend.sh_backward, _backend.sh_forward_two_level = fwd_wrap, bwd_wrap, fwd2_wrap
 
     def step():
         out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg, exact=exact)
@@ -126,12 +132,16 @@ def gpu_bench(scene, steps, warmup, exact=False):
         f1 = time.perf_counter()
     finally:
         _backend.sh_forward, _backend.sh_backward = orig_fwd, orig_bwd
+        _backend.sh_forward_two_level = orig_fwd2
     b = st.b
     for g in grads[:5]:
         assert torch.isfinite(g).all()
+    # (Gaussian, 16-px tile) pairs, the unit of the compositing bytes; the
+    # two-level forward bins per 64-px block, so count them once, untimed
+    P = rasterize.preprocess_and_bin(*params[:4], c2w, cam, D=cfg.tile_culling_radius).P if b.shift else b.P
     return dict(ms_step=(t1 - t0) * 1e3 / steps, ms_fwd_only=(f1 - f0) * 1e3 / steps,
-                ms_composite_fwd=t_fwd.mean_ms(), ms_composite_bwd=t_bwd.mean_ms(), N=b.N, V=b.V, P=b.P,
-                H=cam.h, W=cam.w, C=scene.C)
+                ms_composite_fwd=t_fwd.mean_ms(), ms_composite_bwd=t_bwd.mean_ms(), N=b.N, V=b.V, P=P,
+                block_pairs=b.P if b.shift else None, H=cam.h, W=cam.w, C=scene.C)
 
 
 def gpu_bench_views(scene, steps, warmup):
@@ -426,6 +436,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
                      "alg_bytes_per_launch": int(ab[dom])},
     }
+    if r.get("block_pairs") is not None:
+        line["config"]["block_pairs"] = r["block_pairs"]  # (Gaussian, 64-px block) pairs, what is sorted
     if "ms_fwd_only" in r:
         line["fwd_only_mpix_s"] = round(r["n_pix"] / (r["ms_fwd_only"] / 1e3) / 1e6, 3)
     if not args.no_traffic:

@@ -55,6 +55,11 @@ SIGNATURES = {
     "gsmi_tile_based_vol_rendering_backward_sh_ws": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P,
                                                             _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
                                                             _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P, _SZ, _P]),
+    "gsmi_shift_rects": (_I32, [_I64, _P, _P, _I32, _P, _P, _P]),
+    "gsmi_two_level_offsets_workspace_size": (_SZ, [_I32]),
+    "gsmi_two_level_offsets": (_I32, [_I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
+    "gsmi_render_sh_two_level": (_I32, [_I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
+                                        _P, _P, _P, _I32, _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P, _I32, _P]),
     "gsmi_tile_based_vol_rendering_start_end": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
                                                        _I32, _F, _F, _I32, _I32, _F, _P, _P, _I32, _P]),
     "gsmi_tile_based_vol_rendering_backward_start_end": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,

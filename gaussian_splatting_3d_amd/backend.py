@@ -116,6 +116,24 @@ class _Backend:
             float(thresh), ptr(bg_rgb), ptr(T_final), ptr(n_done), ptr(pre_bg), 1 if write_empty else 0,
             1 if exact else 0, stream_of(out.device)), "tile_based_vol_rendering_sh")
 
+    def sh_forward_two_level(self, mean, cov, sh_coeffs, alpha, tl, br, block_start, block_end, block_ids, shift,
+                             tile_off, tile_ids, tile_end, out, topleft, c2w, n_tiles_h, n_tiles_w, pixel_size_x,
+                             pixel_size_y, H, W, C, thresh, T_final, n_done, exact=False, sh_stride_g=None,
+                             sh_stride_c=None, sh_index=None):
+        """Forward SH compositing from lists per block of 2^shift x 2^shift
+        tiles; records each tile's filtered list in tile_ids[tile_off[t] ..
+        tile_end[t]) for the backward.  No counterpart in the reference."""
+        L = _lib.load()
+        C = int(C)
+        sg = 3 * C * C if sh_stride_g is None else int(sh_stride_g)
+        sc = C * C if sh_stride_c is None else int(sh_stride_c)
+        check(L.gsmi_render_sh_two_level(
+            mean.shape[0], ptr(mean), ptr(cov), ptr(sh_coeffs), sg, sc, ptr(sh_index), ptr(alpha), ptr(tl), ptr(br),
+            ptr(block_start), ptr(block_end), ptr(block_ids), int(shift), ptr(tile_off), ptr(tile_ids),
+            ptr(tile_end), ptr(out), ptr(topleft), ptr(c2w), int(n_tiles_h), int(n_tiles_w), float(pixel_size_x),
+            float(pixel_size_y), int(H), int(W), C, float(thresh), ptr(T_final), ptr(n_done), 1 if exact else 0,
+            stream_of(out.device)), "render_sh_two_level")
+
     def sh_backward(self, mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, T_final, n_done, pre_bg,
                     grad_out, grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, topleft, c2w, tile_size, n_tiles_h,
                     n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb=None, sh_stride_g=None,

@@ -16,6 +16,7 @@
 // below 1/255 by a relative margin of 5e-4, far above the few-ulp gap between
 // q*(1/det) and q/det.  Everything else takes the exact path.
 #include "gsmi_device.h"
+#include "scan.h"
 
 namespace gsmi {
 
@@ -65,6 +66,19 @@ struct RasterArgs {
   // Optional launch order: block b works on tile tile_order[b] (a permutation
   // of the tiles); null: the XCD-contiguous mapping.
   const int32_t* tile_order;
+  // Two-level lists (forward, single view): block-of-tiles lists sstart/send/
+  // sids (blocks of 2^super_shift tiles a side), tile bounds rtl/rbr of the
+  // records; the forward writes each tile's filtered list to fids from
+  // foff[tile] on and its end to fend[tile].
+  const int32_t* sstart;
+  const int32_t* send;
+  const int32_t* sids;
+  const int32_t* rtl;
+  const int32_t* rbr;
+  const int32_t* foff;
+  int32_t* fids;
+  int32_t* fend;
+  int super_shift, n_super_w;
 };
 
 // Pixel of this thread in the block's tile.
@@ -243,12 +257,12 @@ __device__ __forceinline__ float dot_fast(const float* a, const float* b) {
 }
 
 // Stage entries [s0+base, s0+base+cnt) of the tile list into LDS.
-template <int SHN, int CC>
+template <int SHN, int CC, bool IDS_IN_LDS = false>
 __device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
                                             GeoB* s_gb, float2* s_gc, float* s_sh) {
   const int t = threadIdx.x;
   if (t < cnt) {
-    const int g = a.ids[s0 + base + t];
+    const int g = IDS_IN_LDS ? s_id[t] : a.ids[s0 + base + t];
     s_id[t] = g;
     const float2 m = reinterpret_cast<const float2*>(a.mean)[g];
     const float4 c = reinterpret_cast<const float4*>(a.cov)[g];
@@ -270,27 +284,90 @@ __device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int bas
   }
 }
 
+// Two-level lists: the next (up to kBatch) entries of the block-of-tiles list
+// [*s_cur, e0) whose tile bounds hold tile (tx, ty), in list order, into
+// s_id[0..n); the cursor moves past the last one taken.  A depth-ordered list
+// filtered this way is exactly the tile's own list.  256 candidates per round,
+// ranked by ballot; returns n (block-uniform).
+__device__ __forceinline__ int filter_batch(const RasterArgs& a, int e0, int tx, int ty, int* s_id, int* s_cur,
+                                            int* s_cnt, int* s_wh) {
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  if (t == 0) *s_cnt = 0;
+  __syncthreads();
+  for (;;) {
+    const int cur = *s_cur, have = *s_cnt;
+    if (have >= kBatch || cur >= e0) break;
+    const int c = cur + t;
+    bool hit = false;
+    int id = 0;
+    if (c < e0) {
+      id = a.sids[c];
+      const int2 lo = reinterpret_cast<const int2*>(a.rtl)[id];
+      const int2 hi = reinterpret_cast<const int2*>(a.rbr)[id];
+      hit = lo.x <= tx && tx <= hi.x && lo.y <= ty && ty <= hi.y;
+    }
+    const uint64_t bal = __ballot(hit);
+    if (lane == 0) s_wh[w] = __popcll(bal);
+    __syncthreads();
+    const int below =
+        (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+    int pre = 0;
+    for (int v = 0; v < w; ++v) pre += s_wh[v];
+    const int total = s_wh[0] + s_wh[1] + s_wh[2] + s_wh[3];
+    const int rank = pre + below;
+    const int space = kBatch - have;
+    if (hit && rank < space) s_id[have + rank] = id;
+    __syncthreads();  // every read of s_cur, s_cnt, s_wh before they change
+    if (total <= space) {
+      if (t == 0) {
+        *s_cur = cur + kTilePixels;
+        *s_cnt = have + total;
+      }
+    } else {
+      if (hit && rank == space) *s_cur = c;  // the first hit left for the next batch
+      if (t == 0) *s_cnt = kBatch;
+    }
+    __syncthreads();
+  }
+  return *s_cnt;
+}
+
 // ---------------------------------------------------------------------------
 // Forward, SH colour.  vol_render_sh.h:171-248 (+ vol_render_bg.h:12-100).
 // ---------------------------------------------------------------------------
-template <int C, bool BG, bool EXACT>
+// TWO: the tile's list is filtered on the fly from its block's list (see
+// filter_batch) and recorded for the backward; single view, no background.
+template <int C, bool BG, bool EXACT, bool TWO = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_fwd_sh(RasterArgs a) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
+  static_assert(!(TWO && BG), "two-level lists without background");
   __shared__ GeoA s_ga[kBatch];
   __shared__ GeoB s_gb[kBatch];
   __shared__ float2 s_gc[kBatch];
   __shared__ int s_id[kBatch];
   __shared__ float s_sh[kBatch * SHN];
+  __shared__ int s_fcur, s_fcnt, s_fwh[4];
 
   TileCtx tc;
   if (!tile_setup(a, tc)) return;
   const int tile = tc.tile;
   const bool inside = tc.inside;
   const int64_t pix = tc.pix;
-  const int s0 = a.start[tile];
-  const int e0 = a.end[tile];
+  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;  // TWO only (single view)
+  int s0, e0;
+  if constexpr (TWO) {
+    const int sb = (ty >> a.super_shift) * a.n_super_w + (tx >> a.super_shift);
+    s0 = a.sstart[sb];
+    e0 = a.send[sb];
+  } else {
+    s0 = a.start[tile];
+    e0 = a.end[tile];
+  }
   if (s0 == -1 || e0 - s0 == 0) {
+    if constexpr (TWO) {
+      if (threadIdx.x == 0) a.fend[tile] = a.foff[tile];
+    }
     if (inside) {
       if (BG && s0 == -1) {
         a.out[3 * pix + 0] = a.bg[0];
@@ -320,8 +397,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   const float thresh = a.thresh;
   float T = 1.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
   bool done = !inside;
-  const int n = e0 - s0;
-  int ndone = n;
+  const int n = e0 - s0;      // TWO: the block's list; the tile's length is known at the end
+  int ndone = TWO ? -1 : n;
   if (inside && T < thresh) {
     done = true;
     ndone = 0;
@@ -329,10 +406,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   static_assert(kBatch <= 64, "one batch entry per lane in the relevance mask");
   const float4 wr = wave_pixel_rect(px, py);
   const int lane = threadIdx.x & 63;
-  for (int base = 0; base < n; base += kBatch) {
-    const int cnt = min(kBatch, n - base);
+  if (TWO && threadIdx.x == 0) s_fcur = s0;
+  int base = 0;
+  for (;;) {
+    int cnt;
     __syncthreads();
-    stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    if constexpr (TWO) {
+      cnt = filter_batch(a, e0, tx, ty, s_id, &s_fcur, &s_fcnt, s_fwh);
+      if (cnt == 0) break;
+      if ((int)threadIdx.x < cnt) a.fids[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
+      stage_batch<SHN, CC, true>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    } else {
+      if (base >= n) break;
+      cnt = min(kBatch, n - base);
+      stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    }
     __syncthreads();
     // entries of the batch that can reach a pixel of this wave, in order
     const int le = lane < cnt ? lane : 0;
@@ -387,7 +475,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         ndone = base + j + 1;
       }
     }
+    base += cnt;
     if (!__syncthreads_or(!done)) break;
+  }
+  if constexpr (TWO) {
+    if (ndone < 0) ndone = base;  // list ran out first: every filtered entry examined
+    if (threadIdx.x == 0) a.fend[tile] = a.foff[tile] + base;
   }
   if (!inside) return;
   if (BG) {
@@ -1705,4 +1798,108 @@ extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const 
     case 3: return G ? launch_bwd_sh<3, true>(a, st) : launch_bwd_sh<3, false>(a, st);
     default: return G ? launch_bwd_sh<4, true>(a, st) : launch_bwd_sh<4, false>(a, st);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Two-level lists (single view, no background): per-tile offsets of the
+// recorded filtered lists, and the forward that fills them.
+// ---------------------------------------------------------------------------
+namespace gsmi {
+// Capacity of tile t's recorded list: the length of its block's list.
+__global__ void k_tile_caps(int n_tiles_h, int n_tiles_w, int shift, int n_super_w, const int32_t* __restrict__ sstart,
+                            const int32_t* __restrict__ send, int32_t* __restrict__ cap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles_h * n_tiles_w) return;
+  const int tx = t % n_tiles_w, ty = t / n_tiles_w;
+  const int sb = (ty >> shift) * n_super_w + (tx >> shift);
+  const int s0 = sstart[sb], e0 = send[sb];
+  cap[t] = (s0 == -1 || e0 <= s0) ? 0 : e0 - s0;
+}
+}  // namespace gsmi
+
+extern "C" size_t gsmi_two_level_offsets_workspace_size(int n_tiles) {
+  return scan_workspace_bytes(n_tiles < 0 ? 0 : n_tiles, 4);
+}
+
+extern "C" int gsmi_two_level_offsets(int n_tiles_h, int n_tiles_w, int shift, const int32_t* sstart,
+                                      const int32_t* send, int32_t* foff, void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+  if (n_tiles_h <= 0 || n_tiles_w <= 0 || shift < 0 || shift > 4 || !sstart || !send || !foff) return GSMI_ERR_ARG;
+  const int n = n_tiles_h * n_tiles_w;
+  if (!workspace || workspace_bytes < gsmi_two_level_offsets_workspace_size(n)) return GSMI_ERR_WORKSPACE;
+  const int n_super_w = (n_tiles_w + (1 << shift) - 1) >> shift;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_tile_caps, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n_tiles_h, n_tiles_w, shift,
+                     n_super_w, sstart, send, foff);
+  GSMI_CHECK_LAUNCH();
+  return exclusive_scan<int32_t>(foff, foff, n, nullptr, workspace, st);
+}
+
+extern "C" int gsmi_render_sh_two_level(int64_t V, const float* mean, const float* cov, const float* sh,
+                                        int64_t sh_stride_g, int64_t sh_stride_c, const int32_t* sh_index,
+                                        const float* alpha, const int32_t* tl, const int32_t* br,
+                                        const int32_t* sstart, const int32_t* send, const int32_t* sids,
+                                        int shift, const int32_t* foff, int32_t* fids, int32_t* fend, float* out,
+                                        const float* topleft, const float* c2w, int n_tiles_h, int n_tiles_w,
+                                        float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh,
+                                        float* T_final, int32_t* n_done, int exact, void* stream) {
+  int rc = check_common(V, 0, kTile, n_tiles_h, n_tiles_w, H, W);
+  if (rc) return rc;
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (shift < 0 || shift > 4) return GSMI_ERR_ARG;
+  if (!sstart || !send || !foff || !fend || !out || !topleft || !c2w) return GSMI_ERR_ARG;
+  if (V > 0 && (!mean || !cov || !sh || !alpha || !tl || !br || !sids || !fids)) return GSMI_ERR_ARG;
+  if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15) || ((uintptr_t)tl & 7) || ((uintptr_t)br & 7))
+    return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = sh;
+  a.sh_index = sh_index;
+  a.alpha = alpha;
+  a.topleft = topleft;
+  a.c2w = c2w;
+  a.out = out;
+  a.T_final = T_final;
+  a.n_done = n_done;
+  a.sh_sg = sh_stride_g;
+  a.sh_sc = sh_stride_c;
+  a.psx = pixel_size_x;
+  a.psy = pixel_size_y;
+  a.thresh = thresh;
+  a.n_tiles_h = n_tiles_h;
+  a.n_tiles_w = n_tiles_w;
+  a.n_tiles_total = n_tiles_h * n_tiles_w;
+  a.H = H;
+  a.W = W;
+  a.write_empty = 1;
+  a.sstart = sstart;
+  a.send = send;
+  a.sids = sids;
+  a.rtl = tl;
+  a.rbr = br;
+  a.foff = foff;
+  a.fids = fids;
+  a.fend = fend;
+  a.super_shift = shift;
+  a.n_super_w = (n_tiles_w + (1 << shift) - 1) >> shift;
+  hipStream_t st = (hipStream_t)stream;
+  const int grid = xcd_grid(a.n_tiles_total);
+#define GSMI_FWD2(CC_)                                                                      \
+  do {                                                                                       \
+    if (exact)                                                                               \
+      hipLaunchKernelGGL((k_fwd_sh<CC_, false, true, true>), dim3(grid), dim3(256), 0, st, a);  \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_fwd_sh<CC_, false, false, true>), dim3(grid), dim3(256), 0, st, a); \
+  } while (0)
+  switch (C) {
+    case 1: GSMI_FWD2(1); break;
+    case 2: GSMI_FWD2(2); break;
+    case 3: GSMI_FWD2(3); break;
+    default: GSMI_FWD2(4); break;
+  }
+#undef GSMI_FWD2
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
 }

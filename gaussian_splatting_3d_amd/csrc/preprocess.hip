@@ -222,7 +222,9 @@ extern "C" int gsmi_preprocess_count(int64_t N, const float* mean, const float* 
   if (N < 0 || W <= 0 || H <= 0 || !c2w_host || !totals) return GSMI_ERR_ARG;
   if (!skip_culling && (!normals_host || !pts_host)) return GSMI_ERR_ARG;
   if (N > 0 && (!mean || !qvec || !log_s || !vis || !vis_off || !pair_off)) return GSMI_ERR_ARG;
-  if (tile_size != kTile) return GSMI_ERR_UNSUPPORTED;
+  // 16-pixel tiles; 32 or 64 count the pairs of 2x2 / 4x4-tile blocks (the
+  // floor division of the same pixel bounds), for two-level binning
+  if (tile_size != kTile && tile_size != 2 * kTile && tile_size != 4 * kTile) return GSMI_ERR_UNSUPPORTED;
   if (N >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
   if (((uintptr_t)qvec & 15) != 0) return GSMI_ERR_ARG;
   if (!workspace || workspace_bytes < gsmi_preprocess_workspace_size(N)) return GSMI_ERR_WORKSPACE;
@@ -268,6 +270,34 @@ extern "C" int gsmi_preprocess_emit(int64_t N, const float* mean, const float* q
   const CamArgs c = make_cam(c2w_host, fx, fy, cx, cy, W, H, D, 0.0f, 0, tile_size);
   hipLaunchKernelGGL(k_pre_emit, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, (hipStream_t)stream, N, mean,
                      qvec, log_s, logit, c, vis, vis_off, mean2d, cov2d, depth, alpha, tl, br, orig);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Two-level binning: tile bounds -> block bounds (blocks of 2^shift x 2^shift
+// tiles); the floor division of non-negative tile indices is a shift.
+// ---------------------------------------------------------------------------
+namespace gsmi {
+__global__ void k_shift_rects(int64_t V, const int2* __restrict__ tl, const int2* __restrict__ br, int shift,
+                              int2* __restrict__ tl_s, int2* __restrict__ br_s) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const int2 a = tl[v], b = br[v];
+  tl_s[v] = make_int2(a.x >> shift, a.y >> shift);
+  br_s[v] = make_int2(b.x >> shift, b.y >> shift);
+}
+}  // namespace gsmi
+
+extern "C" int gsmi_shift_rects(int64_t V, const int32_t* tl, const int32_t* br, int shift, int32_t* tl_s,
+                                int32_t* br_s, void* stream) {
+  if (V < 0 || shift < 0 || shift > 4) return GSMI_ERR_ARG;
+  if (V == 0) return GSMI_OK;
+  if (!tl || !br || !tl_s || !br_s) return GSMI_ERR_ARG;
+  if (((uintptr_t)tl | (uintptr_t)br | (uintptr_t)tl_s | (uintptr_t)br_s) & 7) return GSMI_ERR_ARG;
+  hipLaunchKernelGGL(k_shift_rects, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, (hipStream_t)stream, V,
+                     reinterpret_cast<const int2*>(tl), reinterpret_cast<const int2*>(br), shift,
+                     reinterpret_cast<int2*>(tl_s), reinterpret_cast<int2*>(br_s));
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
 }

@@ -14,6 +14,12 @@ Forward (all on the current torch stream):
      (tile<<32 | depth) key sort);
   6. gsmi_tile_based_vol_rendering_sh — compositing; SH rows are read from the
      raw [N,3,C_max^2] tensor through the original index (no gather copy).
+  Two-level form (cfg.two_level, the default without background): steps 1-5
+  count, sort and range the pairs per 64-px block of 4 x 4 tiles (about a
+  sixth of the tile pairs at 3M Gaussians / 1080p), and step 6 is
+  gsmi_render_sh_two_level, which filters each tile's list out of its block's
+  depth-ordered list as it composites (the same list, so the same image, T and
+  n_done) and records it for the backward.
 Backward:
   7. gsmi_tile_based_vol_rendering_backward_sh — reverse-order adjoint; SH
      gradients land directly in the raw-layout [N,3,C_max^2] buffer;
@@ -67,11 +73,20 @@ class Binning:
     end: torch.Tensor
     c2w_host: np.ndarray
     consts: dict
+    # two-level binning: lists per block of 2^shift x 2^shift tiles (ids,
+    # start and end above are then the per-tile lists the forward recorded)
+    shift: int = 0
+    block_ids: torch.Tensor | None = None
+    block_start: torch.Tensor | None = None
+    block_end: torch.Tensor | None = None
 
 
 def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, tile_size=TILE, D=6.0,
-                       frustum_culling_radius=1.0, skip_frustum_culling=False, want_keys=False) -> Binning:
-    """Steps 1-5 of the module docstring."""
+                       frustum_culling_radius=1.0, skip_frustum_culling=False, want_keys=False,
+                       block_shift=0) -> Binning:
+    """Steps 1-5 of the module docstring.  With block_shift > 0 the pairs are
+    binned per block of 2^shift x 2^shift tiles instead (P counts block
+    pairs); tl / br stay the tile bounds."""
     L = _lib.load()
     dev = mean.device
     st = stream_of(dev)
@@ -89,9 +104,11 @@ def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, til
     totals = torch.empty(2, dtype=torch.int64, device=dev)
     ws = torch.empty(max(int(L.gsmi_preprocess_workspace_size(N)), 1), dtype=torch.uint8, device=dev)
     c2w_c = host_f32(c2w_h)
+    if block_shift and tile_size != TILE:
+        raise ValueError("two-level binning needs 16-pixel tiles")
     check(L.gsmi_preprocess_count(N, ptr(mean), ptr(qvec), ptr(log_s), c2w_c, host_f32(normals), host_f32(pts),
                                   fx, fy, cx, cy, ic["W"], ic["H"], float(D), float(frustum_culling_radius),
-                                  1 if skip_frustum_culling else 0, tile_size, ptr(vis), ptr(vis_off),
+                                  1 if skip_frustum_culling else 0, tile_size << block_shift, ptr(vis), ptr(vis_off),
                                   ptr(pair_off), ptr(totals), ptr(ws), ws.numel(), st), "preprocess_count")
     V, P = (int(x) for x in totals.cpu().tolist())
     if P >= 2 ** 31:
@@ -107,16 +124,57 @@ def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, til
     check(L.gsmi_preprocess_emit(N, ptr(mean), ptr(qvec), ptr(log_s), ptr(logit), c2w_c, fx, fy, cx, cy, ic["W"],
                                  ic["H"], float(D), tile_size, ptr(vis), ptr(vis_off), ptr(mean2d), ptr(cov2d),
                                  ptr(depth), ptr(alpha), ptr(tl), ptr(br), ptr(orig), st), "preprocess_emit")
-    n_tiles = ic["n_tiles_h"] * ic["n_tiles_w"]
+    nth, ntw = ic["n_tiles_h"], ic["n_tiles_w"]
+    rtl, rbr = tl, br
+    if block_shift:
+        rtl = torch.empty_like(tl)
+        rbr = torch.empty_like(br)
+        check(L.gsmi_shift_rects(V, ptr(tl), ptr(br), block_shift, ptr(rtl), ptr(rbr), st), "shift_rects")
+        nth = (nth + (1 << block_shift) - 1) >> block_shift
+        ntw = (ntw + (1 << block_shift) - 1) >> block_shift
+    n_tiles = nth * ntw
     start = torch.empty(n_tiles, dtype=torch.int32, device=dev)
     end = torch.empty(n_tiles, dtype=torch.int32, device=dev)
     keys = torch.empty(P, dtype=torch.int64, device=dev) if want_keys else None
     bws = torch.empty(max(int(L.gsmi_binning_workspace_size(V, P, n_tiles)), 1), dtype=torch.uint8, device=dev)
-    check(L.gsmi_tile_culling_aabb_start_end(V, P, ptr(tl), ptr(br), ptr(depth), ic["n_tiles_h"], ic["n_tiles_w"],
-                                             ptr(ids), ptr(start), ptr(end), ptr(keys), ptr(bws), bws.numel(), 0,
-                                             st), "binning")
-    return Binning(N, V, P, vis, vis_off, pair_off, mean2d, cov2d, depth, alpha, tl, br, orig, keys, ids, start,
-                   end, c2w_h, ic)
+    check(L.gsmi_tile_culling_aabb_start_end(V, P, ptr(rtl), ptr(rbr), ptr(depth), nth, ntw, ptr(ids), ptr(start),
+                                             ptr(end), ptr(keys), ptr(bws), bws.numel(), 0, st), "binning")
+    b = Binning(N, V, P, vis, vis_off, pair_off, mean2d, cov2d, depth, alpha, tl, br, orig, keys, ids, start, end,
+                c2w_h, ic)
+    if block_shift:
+        b.shift, b.block_ids, b.block_start, b.block_end = block_shift, ids, start, end
+    return b
+
+
+BLOCK_SHIFT = 2  # two-level binning: blocks of 4 x 4 tiles (64 px)
+
+
+def _forward_two_level(b: Binning, sh_coeffs, out, topleft, c2w_d, C, thresh, T, nd, exact):
+    """Two-level forward: composite from the block lists, recording each
+    tile's filtered list; afterwards b.ids / b.start / b.end are those
+    per-tile lists (start = offset, end = offset + recorded length), which
+    the backward reads as ordinary tile lists."""
+    L = _lib.load()
+    ic = b.consts
+    dev = out.device
+    st = stream_of(dev)
+    nth, ntw = ic["n_tiles_h"], ic["n_tiles_w"]
+    n_tiles = nth * ntw
+    cap = (1 << (2 * b.shift)) * b.P  # >= sum over tiles of their block's list length
+    if cap >= 2 ** 31:
+        raise RuntimeError(f"two-level lists need {cap} entries, beyond the int32 range")
+    foff = torch.empty(n_tiles, dtype=torch.int32, device=dev)
+    fend = torch.empty(n_tiles, dtype=torch.int32, device=dev)
+    fids = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+    ows = torch.empty(max(int(L.gsmi_two_level_offsets_workspace_size(n_tiles)), 1), dtype=torch.uint8, device=dev)
+    check(L.gsmi_two_level_offsets(nth, ntw, b.shift, ptr(b.block_start), ptr(b.block_end), ptr(foff), ptr(ows),
+                                   ows.numel(), st), "two_level_offsets")
+    cmax2 = sh_coeffs.shape[-1]
+    _backend.sh_forward_two_level(b.mean2d, b.cov2d, sh_coeffs, b.alpha, b.tl, b.br, b.block_start, b.block_end,
+                                  b.block_ids, b.shift, foff, fids, fend, out, topleft, c2w_d, nth, ntw,
+                                  ic["pixel_size_x"], ic["pixel_size_y"], ic["H"], ic["W"], C, thresh, T, nd,
+                                  exact=exact, sh_stride_g=3 * cmax2, sh_stride_c=cmax2, sh_index=b.orig)
+    b.ids, b.start, b.end = fids, foff, fend
 
 
 @dataclass
@@ -137,9 +195,10 @@ class ViewState:
 
 def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact=False):
     """Steps 1-6: returns (image [H*W*3], ViewState)."""
+    two = bool(getattr(cfg, "two_level", False)) and not cfg.bg and cfg.tile_size == TILE
     b = preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera, tile_size=cfg.tile_size, D=cfg.tile_culling_radius,
                            frustum_culling_radius=cfg.frustum_culling_radius,
-                           skip_frustum_culling=cfg.skip_frustum_culling)
+                           skip_frustum_culling=cfg.skip_frustum_culling, block_shift=BLOCK_SHIFT if two else 0)
     ic = b.consts
     H, W = ic["H"], ic["W"]
     dev = mean.device
@@ -151,10 +210,13 @@ def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact
     nd = torch.empty(H * W, dtype=torch.int32, device=dev)
     pre = torch.empty(H * W * 3, dtype=torch.float32, device=dev) if cfg.bg else None
     cmax2 = sh_coeffs.shape[-1]
-    _backend.sh_forward(b.mean2d, b.cov2d, sh_coeffs, b.alpha, b.start, b.end, b.ids, out, topleft, c2w_d,
-                        cfg.tile_size, ic["n_tiles_h"], ic["n_tiles_w"], ic["pixel_size_x"], ic["pixel_size_y"], H,
-                        W, C, cfg.T_thresh, bg_rgb=bg, T_final=T, n_done=nd, pre_bg=pre, write_empty=True,
-                        exact=exact, sh_stride_g=3 * cmax2, sh_stride_c=cmax2, sh_index=b.orig)
+    if two:
+        _forward_two_level(b, sh_coeffs, out, topleft, c2w_d, C, cfg.T_thresh, T, nd, exact)
+    else:
+        _backend.sh_forward(b.mean2d, b.cov2d, sh_coeffs, b.alpha, b.start, b.end, b.ids, out, topleft, c2w_d,
+                            cfg.tile_size, ic["n_tiles_h"], ic["n_tiles_w"], ic["pixel_size_x"], ic["pixel_size_y"],
+                            H, W, C, cfg.T_thresh, bg_rgb=bg, T_final=T, n_done=nd, pre_bg=pre, write_empty=True,
+                            exact=exact, sh_stride_g=3 * cmax2, sh_stride_c=cmax2, sh_index=b.orig)
     return out, ViewState(b, int(C), cfg, topleft, c2w_d, bg, out, T, nd, pre)
 
 

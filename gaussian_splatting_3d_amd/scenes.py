@@ -31,6 +31,9 @@ class RasterConfig:
     skip_frustum_culling: bool = False
     bg: bool = False
     bg_rgb: tuple = (1.0, 1.0, 1.0)
+    # bin per 64-px block, filter to 16-px tiles inside the forward (same
+    # image, T and n_done); no counterpart in the reference, ignored with bg
+    two_level: bool = True
 
 
 @dataclass

@@ -245,6 +245,33 @@ int gsmi_tile_based_vol_rendering_backward_sh_ws(
     size_t workspace_bytes, void *stream);
 
 /*
+ * Two-level lists (no counterpart in the reference; single view, no
+ * background).  The pairs are binned per block of 2^shift x 2^shift tiles
+ * (gsmi_preprocess_count with tile_size 16 << shift gives their count, then
+ * gsmi_shift_rects and gsmi_tile_culling_aabb_start_end on the block grid),
+ * and the forward filters each tile's list from its block's depth-ordered
+ * list on the fly: the filtered list equals the tile's own list, so images,
+ * T_final and n_done are those of gsmi_tile_based_vol_rendering_sh.  The
+ * forward records each tile's filtered entries (up to where its pixels
+ * saturated) in fids[foff[t] .. fend[t]), so the backward takes (foff, fend,
+ * fids) as its start / end / gaussian_ids.  foff: exclusive prefix of the
+ * tiles' block-list lengths (gsmi_two_level_offsets); fids holds their sum,
+ * at most (2^shift)^2 x the block pair count.
+ */
+int gsmi_shift_rects(int64_t V, const int32_t *tl, const int32_t *br, int shift, int32_t *tl_s, int32_t *br_s,
+                     void *stream);
+size_t gsmi_two_level_offsets_workspace_size(int n_tiles);
+int gsmi_two_level_offsets(int n_tiles_h, int n_tiles_w, int shift, const int32_t *sstart, const int32_t *send,
+                           int32_t *foff, void *workspace, size_t workspace_bytes, void *stream);
+int gsmi_render_sh_two_level(int64_t V, const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
+                             int64_t sh_stride_c, const int32_t *sh_index, const float *alpha, const int32_t *tl,
+                             const int32_t *br, const int32_t *sstart, const int32_t *send, const int32_t *sids,
+                             int shift, const int32_t *foff, int32_t *fids, int32_t *fend, float *out,
+                             const float *topleft, const float *c2w, int n_tiles_h, int n_tiles_w,
+                             float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh,
+                             float *T_final, int32_t *n_done, int exact, void *stream);
+
+/*
  * Replaces tile_based_vol_rendering_start_end (render.cu:399-441,
  * vol_render.h:716-798; bound at bindings.cpp:31): RGB compositing with the
  * Gaussian evaluated in double (kernels.h:195-224).  color [V,3].

@@ -62,10 +62,10 @@ def test_full_size_binning_and_sampled_tiles(orc, which):
     H, W, nth, ntw = ic["H"], ic["W"], ic["n_tiles_h"], ic["n_tiles_w"]
     p = [t.to(DEV).contiguous() for t in (sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
                                           sc.sh_coeffs)]
-    cfg = scenes.RasterConfig()
+    cfg = scenes.RasterConfig()  # two-level lists (per 64-px block, filtered per tile)
 
-    # preprocessing and binning, bit for bit
-    out_x, st_x = rasterize.forward_view(*p, c2w, cam, C, cfg, exact=True)
+    # preprocessing and binning (per-tile lists), bit for bit
+    out_x, st_x = rasterize.forward_view(*p, c2w, cam, C, scenes.RasterConfig(two_level=False), exact=True)
     b = st_x.b
     assert (b.V, b.P) == (ref["idx"].size, ref["P"])
     assert np.array_equal(b.vis.cpu().numpy().astype(bool), ref["mask"])
@@ -97,6 +97,10 @@ def test_full_size_binning_and_sampled_tiles(orc, which):
     assert np.array_equal(g_out[pix3].view(np.uint32), o_out[pix3].view(np.uint32))
     assert np.array_equal(st_x.T.cpu().numpy()[pix].view(np.uint32), o_T[pix].view(np.uint32))
     assert np.array_equal(st_x.n_done.cpu().numpy()[pix], o_nd[pix])
+    # the two-level forward: the same image, T and n_done everywhere
+    out_2, st_2 = rasterize.forward_view(*p, c2w, cam, C, cfg, exact=True)
+    assert st_2.b.shift == 2
+    assert torch.equal(out_2, out_x) and torch.equal(st_2.T, st_x.T) and torch.equal(st_2.n_done, st_x.n_done)
 
     out_f, st_f = rasterize.forward_view(*p, c2w, cam, C, cfg)
     assert np.abs(out_f.cpu().numpy()[pix3] - o_out[pix3]).max() <= IMG_TOL

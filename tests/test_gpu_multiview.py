@@ -50,8 +50,9 @@ def test_views_match_single_view(C):
     flat, st = multiview.forward_views(*p, cams, C, cfg)
     b = st.b
     assert b.B == len(cams)
+    cfg1 = scenes.RasterConfig(two_level=False)  # the single view's own tile lists
     for v, (c2w, cam) in enumerate(cams):
-        out1, s1 = rasterize.forward_view(*p, c2w, cam, C, cfg)
+        out1, s1 = rasterize.forward_view(*p, c2w, cam, C, cfg1)
         m = b.meta[v]
         # records and tile lists: same Gaussians in the same order per tile
         assert int(b.bases[v + 1, 0] - b.bases[v, 0]) == s1.b.V
@@ -128,7 +129,7 @@ def test_full_size_views_match_single_view():
     b = st.b
     for v in (0, len(sc.cameras) - 1):
         c2w, cam = sc.cameras[v]
-        out1, s1 = rasterize.forward_view(*p, c2w, cam, sc.C, cfg)
+        out1, s1 = rasterize.forward_view(*p, c2w, cam, sc.C, scenes.RasterConfig(two_level=False))
         m = b.meta[v]
         assert int(b.bases[v + 1, 1] - b.bases[v, 1]) == s1.b.P
         assert torch.equal(st.image(v), out1.view(cam.h, cam.w, 3)), f"view {v}"

@@ -382,6 +382,83 @@ def test_backward_launch_order_entry_points_agree(C):
     assert rc2 == 4  # GSMI_ERR_WORKSPACE
 
 
+def _wide_scene(seed=29, n=20_000, W=600, H=344):
+    """Tile grid 38 x 22 (ragged 16-px tiles) and block grid 10 x 6 (ragged
+    4 x 4 blocks)."""
+    sc = small_sh_scene(4, n=n, W=W, H=H, seed=seed)
+    sc.cameras = [scenes.orbit_camera(4.0, 0.7, 260.0, 260.0, W, H, near=0.01, far=100.0, height=0.3)]
+    return sc
+
+
+@pytest.mark.parametrize("which", ["c1", "small4", "extreme", "wide"])
+def test_two_level_lists_exact_and_adjoint(orc, which):
+    """Two-level binning (per 64-px block, filtered to 16-px tiles inside the
+    forward): the image, T and n_done stay bit-identical to the oracle in
+    exact colour mode, each tile's recorded list is a prefix of the oracle's
+    tile list covering every entry its pixels examined, and the backward on
+    the recorded lists stays within 1e-3."""
+    sc = {"c1": scenes.plain_forward_10k, "small4": lambda: small_sh_scene(4), "extreme": extreme_shapes_scene,
+          "wide": _wide_scene}[which]()
+    c2w, cam = sc.cameras[0]
+    ref = _oracle_view(orc, sc)
+    cfg = scenes.RasterConfig(two_level=True)
+    p = dev_params(sc)
+    out_e, st = rasterize.forward_view(*p, c2w, cam, sc.C, cfg, exact=True)
+    assert st.b.shift == 2
+    got = out_e.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), ref["out"].view(np.uint32)), np.abs(got - ref["out"]).max()
+    assert np.array_equal(st.T.cpu().numpy().view(np.uint32), ref["T"].view(np.uint32))
+    nd = st.n_done.cpu().numpy()
+    assert np.array_equal(nd, ref["n_done"])
+    foff, fend, fids = (t.cpu().numpy() for t in (st.b.start, st.b.end, st.b.ids))
+    rs, re, rids = ref["start"], ref["end"], ref["ids"]
+    H, W = ref["H"], ref["W"]
+    ntw = ref["n_tiles_w"]
+    ndt = np.zeros(ref["n_tiles_h"] * ntw, np.int64)
+    ys, xs = np.divmod(np.arange(H * W), W)
+    np.maximum.at(ndt, (ys // 16) * ntw + xs // 16, nd.reshape(-1))
+    for t in range(ndt.size):
+        n = fend[t] - foff[t]
+        full = 0 if rs[t] == -1 else re[t] - rs[t]
+        assert ndt[t] <= n <= full, (t, ndt[t], n, full)
+        if n:
+            assert np.array_equal(fids[foff[t]:fend[t]], rids[rs[t]:rs[t] + n]), t
+    out_f, st_f = rasterize.forward_view(*p, c2w, cam, sc.C, cfg)
+    assert np.abs(out_f.cpu().numpy() - ref["out"]).max() <= IMG_TOL
+    gout = torch.rand(out_f.shape, generator=torch.Generator().manual_seed(5)).to(DEV) - 0.5
+    _, _, _, _, g_sh, bd = rasterize.backward_view(st_f, *p, gout)
+    rg, mism = orc.render_sh_bwd(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]], rs, re, rids,
+                                 ref["out"], gout.cpu().numpy(), ref["topleft"], c2w, 16, ref["n_tiles_h"], ntw,
+                                 ref["psx"], ref["psy"], H, W, sc.C, 1e-4)
+    assert mism == 0
+    assert rel_err(bd["mean2d"].cpu(), rg["mean"]) <= GRAD_RTOL
+    assert rel_err(bd["cov"].cpu(), rg["cov"]) <= GRAD_RTOL
+    assert rel_err(bd["alpha"].cpu(), rg["alpha"]) <= GRAD_RTOL
+    gsh = g_sh.cpu().numpy()[ref["idx"]][..., : sc.C * sc.C]
+    assert rel_err(gsh, rg["sh"]) <= GRAD_RTOL
+
+
+def test_two_level_matches_single_level_full_size():
+    """At 500k Gaussians / 1297x840 (the forward+backward workload) the
+    two-level path gives the single-level path's image and n_done bit for bit
+    (same arithmetic, same lists) and its gradients to the float atomics'
+    summation order."""
+    sc = scenes.fwd_bwd_500k()
+    c2w, cam = sc.cameras[0]
+    p = dev_params(sc)
+    gout = torch.rand(cam.h * cam.w * 3, generator=torch.Generator().manual_seed(7)).to(DEV) - 0.5
+    res = []
+    for two in (False, True):
+        out, st = rasterize.forward_view(*p, c2w, cam, sc.C, scenes.RasterConfig(two_level=two))
+        g = rasterize.backward_view(st, *p, gout)
+        res.append((out.cpu().numpy(), st.n_done.cpu().numpy(), [t.cpu() for t in g[:5]]))
+    assert np.array_equal(res[0][0].view(np.uint32), res[1][0].view(np.uint32))
+    assert np.array_equal(res[0][1], res[1][1])
+    for a, b in zip(res[1][2], res[0][2]):
+        assert torch.isfinite(a).all()
+        assert rel_err(a, b) <= 1e-4
+
+
 def test_full_chain_gradients(orc):
     sc = small_sh_scene(4, n=3000)
     c2w, cam = sc.cameras[0]
@@ -488,9 +565,12 @@ def test_mock_scene_golden(C):
     g = np.load(os.path.join(here, "mock_scene_sh.npz"))
     mean, qvec, log_s, color, logit, c2w, ocam_ = mock_scene()
     cam = CameraInfo(ocam_.fx, ocam_.fy, ocam_.cx, ocam_.cy, ocam_.w, ocam_.h, ocam_.near_plane, ocam_.far_plane)
-    cfg = scenes.RasterConfig(tile_culling_radius=10.0, frustum_culling_radius=4.0)
+    cfg = scenes.RasterConfig(tile_culling_radius=10.0, frustum_culling_radius=4.0, two_level=False)
     p = [torch.from_numpy(np.ascontiguousarray(x)).to(DEV) for x in (mean, qvec, log_s, logit, g[f"C{C}_sh"])]
     out, st = rasterize.forward_view(*p, torch.from_numpy(c2w), cam, C, cfg)
+    cfg2 = scenes.RasterConfig(tile_culling_radius=10.0, frustum_culling_radius=4.0, two_level=True)
+    out2, _ = rasterize.forward_view(*p, torch.from_numpy(c2w), cam, C, cfg2)
+    assert torch.equal(out2, out)
     assert st.b.P == int(g[f"C{C}_P"])
     assert np.array_equal(st.b.ids.cpu().numpy(), g[f"C{C}_ids"])
     assert np.array_equal(st.b.start.cpu().numpy(), g[f"C{C}_start"])
