@@ -68,6 +68,10 @@ SIGNATURES = {
     "gsmi_project_backward": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gsmi_preprocess_views_workspace_size": (_SZ, [_I64, _I32]),
     "gsmi_preprocess_views_count": (_I32, [_I64, _I32, _P, _P, _P, _P, _F, _F, _I32, _P, _P, _P, _P, _SZ, _P]),
+    "gsmi_preprocess_views_count_blocks": (_I32, [_I64, _I32, _P, _P, _P, _P, _F, _F, _I32, _I32, _P, _P, _P, _P,
+                                                  _SZ, _P]),
+    "gsmi_render_views_sh_two_level": (_I32, [_I64, _I32, _P, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P,
+                                              _P, _P, _I32, _P, _I32, _F, _P, _P, _I32, _P]),
     "gsmi_preprocess_views_emit": (_I32, [_I64, _I32, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                           _P]),
     "gsmi_bin_views": (_I32, [_I64, _I64, _I32, _P, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),

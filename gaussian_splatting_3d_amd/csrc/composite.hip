@@ -66,10 +66,12 @@ struct RasterArgs {
   // Optional launch order: block b works on tile tile_order[b] (a permutation
   // of the tiles); null: the XCD-contiguous mapping.
   const int32_t* tile_order;
-  // Two-level lists (forward, single view): block-of-tiles lists sstart/send/
-  // sids (blocks of 2^super_shift tiles a side), tile bounds rtl/rbr of the
-  // records; the forward writes each tile's filtered list to fids from
-  // foff[tile] on and its end to fend[tile].
+  // Two-level lists (forward): block-of-tiles lists sstart/send/sids (blocks
+  // of 2^super_shift tiles a side), tile bounds rtl/rbr of the records; with
+  // fids the forward writes each tile's filtered list to fids from
+  // foff[tile] on and its end to fend[tile].  Multi-view: bviews[b] holds
+  // view b's block grid (n_tiles_w/h, tile_base in block units).
+  const gsmi_view* bviews;
   const int32_t* sstart;
   const int32_t* send;
   const int32_t* sids;
@@ -84,6 +86,7 @@ struct RasterArgs {
 // Pixel of this thread in the block's tile.
 struct TileCtx {
   int tile;  // global tile id (index into start/end)
+  int view;  // view of the tile (0 for a single view)
   int gx, gy;
   bool inside;
   int64_t pix;  // flat pixel index into out / T_final / n_done / grad_out
@@ -103,8 +106,10 @@ __device__ __forceinline__ bool tile_setup(const RasterArgs& a, TileCtx& c) {
   int local = tile, ntw = a.n_tiles_w, W = a.W, H = a.H;
   float tlx, tly, psx = a.psx, psy = a.psy;
   int64_t pix_base = 0;
+  c.view = 0;
   if (a.views) {
-    const gsmi_view& v = a.views[view_of_tile(a.views, a.n_views, tile)];
+    c.view = view_of_tile(a.views, a.n_views, tile);
+    const gsmi_view& v = a.views[c.view];
     local = tile - v.tile_base;
     ntw = v.n_tiles_w;
     W = v.W;
@@ -354,10 +359,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   const int tile = tc.tile;
   const bool inside = tc.inside;
   const int64_t pix = tc.pix;
-  const int tx = tile % a.n_tiles_w, ty = tile / a.n_tiles_w;  // TWO only (single view)
+  const int tx = tc.gx / kTile, ty = tc.gy / kTile;  // the view's own tile coordinates (TWO only)
   int s0, e0;
   if constexpr (TWO) {
-    const int sb = (ty >> a.super_shift) * a.n_super_w + (tx >> a.super_shift);
+    int sb;
+    if (a.views) {
+      const gsmi_view& bv = a.bviews[tc.view];
+      sb = bv.tile_base + (ty >> a.super_shift) * bv.n_tiles_w + (tx >> a.super_shift);
+    } else {
+      sb = (ty >> a.super_shift) * a.n_super_w + (tx >> a.super_shift);
+    }
     s0 = a.sstart[sb];
     e0 = a.send[sb];
   } else {
@@ -366,7 +377,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   }
   if (s0 == -1 || e0 - s0 == 0) {
     if constexpr (TWO) {
-      if (threadIdx.x == 0) a.fend[tile] = a.foff[tile];
+      if (a.fend && threadIdx.x == 0) a.fend[tile] = a.foff[tile];
     }
     if (inside) {
       if (BG && s0 == -1) {
@@ -414,7 +425,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     if constexpr (TWO) {
       cnt = filter_batch(a, e0, tx, ty, s_id, &s_fcur, &s_fcnt, s_fwh);
       if (cnt == 0) break;
-      if ((int)threadIdx.x < cnt) a.fids[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
+      if (a.fids && (int)threadIdx.x < cnt) a.fids[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
       stage_batch<SHN, CC, true>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
     } else {
       if (base >= n) break;
@@ -480,7 +491,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   }
   if constexpr (TWO) {
     if (ndone < 0) ndone = base;  // list ran out first: every filtered entry examined
-    if (threadIdx.x == 0) a.fend[tile] = a.foff[tile] + base;
+    if (a.fend && threadIdx.x == 0) a.fend[tile] = a.foff[tile] + base;
   }
   if (!inside) return;
   if (BG) {
@@ -1817,6 +1828,26 @@ __global__ void k_tile_caps(int n_tiles_h, int n_tiles_w, int shift, int n_super
 }
 }  // namespace gsmi
 
+static int launch_fwd_two(const RasterArgs& a, int C, int exact, hipStream_t st) {
+  const int grid = xcd_grid(a.n_tiles_total);
+#define GSMI_FWD2(CC_)                                                                      \
+  do {                                                                                       \
+    if (exact)                                                                               \
+      hipLaunchKernelGGL((k_fwd_sh<CC_, false, true, true>), dim3(grid), dim3(256), 0, st, a);  \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_fwd_sh<CC_, false, false, true>), dim3(grid), dim3(256), 0, st, a); \
+  } while (0)
+  switch (C) {
+    case 1: GSMI_FWD2(1); break;
+    case 2: GSMI_FWD2(2); break;
+    case 3: GSMI_FWD2(3); break;
+    default: GSMI_FWD2(4); break;
+  }
+#undef GSMI_FWD2
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
+}
+
 extern "C" size_t gsmi_two_level_offsets_workspace_size(int n_tiles) {
   return scan_workspace_bytes(n_tiles < 0 ? 0 : n_tiles, 4);
 }
@@ -1884,22 +1915,48 @@ extern "C" int gsmi_render_sh_two_level(int64_t V, const float* mean, const floa
   a.fend = fend;
   a.super_shift = shift;
   a.n_super_w = (n_tiles_w + (1 << shift) - 1) >> shift;
-  hipStream_t st = (hipStream_t)stream;
-  const int grid = xcd_grid(a.n_tiles_total);
-#define GSMI_FWD2(CC_)                                                                      \
-  do {                                                                                       \
-    if (exact)                                                                               \
-      hipLaunchKernelGGL((k_fwd_sh<CC_, false, true, true>), dim3(grid), dim3(256), 0, st, a);  \
-    else                                                                                     \
-      hipLaunchKernelGGL((k_fwd_sh<CC_, false, false, true>), dim3(grid), dim3(256), 0, st, a); \
-  } while (0)
-  switch (C) {
-    case 1: GSMI_FWD2(1); break;
-    case 2: GSMI_FWD2(2); break;
-    case 3: GSMI_FWD2(3); break;
-    default: GSMI_FWD2(4); break;
-  }
-#undef GSMI_FWD2
-  GSMI_CHECK_LAUNCH();
-  return GSMI_OK;
+  return launch_fwd_two(a, C, exact, (hipStream_t)stream);
+}
+
+extern "C" int gsmi_render_views_sh_two_level(int64_t V, int B, const gsmi_view* views,
+                                              const gsmi_view* block_views, int n_tiles_total, const float* mean,
+                                              const float* cov, const float* sh, int64_t sh_stride_g,
+                                              int64_t sh_stride_c, const int32_t* sh_index, const float* alpha,
+                                              const int32_t* tl, const int32_t* br, const int32_t* bstart,
+                                              const int32_t* bend, const int32_t* bids, int shift, float* out,
+                                              int C, float thresh, float* T_final, int32_t* n_done, int exact,
+                                              void* stream) {
+  int rc = check_views(V, 0, B, views, n_tiles_total);
+  if (rc) return rc;
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (shift < 0 || shift > 4) return GSMI_ERR_ARG;
+  if (!block_views || ((uintptr_t)block_views & 7) || !bstart || !bend || !out) return GSMI_ERR_ARG;
+  if (V > 0 && (!mean || !cov || !sh || !alpha || !tl || !br || !bids)) return GSMI_ERR_ARG;
+  if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15) || ((uintptr_t)tl & 7) || ((uintptr_t)br & 7))
+    return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = sh;
+  a.sh_index = sh_index;
+  a.alpha = alpha;
+  a.out = out;
+  a.T_final = T_final;
+  a.n_done = n_done;
+  a.sh_sg = sh_stride_g;
+  a.sh_sc = sh_stride_c;
+  a.thresh = thresh;
+  a.write_empty = 1;
+  a.views = views;
+  a.n_views = B;
+  a.n_tiles_total = n_tiles_total;
+  a.bviews = block_views;
+  a.sstart = bstart;
+  a.send = bend;
+  a.sids = bids;
+  a.rtl = tl;
+  a.rbr = br;
+  a.super_shift = shift;
+  return launch_fwd_two(a, C, exact, (hipStream_t)stream);
 }

@@ -316,8 +316,8 @@ namespace gsmi {
 __global__ __launch_bounds__(256) void k_mv_count(int64_t N, int B, const float* __restrict__ mean,
                                                   const float* __restrict__ qvec, const float* __restrict__ log_s,
                                                   const gsmi_view* __restrict__ views, float D,
-                                                  float frustum_radius, int skip_culling, uint8_t* __restrict__ vis,
-                                                  Int2* __restrict__ cnt) {
+                                                  float frustum_radius, int skip_culling, int shift,
+                                                  uint8_t* __restrict__ vis, Int2* __restrict__ cnt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const float m0 = mean[3 * i], m1 = mean[3 * i + 1], m2 = mean[3 * i + 2];
@@ -331,8 +331,8 @@ __global__ __launch_bounds__(256) void k_mv_count(int64_t N, int B, const float*
     if (keep) {
       const Proj p = project_one(m0, m1, m2, q, s0, s1, s2, v.c2w);
       int lx, hx, ly, hy;
-      aabb_axis(p.mx, p.c[0], D, v.fx, v.cx, v.W - 1, kTile, &lx, &hx);
-      aabb_axis(p.my, p.c[3], D, v.fy, v.cy, v.H - 1, kTile, &ly, &hy);
+      aabb_axis(p.mx, p.c[0], D, v.fx, v.cx, v.W - 1, kTile << shift, &lx, &hx);
+      aabb_axis(p.my, p.c[3], D, v.fy, v.cy, v.H - 1, kTile << shift, &ly, &hy);
       pairs = (hx - lx + 1) * (hy - ly + 1);
     }
     const int64_t e = (int64_t)b * N + i;
@@ -401,11 +401,12 @@ extern "C" size_t gsmi_preprocess_views_workspace_size(int64_t N, int B) {
   return align_up(n * 8) + align_up(scan_workspace_bytes(n, 8)) + 256;
 }
 
-extern "C" int gsmi_preprocess_views_count(int64_t N, int B, const gsmi_view* views, const float* mean,
-                                           const float* qvec, const float* log_s, float D, float frustum_radius,
-                                           int skip_culling, uint8_t* vis, int32_t* vis_off, int64_t* bases,
-                                           void* workspace, size_t workspace_bytes, void* stream) {
-  if (N < 0 || B <= 0 || !views || !bases) return GSMI_ERR_ARG;
+extern "C" int gsmi_preprocess_views_count_blocks(int64_t N, int B, const gsmi_view* views, const float* mean,
+                                                  const float* qvec, const float* log_s, float D,
+                                                  float frustum_radius, int skip_culling, int shift, uint8_t* vis,
+                                                  int32_t* vis_off, int64_t* bases, void* workspace,
+                                                  size_t workspace_bytes, void* stream) {
+  if (N < 0 || B <= 0 || !views || !bases || shift < 0 || shift > 4) return GSMI_ERR_ARG;
   if (N > 0 && (!mean || !qvec || !log_s || !vis || !vis_off)) return GSMI_ERR_ARG;
   if (N * B >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
   if (((uintptr_t)qvec & 15) != 0 || ((uintptr_t)views & 7) != 0) return GSMI_ERR_ARG;
@@ -420,7 +421,7 @@ extern "C" int gsmi_preprocess_views_count(int64_t N, int B, const gsmi_view* vi
   Int2* total = reinterpret_cast<Int2*>(p);
   if (N > 0) {
     hipLaunchKernelGGL(k_mv_count, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, B, mean, qvec, log_s,
-                       views, D, frustum_radius, skip_culling, vis, cnt);
+                       views, D, frustum_radius, skip_culling, shift, vis, cnt);
     GSMI_CHECK_LAUNCH();
   }
   int rc = exclusive_scan<Int2>(cnt, cnt, n, total, scan_ws, st);
@@ -430,6 +431,14 @@ extern "C" int gsmi_preprocess_views_count(int64_t N, int B, const gsmi_view* vi
                      bases);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
+}
+
+extern "C" int gsmi_preprocess_views_count(int64_t N, int B, const gsmi_view* views, const float* mean,
+                                           const float* qvec, const float* log_s, float D, float frustum_radius,
+                                           int skip_culling, uint8_t* vis, int32_t* vis_off, int64_t* bases,
+                                           void* workspace, size_t workspace_bytes, void* stream) {
+  return gsmi_preprocess_views_count_blocks(N, B, views, mean, qvec, log_s, D, frustum_radius, skip_culling, 0, vis,
+                                            vis_off, bases, workspace, workspace_bytes, stream);
 }
 
 extern "C" int gsmi_preprocess_views_emit(int64_t N, int B, const gsmi_view* views, const float* mean,

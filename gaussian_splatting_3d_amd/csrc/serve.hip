@@ -127,17 +127,26 @@ struct Slot {
   std::thread waiter;
 };
 
-// Bytes of phase A / phase B for a batch (layout carved in run_batch).
+// Pairs are binned per block of 2^kBlockShift x 2^kBlockShift tiles; each
+// tile filters its block's list while compositing (the same list, so the
+// same image as per-tile binning, with a sixth of the pairs to sort at 1080p).
+constexpr int kBlockShift = 2;
+
+// Bytes of phase A / phase B for a batch (layout carved in run_batch); P
+// counts (record, block) pairs, n_blocks the blocks of all views.
 static size_t phase_a_bytes(int64_t N, int B) {
-  return align_up((size_t)B * sizeof(gsmi_view)) + align_up((size_t)(B * N)) + align_up((size_t)(B * N) * 4) +
-         align_up((size_t)(2 * (B + 1)) * 8) + align_up(gsmi_preprocess_views_workspace_size(N, B));
+  return align_up((size_t)(2 * B) * sizeof(gsmi_view)) + align_up((size_t)(B * N)) +
+         align_up((size_t)(B * N) * 4) + align_up((size_t)(2 * (B + 1)) * 8) +
+         align_up(gsmi_preprocess_views_workspace_size(N, B));
 }
 
-static size_t phase_b_bytes(int64_t V, int64_t P, int64_t n_tiles, int64_t n_pix) {
-  return align_up(V * 8) + align_up(V * 16) + align_up(V * 4) * 2 + align_up(V * 8) * 2 + align_up(V * 4) +
-         align_up(P * 4) + align_up(n_tiles * 4) * 2 + align_up(gsmi_binning_workspace_size(V, P, (int)n_tiles)) +
+static size_t phase_b_bytes(int64_t V, int64_t P, int64_t n_blocks, int64_t n_pix) {
+  return align_up(V * 8) + align_up(V * 16) + align_up(V * 4) * 2 + align_up(V * 8) * 4 + align_up(V * 4) +
+         align_up(P * 4) + align_up(n_blocks * 4) * 2 + align_up(gsmi_binning_workspace_size(V, P, (int)n_blocks)) +
          align_up(n_pix * 12);
 }
+
+static int blocks_of(int n_tiles) { return (n_tiles + (1 << kBlockShift) - 1) >> kBlockShift; }
 
 }  // namespace gsmi_serve
 
@@ -206,8 +215,10 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
   const Scene sc = s->scenes[reqs[0].scene];
   const int64_t N = sc.N;
   int B = (int)reqs.size();
-  // tile / pixel numbering of the batch
-  int64_t tile_base = 0, pix_base = 0;
+  // tile / block / pixel numbering of the batch; h_views[B + b] is view b
+  // with its block grid
+  const int B0 = B;
+  int64_t tile_base = 0, block_base = 0, pix_base = 0;
   for (int b = 0; b < B; ++b) {
     gsmi_view& v = reqs[b].view;
     v.tile_base = (int32_t)tile_base;
@@ -215,6 +226,12 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
     tile_base += (int64_t)v.n_tiles_w * v.n_tiles_h;
     pix_base += (int64_t)v.W * v.H;
     sl.h_views[b] = v;
+    gsmi_view bv = v;
+    bv.n_tiles_w = blocks_of(v.n_tiles_w);
+    bv.n_tiles_h = blocks_of(v.n_tiles_h);
+    bv.tile_base = (int32_t)block_base;
+    block_base += (int64_t)bv.n_tiles_w * bv.n_tiles_h;
+    sl.h_views[B + b] = bv;
   }
   // ---- phase A: count
   const size_t bytesA = phase_a_bytes(N, B);
@@ -225,7 +242,8 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
   }
   char* p = s->arena.ptr(offA);
   gsmi_view* views_d = reinterpret_cast<gsmi_view*>(p);
-  p += align_up((size_t)B * sizeof(gsmi_view));
+  gsmi_view* bviews_d = views_d + B0;
+  p += align_up((size_t)(2 * B) * sizeof(gsmi_view));
   uint8_t* vis = reinterpret_cast<uint8_t*>(p);
   p += align_up((size_t)(B * N));
   int32_t* vis_off = reinterpret_cast<int32_t*>(p);
@@ -235,13 +253,13 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
   void* cws = p;
   const size_t cws_bytes = gsmi_preprocess_views_workspace_size(N, B);
   hipStream_t st = sl.stream;
-  int rc = hipMemcpyAsync(views_d, sl.h_views, (size_t)B * sizeof(gsmi_view), hipMemcpyHostToDevice, st) ==
+  int rc = hipMemcpyAsync(views_d, sl.h_views, (size_t)(2 * B) * sizeof(gsmi_view), hipMemcpyHostToDevice, st) ==
                    hipSuccess
                ? GSMI_OK
                : GSMI_ERR_LAUNCH;
   if (!rc)
-    rc = gsmi_preprocess_views_count(N, B, views_d, sc.mean, sc.qvec, sc.log_s, s->D, s->frustum_radius, 0, vis,
-                                     vis_off, bases_d, cws, cws_bytes, st);
+    rc = gsmi_preprocess_views_count_blocks(N, B, views_d, sc.mean, sc.qvec, sc.log_s, s->D, s->frustum_radius, 0,
+                                            kBlockShift, vis, vis_off, bases_d, cws, cws_bytes, st);
   if (!rc && hipMemcpyAsync(sl.h_bases, bases_d, (size_t)(2 * (B + 1)) * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
     rc = GSMI_ERR_LAUNCH;
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = GSMI_ERR_LAUNCH;
@@ -261,7 +279,7 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
     const int64_t V = sl.h_bases[2 * kk], P = sl.h_bases[2 * kk + 1];
     int64_t nt = 0, np = 0;
     for (int b = 0; b < kk; ++b) {
-      nt += (int64_t)reqs[b].view.n_tiles_w * reqs[b].view.n_tiles_h;
+      nt += (int64_t)sl.h_views[B0 + b].n_tiles_w * sl.h_views[B0 + b].n_tiles_h;
       np += (int64_t)reqs[b].view.W * reqs[b].view.H;
     }
     return std::make_pair(P < (int64_t(1) << 31), phase_b_bytes(V, P, nt, np));
@@ -286,9 +304,10 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
     return false;
   }
   const int64_t V = sl.h_bases[2 * B], P = sl.h_bases[2 * B + 1];
-  int64_t n_tiles = 0, n_pix = 0;
+  int64_t n_tiles = 0, n_blocks = 0, n_pix = 0;
   for (int b = 0; b < B; ++b) {
     n_tiles += (int64_t)reqs[b].view.n_tiles_w * reqs[b].view.n_tiles_h;
+    n_blocks += (int64_t)sl.h_views[B0 + b].n_tiles_w * sl.h_views[B0 + b].n_tiles_h;
     n_pix += (int64_t)reqs[b].view.W * reqs[b].view.H;
   }
   const size_t bytesB = nb.second;
@@ -311,21 +330,25 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
   float* alpha = reinterpret_cast<float*>(take(V * 4));
   int32_t* tl = reinterpret_cast<int32_t*>(take(V * 8));
   int32_t* br = reinterpret_cast<int32_t*>(take(V * 8));
+  int32_t* tl_b = reinterpret_cast<int32_t*>(take(V * 8));
+  int32_t* br_b = reinterpret_cast<int32_t*>(take(V * 8));
   int32_t* orig = reinterpret_cast<int32_t*>(take(V * 4));
   int32_t* ids = reinterpret_cast<int32_t*>(take(P * 4));
-  int32_t* start = reinterpret_cast<int32_t*>(take(n_tiles * 4));
-  int32_t* end = reinterpret_cast<int32_t*>(take(n_tiles * 4));
-  const size_t bws_bytes = gsmi_binning_workspace_size(V, P, (int)n_tiles);
+  int32_t* start = reinterpret_cast<int32_t*>(take(n_blocks * 4));
+  int32_t* end = reinterpret_cast<int32_t*>(take(n_blocks * 4));
+  const size_t bws_bytes = gsmi_binning_workspace_size(V, P, (int)n_blocks);
   void* bws = take(bws_bytes);
   float* out = reinterpret_cast<float*>(take(n_pix * 12));
   rc = gsmi_preprocess_views_emit(N, B, views_d, sc.mean, sc.qvec, sc.log_s, sc.logit, s->D, vis, vis_off, mean2d,
                                   cov2d, depth, alpha, tl, br, orig, st);
+  if (!rc) rc = gsmi_shift_rects(V, tl, br, kBlockShift, tl_b, br_b, st);
   if (!rc)
-    rc = gsmi_bin_views(V, P, B, views_d, bases_d, (int)n_tiles, tl, br, depth, ids, start, end, bws, bws_bytes, st);
+    rc = gsmi_bin_views(V, P, B, bviews_d, bases_d, (int)n_blocks, tl_b, br_b, depth, ids, start, end, bws,
+                        bws_bytes, st);
   if (!rc)
-    rc = gsmi_render_views_sh(V, P, B, views_d, (int)n_tiles, mean2d, cov2d, sc.sh, 3 * sc.sh_stride_c,
-                              sc.sh_stride_c, orig, alpha, start, end, ids, out, sc.C, s->thresh, nullptr, nullptr,
-                              nullptr, nullptr, 1, s->exact, st);
+    rc = gsmi_render_views_sh_two_level(V, B, views_d, bviews_d, (int)n_tiles, mean2d, cov2d, sc.sh,
+                                        3 * sc.sh_stride_c, sc.sh_stride_c, orig, alpha, tl, br, start, end, ids,
+                                        kBlockShift, out, sc.C, s->thresh, nullptr, nullptr, s->exact, st);
   for (int b = 0; !rc && b < B; ++b) {
     const gsmi_view& v = reqs[b].view;
     if (hipMemcpyAsync(reqs[b].out, out + 3 * v.pix_base, (size_t)v.W * v.H * 12, hipMemcpyDeviceToDevice, st) !=
@@ -521,7 +544,7 @@ extern "C" int gsmi_server_create(int device, int64_t arena_bytes, int max_views
   for (auto& sl : s->slots) {
     bool ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
-              hipHostMalloc((void**)&sl.h_views, (size_t)max_views * sizeof(gsmi_view), hipHostMallocDefault) ==
+              hipHostMalloc((void**)&sl.h_views, (size_t)(2 * max_views) * sizeof(gsmi_view), hipHostMallocDefault) ==
                   hipSuccess &&
               hipHostMalloc((void**)&sl.h_bases, (size_t)(2 * (max_views + 1)) * 8, hipHostMallocDefault) ==
                   hipSuccess;

@@ -337,6 +337,14 @@ int gsmi_preprocess_views_count(int64_t N, int B, const gsmi_view *views, const 
                                 int32_t *vis_off, int64_t *bases, void *workspace, size_t workspace_bytes,
                                 void *stream);
 
+/* As gsmi_preprocess_views_count with the pairs counted per block of
+ * 2^shift x 2^shift tiles (shift 0: the call above).  No counterpart in the
+ * reference: the count of the two-level binning of a batch. */
+int gsmi_preprocess_views_count_blocks(int64_t N, int B, const gsmi_view *views, const float *mean,
+                                       const float *qvec, const float *log_s, float D, float frustum_radius,
+                                       int skip_culling, int shift, uint8_t *vis, int32_t *vis_off, int64_t *bases,
+                                       void *workspace, size_t workspace_bytes, void *stream);
+
 /* Compacted records of every visible (view, Gaussian), as
  * gsmi_preprocess_emit; tl/br are in the view's own tile coordinates. */
 int gsmi_preprocess_views_emit(int64_t N, int B, const gsmi_view *views, const float *mean, const float *qvec,
@@ -359,6 +367,20 @@ int gsmi_render_views_sh(int64_t V, int64_t P, int B, const gsmi_view *views, in
                          const int32_t *end, const int32_t *gaussian_ids, float *out, int C, float thresh,
                          const float *bg, float *T_final, int32_t *n_done, float *pre_bg, int flags, int exact,
                          void *stream);
+
+/* Two-level form of gsmi_render_views_sh (no counterpart in the reference;
+ * forward only, no background): block_views[b] is view b with its block grid
+ * (n_tiles_w/h = ceil(tiles / 2^shift), tile_base numbering the blocks of
+ * all views); bstart/bend/bids are gsmi_bin_views' lists over block_views
+ * (records' tl/br shifted by gsmi_shift_rects), tl/br the records' tile
+ * bounds.  Each tile filters its block's list as it composites, so the
+ * images (and T_final / n_done when given) equal gsmi_render_views_sh's. */
+int gsmi_render_views_sh_two_level(int64_t V, int B, const gsmi_view *views, const gsmi_view *block_views,
+                                   int n_tiles_total, const float *mean, const float *cov, const float *sh,
+                                   int64_t sh_stride_g, int64_t sh_stride_c, const int32_t *sh_index,
+                                   const float *alpha, const int32_t *tl, const int32_t *br, const int32_t *bstart,
+                                   const int32_t *bend, const int32_t *bids, int shift, float *out, int C,
+                                   float thresh, float *T_final, int32_t *n_done, int exact, void *stream);
 
 /* Reverse-order adjoint of gsmi_render_views_sh; gradients of records seen
  * in several views accumulate (SH rows through sh_index).  workspace (may be

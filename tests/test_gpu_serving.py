@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from gaussian_splatting_3d_amd import _lib, multiview, rasterize, scenes, serving
+from gaussian_splatting_3d_amd import _lib, rasterize, scenes, serving
 
 pytestmark = pytest.mark.gpu
 
@@ -62,19 +62,24 @@ def test_batched_requests_match_per_call():
         assert torch.equal(out, reference(p, C, c2w, cam)), rid
 
 
-def _phase_bytes(N, views, bases, L):
-    """Mirror of the server's per-batch arena accounting."""
+def _phase_bytes(N, p, cams, L):
+    """Mirror of the server's per-batch arena accounting (pairs counted per
+    block of 4 x 4 tiles, as the server bins them)."""
     au = lambda x: (int(x) + 255) & ~255
-    B = len(views)
-    a = au(B * 256) + au(B * N) + au(B * N * 4) + au(2 * (B + 1) * 8) + \
+    B = len(cams)
+    a = au(2 * B * 256) + au(B * N) + au(B * N * 4) + au(2 * (B + 1) * 8) + \
         au(L.gsmi_preprocess_views_workspace_size(N, B))
+    per = []
+    for c2w, cam in cams:
+        bn = rasterize.preprocess_and_bin(*p[:4], c2w, cam, block_shift=2)
+        ic = bn.consts
+        nb = ((ic["n_tiles_h"] + 3) // 4) * ((ic["n_tiles_w"] + 3) // 4)
+        per.append((bn.V, bn.P, nb, cam.h * cam.w))
 
     def b(k):
-        V, P = int(bases[k, 0]), int(bases[k, 1])
-        nt = sum(m["n_tiles"] for m in views[:k])
-        npx = sum(m["H"] * m["W"] for m in views[:k])
-        return au(V * 8) + au(V * 16) + au(V * 4) * 2 + au(V * 8) * 2 + au(V * 4) + au(P * 4) + au(nt * 4) * 2 + \
-            au(L.gsmi_binning_workspace_size(V, P, nt)) + au(npx * 12)
+        V, P, nb, npx = (sum(x[i] for x in per[:k]) for i in range(4))
+        return au(V * 8) + au(V * 16) + au(V * 4) * 2 + au(V * 8) * 4 + au(V * 4) + au(P * 4) + au(nb * 4) * 2 + \
+            au(L.gsmi_binning_workspace_size(V, P, nb)) + au(npx * 12)
 
     return a, b
 
@@ -83,8 +88,7 @@ def test_admission_splits_batches_that_do_not_fit():
     p, C = scene(4, 6000, 3)
     cams = [camera(k) for k in (2, 2, 2, 2)]  # four 160x128 views
     L = _lib.load()
-    _, st = multiview.forward_views(*p, cams, C, scenes.RasterConfig())
-    a, b = _phase_bytes(6000, st.b.meta, st.b.bases, L)
+    a, b = _phase_bytes(6000, p, cams, L)
     arena = a + b(1) + 4096  # room for the count of all four views and the buffers of one
     assert arena < a + b(2)
     with serving.RenderServer(arena_bytes=arena, max_views=4, n_slots=1) as srv:
