@@ -290,18 +290,20 @@ __device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int bas
 }
 
 // Two-level lists: the next (up to kBatch) entries of the block-of-tiles list
-// [*s_cur, e0) whose tile bounds hold tile (tx, ty), in list order, into
-// s_id[0..n); the cursor moves past the last one taken.  A depth-ordered list
-// filtered this way is exactly the tile's own list.  256 candidates per round,
-// ranked by ballot; returns n (block-uniform).
-__device__ __forceinline__ int filter_batch(const RasterArgs& a, int e0, int tx, int ty, int* s_id, int* s_cur,
-                                            int* s_cnt, int* s_wh) {
+// [cur, e0) whose tile bounds hold tile (tx, ty), in list order, into
+// s_id[0..n).  Each round tests 256 candidates and appends the hits, ranked
+// by ballot, to the ring s_q (head qh, qn queued); rounds run until a batch
+// is queued or the list ends, and the batch is taken from the head.  A
+// depth-ordered list filtered this way is exactly the tile's own list.  cur,
+// qh, qn and the returned n are block-uniform and live in registers; s_wh
+// alternates between two halves so one barrier per round suffices.
+constexpr int kFilterQueue = 512;  // >= kBatch - 1 + kTilePixels queued entries
+static_assert((kFilterQueue & (kFilterQueue - 1)) == 0 && kFilterQueue >= kBatch - 1 + kTilePixels, "ring size");
+
+__device__ __forceinline__ int filter_batch(const RasterArgs& a, int e0, int tx, int ty, int* s_id, int* s_q,
+                                            int* s_wh, int& cur, int& qh, int& qn, int& par) {
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  if (t == 0) *s_cnt = 0;
-  __syncthreads();
-  for (;;) {
-    const int cur = *s_cur, have = *s_cnt;
-    if (have >= kBatch || cur >= e0) break;
+  while (qn < kBatch && cur < e0) {
     const int c = cur + t;
     bool hit = false;
     int id = 0;
@@ -312,29 +314,25 @@ __device__ __forceinline__ int filter_batch(const RasterArgs& a, int e0, int tx,
       hit = lo.x <= tx && tx <= hi.x && lo.y <= ty && ty <= hi.y;
     }
     const uint64_t bal = __ballot(hit);
-    if (lane == 0) s_wh[w] = __popcll(bal);
+    int* wh = s_wh + 4 * par;
+    if (lane == 0) wh[w] = __popcll(bal);
     __syncthreads();
     const int below =
         (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
     int pre = 0;
-    for (int v = 0; v < w; ++v) pre += s_wh[v];
-    const int total = s_wh[0] + s_wh[1] + s_wh[2] + s_wh[3];
-    const int rank = pre + below;
-    const int space = kBatch - have;
-    if (hit && rank < space) s_id[have + rank] = id;
-    __syncthreads();  // every read of s_cur, s_cnt, s_wh before they change
-    if (total <= space) {
-      if (t == 0) {
-        *s_cur = cur + kTilePixels;
-        *s_cnt = have + total;
-      }
-    } else {
-      if (hit && rank == space) *s_cur = c;  // the first hit left for the next batch
-      if (t == 0) *s_cnt = kBatch;
-    }
-    __syncthreads();
+    for (int v = 0; v < w; ++v) pre += wh[v];
+    const int total = wh[0] + wh[1] + wh[2] + wh[3];
+    if (hit) s_q[(qh + qn + pre + below) & (kFilterQueue - 1)] = id;
+    qn += total;
+    cur += kTilePixels;
+    par ^= 1;
   }
-  return *s_cnt;
+  __syncthreads();  // the rounds' queue writes
+  const int n = min(kBatch, qn);
+  if (t < n) s_id[t] = s_q[(qh + t) & (kFilterQueue - 1)];
+  qh = (qh + n) & (kFilterQueue - 1);
+  qn -= n;
+  return n;
 }
 
 // ---------------------------------------------------------------------------
@@ -352,7 +350,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   __shared__ float2 s_gc[kBatch];
   __shared__ int s_id[kBatch];
   __shared__ float s_sh[kBatch * SHN];
-  __shared__ int s_fcur, s_fcnt, s_fwh[4];
+  __shared__ int s_fq[TWO ? kFilterQueue : 1];
+  __shared__ int s_fwh[8];
 
   TileCtx tc;
   if (!tile_setup(a, tc)) return;
@@ -417,13 +416,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   static_assert(kBatch <= 64, "one batch entry per lane in the relevance mask");
   const float4 wr = wave_pixel_rect(px, py);
   const int lane = threadIdx.x & 63;
-  if (TWO && threadIdx.x == 0) s_fcur = s0;
+  int fcur = s0, fqh = 0, fqn = 0, fpar = 0;  // filter state (TWO), block-uniform
   int base = 0;
   for (;;) {
     int cnt;
     __syncthreads();
     if constexpr (TWO) {
-      cnt = filter_batch(a, e0, tx, ty, s_id, &s_fcur, &s_fcnt, s_fwh);
+      cnt = filter_batch(a, e0, tx, ty, s_id, s_fq, s_fwh, fcur, fqh, fqn, fpar);
       if (cnt == 0) break;
       if (a.fids && (int)threadIdx.x < cnt) a.fids[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
       stage_batch<SHN, CC, true>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
