@@ -523,7 +523,8 @@ def train_main(args):
         "dtype": "f32",
         "data": "synthetic (seeded scene and target, random-init Gaussians)",
         "config": {"workload": "train_step_3000k_1920x1080_sh3", "gaussians": scene.N, "visible": st.b.V,
-                   "pairs": st.b.P, "width": cam.w, "height": cam.h, "sh_degree": scene.C - 1,
+                   ("block_pairs" if st.b.shift else "pairs"): st.b.P, "width": cam.w, "height": cam.h,
+                   "sh_degree": scene.C - 1,
                    "ssim_mult": 0.2, "ssim_win": 11},
         "mpix_s": round(cam.h * cam.w / (ms / 1e3) / 1e6, 3),
         "roofline": None,
