@@ -195,7 +195,9 @@ def backward_views(s: ViewsState, mean, qvec, log_s, logit, sh_coeffs, grad_out)
     go = grad_out.contiguous().view(-1)
     if go.numel() != b.n_pix * 3:
         raise RuntimeError(f"grad_out has {go.numel()} values, expected {b.n_pix * 3}")
-    nws = int(L.gsmi_backward_order_workspace_size(b.n_tiles))  # launch order, most expensive tiles first
+    if b.V == 0:  # no view sees a Gaussian: every gradient is zero (no launch takes empty buffers)
+        return torch.zeros_like(mean), torch.zeros_like(qvec), torch.zeros_like(log_s), torch.zeros_like(logit), g_sh
+    nws =int(L.gsmi_backward_order_workspace_size(b.n_tiles))  # launch order, most expensive tiles first
     ows = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
     check(L.gsmi_render_views_backward_sh(b.V, b.P, b.B, ptr(b.views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d),
                                           ptr(sh_coeffs), 3 * cmax2, cmax2, ptr(b.orig), ptr(b.alpha), ptr(b.start),

@@ -165,7 +165,7 @@ def _forward_two_level(b: Binning, sh_coeffs, out, topleft, c2w_d, C, thresh, T,
         raise RuntimeError(f"two-level lists need {cap} entries, beyond the int32 range")
     foff = torch.empty(n_tiles, dtype=torch.int32, device=dev)
     fend = torch.empty(n_tiles, dtype=torch.int32, device=dev)
-    fids = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+    fids = torch.empty(cap, dtype=torch.int32, device=dev)  # empty when nothing is visible (P = 0)
     ows = torch.empty(max(int(L.gsmi_two_level_offsets_workspace_size(n_tiles)), 1), dtype=torch.uint8, device=dev)
     check(L.gsmi_two_level_offsets(nth, ntw, b.shift, ptr(b.block_start), ptr(b.block_end), ptr(foff), ptr(ows),
                                    ows.numel(), st), "two_level_offsets")
@@ -237,6 +237,9 @@ def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g
     g_al = bnd[6 * b.V:]
     if g_sh is None:
         g_sh = torch.zeros_like(sh_coeffs)
+    if b.V == 0:  # nothing visible: every gradient is zero (no launch takes empty buffers)
+        return (torch.zeros_like(mean), torch.zeros_like(qvec), torch.zeros_like(log_s), torch.zeros_like(logit),
+                g_sh, dict(mean2d=g_m2d, cov=g_cov, alpha=g_al))
     cmax2 = sh_coeffs.shape[-1]
     _backend.sh_backward(b.mean2d, b.cov2d, sh_coeffs, b.alpha, b.start, b.end, b.ids, s.out, s.T, s.n_done,
                          s.pre_bg, grad_out.contiguous().view(-1), g_m2d, g_cov, g_sh, g_al, s.topleft, s.c2w_d,

@@ -249,6 +249,23 @@ def test_binning_ties_and_ragged_counts(orc):
         assert np.array_equal(keys.cpu().numpy().view(np.uint64), want["sorted_keys"])
 
 
+@pytest.mark.parametrize("two_level", [False, True])
+def test_empty_view(two_level):
+    """Every Gaussian behind the camera: no visible record and no pair; the
+    image is zero, T one, n_done zero, and every gradient zero."""
+    sc = small_sh_scene(4, n=500)
+    c2w, cam = sc.cameras[0]
+    c = torch.as_tensor(np.asarray(c2w), dtype=torch.float32).reshape(3, 4)
+    sc.mean = (c[:, 3] - 10.0 * c[:, 2]).expand(sc.N, 3) + 0.1 * torch.randn(sc.N, 3)
+    p = dev_params(sc)
+    out, st = rasterize.forward_view(*p, c2w, cam, sc.C, scenes.RasterConfig(two_level=two_level), exact=True)
+    assert st.b.V == 0 and st.b.P == 0
+    assert not out.any() and bool((st.T == 1).all()) and not st.n_done.any()
+    grads = rasterize.backward_view(st, *p, torch.ones_like(out))
+    for g in grads[:5]:
+        assert not g.any()
+
+
 def test_forward_deterministic():
     sc = small_sh_scene(4, n=20000)
     c2w, cam = sc.cameras[0]

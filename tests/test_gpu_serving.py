@@ -103,6 +103,37 @@ def test_admission_splits_batches_that_do_not_fit():
         assert torch.equal(out, reference(p, C, c2w, cam))
 
 
+def _facing_away(c2w):
+    """The camera turned half round about its own y axis (same centre)."""
+    c = np.array(np.asarray(c2w), dtype=np.float32).reshape(3, 4)
+    c[:, 0] *= -1.0
+    c[:, 2] *= -1.0
+    return c
+
+
+def test_empty_view_is_served():
+    """A camera that sees no Gaussian (every record culled, no pair) is
+    served a zero image, alone and in one batch with a camera that sees the
+    scene."""
+    p, C = scene(4, 2000, 6)
+    c2w, cam = camera(1)
+    away = _facing_away(c2w)
+    with serving.RenderServer(arena_bytes=64 << 20, max_views=4, n_slots=1) as srv:
+        sid = srv.add_scene(*p, C)
+        rid, out = srv.submit(sid, away, cam)
+        srv.drain()
+        (done,) = srv.poll()
+        assert done.request_id == rid and done.status == 0
+        assert not out.any()
+        r1, o1 = srv.submit(sid, c2w, cam)
+        r2, o2 = srv.submit(sid, away, cam)
+        srv.drain()
+        res = {x.request_id: x.status for x in srv.poll()}
+    assert res == {r1: 0, r2: 0}
+    assert not o2.any()
+    assert torch.equal(o1, reference(p, C, c2w, cam))
+
+
 def test_request_that_cannot_fit_fails_cleanly():
     p, C = scene(4, 6000, 4)
     with serving.RenderServer(arena_bytes=64 << 10, max_views=4, n_slots=1) as srv:
