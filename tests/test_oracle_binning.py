@@ -68,3 +68,39 @@ def test_negative_depth_sorts_after_positive(orc):
     lo = np.zeros((3, 2), np.int32)
     b = orc.bin_gaussians(lo, lo, np.array([-0.5, 3.0, 0.25], np.float32), 1, 1)
     assert b["ids"].tolist() == [2, 1, 0]
+
+
+@pytest.mark.parametrize("W,H", [(256, 256), (600, 344)])
+def test_two_level_block_lists_filter_to_tile_lists(orc, W, H):
+    """Two-level binning: bounds counted with 64-px tiles are the 16-px bounds
+    shifted right by 2 (so the block pair count is the shifted bounds'
+    area), and filtering each 4 x 4-tile block's depth-ordered list by the
+    records' 16-px bounds gives exactly each tile's own list, ties in depth
+    and ragged block edges (600 x 344: 38 x 22 tiles, 10 x 6 blocks)
+    included."""
+    rng = np.random.default_rng(W)
+    V = 3000
+    cam = orc.Camera(300.0, 300.0, W / 2, H / 2, W, H, 0.01, 5.0)
+    m = np.stack([rng.uniform(-W / 560, W / 560, V), rng.uniform(-H / 560, H / 560, V)], 1).astype(np.float32)
+    sx, sy = np.exp(rng.uniform(-6.5, -3.0, (2, V)))
+    r = rng.uniform(-0.9, 0.9, V)
+    cov = np.stack([sx * sx, r * sx * sy, r * sx * sy, sy * sy], 1).astype(np.float32)
+    depth = rng.uniform(0.1, 5.0, V).astype(np.float32)
+    depth[::7] = 1.0  # ties: order by index
+    P16, tl, br = orc.aabb(m, cov, cam, 6.0, 16)
+    P64, tl64, br64 = orc.aabb(m, cov, cam, 6.0, 64)
+    assert np.array_equal(tl64, tl >> 2) and np.array_equal(br64, br >> 2)
+    assert P64 == int(((br64 - tl64 + 1).prod(1)).sum()) and P64 < P16
+    nth, ntw = -(-H // 16), -(-W // 16)
+    nbh, nbw = -(-nth // 4), -(-ntw // 4)
+    tiles = orc.bin_gaussians(tl, br, depth, nth, ntw)
+    blocks = orc.bin_gaussians(tl >> 2, br >> 2, depth, nbh, nbw)
+    for ty in range(nth):
+        for tx in range(ntw):
+            t, b = ty * ntw + tx, (ty >> 2) * nbw + (tx >> 2)
+            s0, e0 = blocks["start"][b], blocks["end"][b]
+            cand = blocks["ids"][s0:e0] if s0 >= 0 else np.zeros(0, np.int64)
+            hit = (tl[cand, 0] <= tx) & (tx <= br[cand, 0]) & (tl[cand, 1] <= ty) & (ty <= br[cand, 1])
+            s1, e1 = tiles["start"][t], tiles["end"][t]
+            want = tiles["ids"][s1:e1] if s1 >= 0 else np.zeros(0, np.int64)
+            assert np.array_equal(cand[hit], want), (tx, ty)
