@@ -106,7 +106,7 @@ def gpu_bench(scene, steps, warmup, exact=False):
     _backend.sh_forward, _backend.sh_backward, _backend.sh_forward_two_level = fwd_wrap, bwd_wrap, fwd2_wrap
 
     def step():
-        out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg, exact=exact)
+        out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg, exact=exact, prepare_backward=True)
         # mean |out - target| and its gradient sign(out - target) / numel, one fused pass
         _, grad_out = training.photometric_loss(out, target.view_as(out), "l1")
         grads = rasterize.backward_view(st, *params, grad_out)
@@ -490,7 +490,7 @@ def train_main(args):
     dc = densify.DensityControl(scene.N, device=dev)
 
     def step():
-        out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg)
+        out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg, prepare_backward=True)
         loss, grad_out = training.photometric_loss(out.view(cam.h, cam.w, 3), target, "l1", ssim_mult=0.2,
                                                    ssim_win=11)
         g = rasterize.backward_view(st, *params, grad_out.view(-1))

@@ -79,14 +79,20 @@ class Binning:
     block_ids: torch.Tensor | None = None
     block_start: torch.Tensor | None = None
     block_end: torch.Tensor | None = None
+    # camera constants on the device: c2w [3,4], topleft [2] (and bg [3])
+    c2w_d: torch.Tensor | None = None
+    topleft_d: torch.Tensor | None = None
+    bg_d: torch.Tensor | None = None
 
 
 def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, tile_size=TILE, D=6.0,
                        frustum_culling_radius=1.0, skip_frustum_culling=False, want_keys=False,
-                       block_shift=0) -> Binning:
+                       block_shift=0, bg_rgb=None, on_sync=None) -> Binning:
     """Steps 1-5 of the module docstring.  With block_shift > 0 the pairs are
     binned per block of 2^shift x 2^shift tiles instead (P counts block
-    pairs); tl / br stay the tile bounds."""
+    pairs); tl / br stay the tile bounds.  on_sync(V), when given, runs right
+    after the pair count is read back: device work it enqueues runs while the
+    host prepares the emission (the GPU would idle there otherwise)."""
     L = _lib.load()
     dev = mean.device
     st = stream_of(dev)
@@ -113,6 +119,16 @@ def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, til
     V, P = (int(x) for x in totals.cpu().tolist())
     if P >= 2 ** 31:
         raise RuntimeError(f"{P} (Gaussian, tile) pairs exceed the int32 range of the tile lists")
+    # The camera constants go to the device now, while the stream is empty: a
+    # copy from pageable memory returns only when the stream has drained, so
+    # made after the binning launches it would idle the GPU until the host
+    # caught up again.
+    consts = [c2w_h.reshape(-1), np.asarray(ic["topleft"], np.float32)]
+    if bg_rgb is not None:
+        consts.append(np.asarray(bg_rgb, np.float32))
+    consts_d = torch.from_numpy(np.concatenate(consts).astype(np.float32)).to(dev)
+    if on_sync is not None:
+        on_sync(V)
     mean2d = torch.empty(V, 2, dtype=torch.float32, device=dev)
     cov2d = torch.empty(V, 2, 2, dtype=torch.float32, device=dev)
     depth = torch.empty(V, 1, dtype=torch.float32, device=dev)
@@ -141,6 +157,8 @@ def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, til
                                              ptr(end), ptr(keys), ptr(bws), bws.numel(), 0, st), "binning")
     b = Binning(N, V, P, vis, vis_off, pair_off, mean2d, cov2d, depth, alpha, tl, br, orig, keys, ids, start, end,
                 c2w_h, ic)
+    b.c2w_d, b.topleft_d = consts_d[:12], consts_d[12:14]
+    b.bg_d = consts_d[14:17] if bg_rgb is not None else None
     if block_shift:
         b.shift, b.block_ids, b.block_start, b.block_end = block_shift, ids, start, end
     return b
@@ -191,20 +209,31 @@ class ViewState:
     T: torch.Tensor
     n_done: torch.Tensor
     pre_bg: torch.Tensor | None
+    # zeroed gradient buffers made during the forward (prepare_backward);
+    # taken by the first backward_view
+    g_sh: torch.Tensor | None = None
+    bnd: torch.Tensor | None = None
 
 
-def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact=False):
-    """Steps 1-6: returns (image [H*W*3], ViewState)."""
+def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact=False, prepare_backward=False):
+    """Steps 1-6: returns (image [H*W*3], ViewState).  prepare_backward: also
+    make the backward's zeroed gradient buffers, at the point of the forward
+    where the GPU would otherwise wait for the host."""
     two = bool(getattr(cfg, "two_level", False)) and not cfg.bg and cfg.tile_size == TILE
+    early = {}
+
+    def on_sync(V):
+        early["g_sh"] = torch.zeros_like(sh_coeffs)
+        early["bnd"] = torch.zeros(V * 7, dtype=torch.float32, device=mean.device)
+
     b = preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera, tile_size=cfg.tile_size, D=cfg.tile_culling_radius,
                            frustum_culling_radius=cfg.frustum_culling_radius,
-                           skip_frustum_culling=cfg.skip_frustum_culling, block_shift=BLOCK_SHIFT if two else 0)
+                           skip_frustum_culling=cfg.skip_frustum_culling, block_shift=BLOCK_SHIFT if two else 0,
+                           bg_rgb=cfg.bg_rgb if cfg.bg else None, on_sync=on_sync if prepare_backward else None)
     ic = b.consts
     H, W = ic["H"], ic["W"]
     dev = mean.device
-    topleft = torch.tensor(ic["topleft"], dtype=torch.float32, device=dev)
-    c2w_d = torch.from_numpy(b.c2w_host).to(dev)
-    bg = torch.tensor(cfg.bg_rgb, dtype=torch.float32, device=dev) if cfg.bg else None
+    topleft, c2w_d, bg = b.topleft_d, b.c2w_d, b.bg_d
     out = torch.empty(H * W * 3, dtype=torch.float32, device=dev)
     T = torch.empty(H * W, dtype=torch.float32, device=dev)
     nd = torch.empty(H * W, dtype=torch.int32, device=dev)
@@ -217,7 +246,7 @@ def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact
                             cfg.tile_size, ic["n_tiles_h"], ic["n_tiles_w"], ic["pixel_size_x"], ic["pixel_size_y"],
                             H, W, C, cfg.T_thresh, bg_rgb=bg, T_final=T, n_done=nd, pre_bg=pre, write_empty=True,
                             exact=exact, sh_stride_g=3 * cmax2, sh_stride_c=cmax2, sh_index=b.orig)
-    return out, ViewState(b, int(C), cfg, topleft, c2w_d, bg, out, T, nd, pre)
+    return out, ViewState(b, int(C), cfg, topleft, c2w_d, bg, out, T, nd, pre, early.get("g_sh"), early.get("bnd"))
 
 
 def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g_sh=None):
@@ -231,12 +260,15 @@ def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g
     L = _lib.load()
     # one fill for the three boundary gradients; cov first so every part
     # keeps the alignment of a separate allocation (16 / 16 / 8 bytes)
-    bnd = torch.zeros(b.V * 7, dtype=torch.float32, device=dev)
+    bnd, s.bnd = s.bnd, None
+    if bnd is None:
+        bnd = torch.zeros(b.V * 7, dtype=torch.float32, device=dev)
     g_cov = bnd[: 4 * b.V].view(b.V, 2, 2)
     g_m2d = bnd[4 * b.V: 6 * b.V].view(b.V, 2)
     g_al = bnd[6 * b.V:]
+    pre_sh, s.g_sh = s.g_sh, None
     if g_sh is None:
-        g_sh = torch.zeros_like(sh_coeffs)
+        g_sh = pre_sh if pre_sh is not None else torch.zeros_like(sh_coeffs)
     if b.V == 0:  # nothing visible: every gradient is zero (no launch takes empty buffers)
         return (torch.zeros_like(mean), torch.zeros_like(qvec), torch.zeros_like(log_s), torch.zeros_like(logit),
                 g_sh, dict(mean2d=g_m2d, cov=g_cov, alpha=g_al))
@@ -260,7 +292,8 @@ class _RasterizeSH(torch.autograd.Function):
     @staticmethod
     def forward(ctx, mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg):
         mean, qvec, log_s, logit, sh_coeffs = (t.contiguous() for t in (mean, qvec, log_s, logit, sh_coeffs))
-        out, state = forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg)
+        out, state = forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg,
+                                  prepare_backward=any(ctx.needs_input_grad[:5]))
         ctx.state = state
         ctx.save_for_backward(mean, qvec, log_s, logit, sh_coeffs)
         ic = state.b.consts

@@ -266,6 +266,33 @@ def test_empty_view(two_level):
         assert not g.any()
 
 
+def test_prepared_backward_buffers():
+    """forward_view(prepare_backward=True) zeroes the backward's gradient
+    buffers during the forward; the gradients equal those of the plain path
+    (to the float atomics' order), and a second backward on the same state
+    does not accumulate into the first one's buffers."""
+    sc = small_sh_scene(4, n=3000)
+    c2w, cam = sc.cameras[0]
+    p = dev_params(sc)
+    cfg = scenes.RasterConfig()
+    gout = None
+    res = []
+    for prep in (False, True):
+        out, st = rasterize.forward_view(*p, c2w, cam, sc.C, cfg, prepare_backward=prep)
+        assert (st.g_sh is not None) == prep
+        if gout is None:
+            gout = torch.rand(out.shape, generator=torch.Generator().manual_seed(13)).to(DEV) - 0.5
+        res.append([g.cpu() for g in rasterize.backward_view(st, *p, gout)[:5]])
+        if prep:
+            assert st.g_sh is None and st.bnd is None
+            again = [g.cpu() for g in rasterize.backward_view(st, *p, gout)[:5]]
+            for a, b in zip(again, res[1]):
+                assert rel_err(a, b) <= 1e-5
+    for a, b in zip(res[1], res[0]):
+        assert float(b.abs().max()) > 0
+        assert rel_err(a, b) <= 1e-5
+
+
 def test_forward_deterministic():
     sc = small_sh_scene(4, n=20000)
     c2w, cam = sc.cameras[0]
