@@ -290,10 +290,10 @@ def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g
 
 class _RasterizeSH(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg):
+    def forward(ctx, mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, prepare_backward):
         mean, qvec, log_s, logit, sh_coeffs = (t.contiguous() for t in (mean, qvec, log_s, logit, sh_coeffs))
         out, state = forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg,
-                                  prepare_backward=any(ctx.needs_input_grad[:5]))
+                                  prepare_backward=prepare_backward)
         ctx.state = state
         ctx.save_for_backward(mean, qvec, log_s, logit, sh_coeffs)
         ic = state.b.consts
@@ -303,11 +303,13 @@ class _RasterizeSH(torch.autograd.Function):
     def backward(ctx, grad):
         mean, qvec, log_s, logit, sh_coeffs = ctx.saved_tensors
         g_mean, g_q, g_ls, g_lg, g_sh, _ = backward_view(ctx.state, mean, qvec, log_s, logit, sh_coeffs, grad)
-        return g_mean, g_q, g_ls, g_lg, g_sh, None, None, None, None
+        return g_mean, g_q, g_ls, g_lg, g_sh, None, None, None, None, None
 
 
 def rasterize_sh(mean, qvec, svec_before_activation, alpha_before_activation, sh_coeffs, c2w, camera, C, cfg):
     """Image [H,W,3] of one camera; differentiable w.r.t. the five raw
     parameter tensors (SHRenderer storage convention)."""
-    return _RasterizeSH.apply(mean, qvec, svec_before_activation, alpha_before_activation, sh_coeffs, c2w, camera,
-                              int(C), cfg)
+    params = (mean, qvec, svec_before_activation, alpha_before_activation, sh_coeffs)
+    # a backward will follow only with grad mode on and a parameter that needs it
+    prepare = torch.is_grad_enabled() and any(t.requires_grad for t in params)
+    return _RasterizeSH.apply(*params, c2w, camera, int(C), cfg, prepare)

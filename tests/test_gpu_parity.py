@@ -593,6 +593,8 @@ def test_sh_renderer_module_matches_fused_path(orc):
     img.sum().backward()
     for prm in (m.mean, m.qvec, m.svec_before_activation, m.alpha_before_activation, m.sh_coeffs):
         assert prm.grad is not None and torch.isfinite(prm.grad).all()
+    with torch.no_grad():  # inference: no gradient buffers are prepared, same image
+        assert torch.equal(m(c2w, cam), img.detach())
 
 
 @pytest.mark.parametrize("C", [1, 2])
