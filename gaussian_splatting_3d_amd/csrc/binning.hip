@@ -320,8 +320,10 @@ __global__ void k_tile_ranges(const K* __restrict__ keys, int64_t P, int32_t* __
 // Depth keys and identity values for the V visible Gaussians, and their
 // tile bounds packed as one 16-byte record {tl.x, tl.y, br.x, br.y}, so the
 // gather in depth order below touches one cache line per Gaussian.
+// shift > 0: the bounds are binned per block of 2^shift x 2^shift tiles
+// (bounds are >= 0, so >> is the floor division).
 __global__ void k_depth_keys(const float* __restrict__ depth, const int32_t* __restrict__ tl,
-                             const int32_t* __restrict__ br, int64_t V, uint32_t* __restrict__ dkey,
+                             const int32_t* __restrict__ br, int64_t V, int shift, uint32_t* __restrict__ dkey,
                              int32_t* __restrict__ dval, int4* __restrict__ tlbr) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
@@ -329,7 +331,7 @@ __global__ void k_depth_keys(const float* __restrict__ depth, const int32_t* __r
   dval[v] = (int32_t)v;
   const int2 a = reinterpret_cast<const int2*>(tl)[v];
   const int2 b = reinterpret_cast<const int2*>(br)[v];
-  tlbr[v] = make_int4(a.x, a.y, b.x, b.y);
+  tlbr[v] = make_int4(a.x >> shift, a.y >> shift, b.x >> shift, b.y >> shift);
 }
 
 // Pair count and tile rectangle of the Gaussian at depth rank r
@@ -832,7 +834,8 @@ static int bin_impl(int64_t V, int64_t P, const int32_t* tl, const int32_t* br, 
   int rc;
   if (V > 0) {
     const unsigned g = (unsigned)((V + 255) / 256);
-    hipLaunchKernelGGL(k_depth_keys, dim3(g), dim3(256), 0, st, depth, tl, br, V, w.dkey, w.dval, w.tlbr);
+    hipLaunchKernelGGL(k_depth_keys, dim3(g), dim3(256), 0, st, depth, tl, br, V, (flags >> 8) & 7, w.dkey,
+                       w.dval, w.tlbr);
     GSMI_CHECK_LAUNCH();
     rc = radix_sort<uint32_t, 8>(V, w.dkey, w.dval, 32, w.sort_ws, st);
     if (rc) return rc;
@@ -889,6 +892,7 @@ extern "C" int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int3
   if (((uintptr_t)tl & 7) || ((uintptr_t)br & 7)) return GSMI_ERR_ARG;
   if (P > 0 && !gaussian_ids) return GSMI_ERR_ARG;
   if (P >= (int64_t(1) << 31) || V >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
+  if (((flags >> 8) & 7) > 4 || (flags & ~0x7ff)) return GSMI_ERR_ARG;
   const int64_t n_tiles = (int64_t)n_tiles_h * n_tiles_w;
   if (n_tiles > (1 << 24)) return GSMI_ERR_UNSUPPORTED;
   if (!workspace || workspace_bytes < bin_ws_bytes(V, P)) return GSMI_ERR_WORKSPACE;

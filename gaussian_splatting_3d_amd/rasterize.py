@@ -141,11 +141,7 @@ def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, til
                                  ic["H"], float(D), tile_size, ptr(vis), ptr(vis_off), ptr(mean2d), ptr(cov2d),
                                  ptr(depth), ptr(alpha), ptr(tl), ptr(br), ptr(orig), st), "preprocess_emit")
     nth, ntw = ic["n_tiles_h"], ic["n_tiles_w"]
-    rtl, rbr = tl, br
-    if block_shift:
-        rtl = torch.empty_like(tl)
-        rbr = torch.empty_like(br)
-        check(L.gsmi_shift_rects(V, ptr(tl), ptr(br), block_shift, ptr(rtl), ptr(rbr), st), "shift_rects")
+    if block_shift:  # the binning shifts the bounds itself (flags bits 8-10)
         nth = (nth + (1 << block_shift) - 1) >> block_shift
         ntw = (ntw + (1 << block_shift) - 1) >> block_shift
     n_tiles = nth * ntw
@@ -153,8 +149,9 @@ def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, til
     end = torch.empty(n_tiles, dtype=torch.int32, device=dev)
     keys = torch.empty(P, dtype=torch.int64, device=dev) if want_keys else None
     bws = torch.empty(max(int(L.gsmi_binning_workspace_size(V, P, n_tiles)), 1), dtype=torch.uint8, device=dev)
-    check(L.gsmi_tile_culling_aabb_start_end(V, P, ptr(rtl), ptr(rbr), ptr(depth), nth, ntw, ptr(ids), ptr(start),
-                                             ptr(end), ptr(keys), ptr(bws), bws.numel(), 0, st), "binning")
+    check(L.gsmi_tile_culling_aabb_start_end(V, P, ptr(tl), ptr(br), ptr(depth), nth, ntw, ptr(ids), ptr(start),
+                                             ptr(end), ptr(keys), ptr(bws), bws.numel(), block_shift << 8, st),
+          "binning")
     b = Binning(N, V, P, vis, vis_off, pair_off, mean2d, cov2d, depth, alpha, tl, br, orig, keys, ids, start, end,
                 c2w_h, ic)
     b.c2w_d, b.topleft_d = consts_d[:12], consts_d[12:14]

@@ -105,6 +105,10 @@ size_t gsmi_binning_workspace_size(int64_t V, int64_t P, int n_tiles);
  * depth bits, pair expansion in that order, stable sort of the pairs by
  * tile.  flags bit 0: check that P equals the pair count of tl/br before any
  * pair is written (synchronises the stream; GSMI_ERR_COUNT on mismatch).
+ * flags bits 8-10 (no counterpart in the reference): shift s <= 4; the
+ * bounds are binned per block of 2^s x 2^s tiles (tl >> s, br >> s; P, the
+ * grid n_tiles_h x n_tiles_w and start/end are then the blocks'), as the
+ * two-level lists need.  Other bits must be zero.
  */
 int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int32_t *tl, const int32_t *br,
                                      const float *depth, int n_tiles_h, int n_tiles_w, int32_t *gaussian_ids,
@@ -248,7 +252,8 @@ int gsmi_tile_based_vol_rendering_backward_sh_ws(
  * Two-level lists (no counterpart in the reference; single view, no
  * background).  The pairs are binned per block of 2^shift x 2^shift tiles
  * (gsmi_preprocess_count with tile_size 16 << shift gives their count, then
- * gsmi_shift_rects and gsmi_tile_culling_aabb_start_end on the block grid),
+ * gsmi_tile_culling_aabb_start_end on the block grid with the shift in its
+ * flags, or over bounds shifted by gsmi_shift_rects),
  * and the forward filters each tile's list from its block's depth-ordered
  * list on the fly: the filtered list equals the tile's own list, so images,
  * T_final and n_done are those of gsmi_tile_based_vol_rendering_sh.  The
