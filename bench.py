@@ -8,10 +8,12 @@ between raw Gaussian parameters and their gradients.  Inputs are resident in
 HBM before timing starts.
 
 Prints one JSON line (metric/value/unit per the workload definition) with a
-`roofline` object for the dominant kernel (algorithmic bytes / HIP-event time
-of that kernel on the stream it runs on, against 8 TB/s) and a
-`cpu_baseline` object (the CPU oracle on a bounded sample of the same
-workload, scaled, single thread).
+`roofline` object for the dominant kernel (algorithmic bytes of the
+(Gaussian, tile) pairs the launch processes / HIP-event time of that kernel
+on the stream it runs on, against 8 TB/s; beside it the DRAM bytes the
+hardware counters measure and the VALU issue rate) and a `cpu_baseline`
+object (the CPU oracle on all host cores, on a stratified 5% tile sample of
+the same workload, scaled, median of 3).
 """
 from __future__ import annotations
 
@@ -71,7 +73,17 @@ class Timed:
         return float(np.mean([s.elapsed_time(e) for s, e in self.pairs])) if self.pairs else float("nan")
 
 
-def gpu_bench(scene, steps, warmup, exact=False):
+def examined_pairs(n_done, H, W, tile=16):
+    """(Gaussian, tile) pairs a compositing launch processes: per tile, the
+    list entries up to where its last pixel saturated (the largest n_done of
+    its pixels), summed over the tiles."""
+    nth, ntw = -(-H // tile), -(-W // tile)
+    nd = torch.zeros(nth * tile, ntw * tile, dtype=torch.int64, device=n_done.device)
+    nd[:H, :W] = n_done.view(H, W).to(torch.int64)
+    return int(nd.view(nth, tile, ntw, tile).amax(dim=(1, 3)).sum())
+
+
+def gpu_bench(scene, steps, warmup, exact=False, split=False):
     from gaussian_splatting_3d_amd import _lib, rasterize, scenes, training
     from gaussian_splatting_3d_amd.backend import _backend
 
@@ -109,7 +121,7 @@ def gpu_bench(scene, steps, warmup, exact=False):
         out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg, exact=exact, prepare_backward=True)
         # mean |out - target| and its gradient sign(out - target) / numel, one fused pass
         _, grad_out = training.photometric_loss(out, target.view_as(out), "l1")
-        grads = rasterize.backward_view(st, *params, grad_out)
+        grads = rasterize.backward_view(st, *params, grad_out, split_bf16=split)
         return st, grads
 
     try:
@@ -141,7 +153,8 @@ def gpu_bench(scene, steps, warmup, exact=False):
     P = rasterize.preprocess_and_bin(*params[:4], c2w, cam, D=cfg.tile_culling_radius).P if b.shift else b.P
     return dict(ms_step=(t1 - t0) * 1e3 / steps, ms_fwd_only=(f1 - f0) * 1e3 / steps,
                 ms_composite_fwd=t_fwd.mean_ms(), ms_composite_bwd=t_bwd.mean_ms(), N=b.N, V=b.V, P=P,
-                block_pairs=b.P if b.shift else None, H=cam.h, W=cam.w, C=scene.C)
+                P_exam=examined_pairs(st.n_done, cam.h, cam.w), block_pairs=b.P if b.shift else None, H=cam.h,
+                W=cam.w, C=scene.C)
 
 
 def gpu_bench_views(scene, steps, warmup):
@@ -196,79 +209,104 @@ def gpu_bench_views(scene, steps, warmup):
     for gr in grads:
         assert torch.isfinite(gr).all()
     b = st.b
+    P_exam = sum(examined_pairs(st.n_done[m["pix_base"]:m["pix_base"] + m["H"] * m["W"]], m["H"], m["W"])
+                 for m in b.meta)
     return dict(ms_step=(t1 - t0) * 1e3 / steps, ms_composite_fwd=t_fwd.mean_ms(), ms_composite_bwd=t_bwd.mean_ms(),
-                N=b.N, V=b.V, P=b.P, n_pix=n_pix, views=b.B, C=scene.C,
+                N=b.N, V=b.V, P=b.P, P_exam=P_exam, n_pix=n_pix, views=b.B, C=scene.C,
                 H=scene.cameras[0][1].h, W=scene.cameras[0][1].w)
 
 
-def cpu_baseline(scene, budget_s=20.0):
-    """CPU oracle on a bounded sample of the same workload, single thread:
+def host_threads():
+    """CPU threads this process may use: OMP_NUM_THREADS when set (the share
+    of the host a GPU job gets), else the CPUs it is allowed to run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, reps=3, tile_frac=0.05):
+    """The CPU oracle on all host threads, on the same seeded workload:
     full preprocessing (activation, cull, projection, tile bounds) and
-    binning (key emission + stable sort + ranges), compositing forward and
-    backward on a seeded sample of tiles (scaled by total/sample pair count),
-    projection adjoint on a sample of Gaussians (scaled by V)."""
+    binning (key emission + stable sort + ranges); compositing forward and
+    backward on a stratified sample of tile_frac of the non-empty tiles
+    (every k-th tile in order of list length, so short and long lists are
+    both represented), scaled by non-empty / sampled tiles; projection
+    adjoint on a sample of the visible Gaussians, scaled by V.  Median of
+    `reps` repetitions of the extrapolated step time."""
     from oracle import oracle as O
 
-    torch.set_num_threads(1)
+    threads = host_threads()
+    torch.set_num_threads(threads)
+    O.set_threads(threads)
     c2w, cam = scene.cameras[0]
     ocam = O.Camera(cam.fx, cam.fy, cam.cx, cam.cy, cam.w, cam.h, cam.near_plane, cam.far_plane)
     C = scene.C
-    t0 = time.perf_counter()
-    s, a = O.activate(scene.svec_before_activation, scene.alpha_before_activation)
-    normals, pts = O.frustum(c2w, ocam)
-    mask = O.cull(scene.mean, s, normals, pts, 1.0)
-    idx = np.nonzero(mask)[0]
     mean = scene.mean.numpy()
-    m2d, cov, depth = O.project(mean[idx], scene.qvec.numpy()[idx], s[idx], c2w)
-    P, tl, br = O.aabb(m2d, cov, ocam, 6.0)
-    ic = O.image_constants(ocam)
-    bn = O.bin_gaussians(tl, br, depth, ic["n_tiles_h"], ic["n_tiles_w"])
-    t_pre = time.perf_counter() - t0
-    # compositing on a tile sample
-    rng = np.random.default_rng(0)
-    n_tiles = ic["n_tiles_h"] * ic["n_tiles_w"]
-    counts = np.where(bn["start"] >= 0, bn["end"] - bn["start"], 0).astype(np.int64)
-    order = rng.permutation(n_tiles)
-    sh_v = np.ascontiguousarray(scene.sh_coeffs.numpy()[idx][..., : C * C])
-    al = a[idx]
-    frac = 0.0
-    t_comp = 0.0
-    sampled_pairs = 0
-    chosen = []
-    for k in range(1, n_tiles + 1):
-        chosen = order[:k]
-        if counts[chosen].sum() >= max(1, 0.004 * P) or k == n_tiles:
-            break
-    st = -np.ones(n_tiles, np.int32)
-    en = -np.ones(n_tiles, np.int32)
-    st[chosen] = bn["start"][chosen]
-    en[chosen] = bn["end"][chosen]
-    sampled_pairs = int(counts[chosen].sum())
-    c0 = time.perf_counter()
-    out, T, nd, _ = O.render_sh_fwd(m2d, cov, sh_v, al, st, en, bn["ids"], ic["topleft"], c2w, 16,
-                                    ic["n_tiles_h"], ic["n_tiles_w"], ic["psx"], ic["psy"], ic["H"], ic["W"], C,
-                                    1e-4)
-    gout = np.sign(out - 0.5).astype(np.float32) / out.size
-    O.render_sh_bwd(m2d, cov, sh_v, al, st, en, bn["ids"], out, gout, ic["topleft"], c2w, 16, ic["n_tiles_h"],
-                    ic["n_tiles_w"], ic["psx"], ic["psy"], ic["H"], ic["W"], C, 1e-4)
-    t_comp = time.perf_counter() - c0
-    frac = sampled_pairs / max(P, 1)
-    # projection adjoint on a Gaussian sample
-    ns = min(len(idx), 20000)
-    sub = idx[:ns]
-    a0 = time.perf_counter()
-    O.projection_adjoint(mean[sub], scene.qvec.numpy()[sub], scene.svec_before_activation.numpy()[sub],
-                         scene.alpha_before_activation.numpy()[sub], scene.sh_coeffs.numpy()[sub], c2w,
-                         np.arange(ns), np.zeros((ns, 2), np.float32), np.zeros((ns, 4), np.float32),
-                         np.zeros((ns, 3, C * C), np.float32), np.zeros(ns, np.float32), C)
-    t_adj = (time.perf_counter() - a0) * len(idx) / max(ns, 1)
-    t_step = t_pre + t_comp / max(frac, 1e-12) + t_adj
-    return dict(value=ic["H"] * ic["W"] / t_step / 1e6, unit="Mpix/s", cores=1, kind="port",
-                sample=(f"CPU oracle, 1 thread: full preprocess+binning of N={scene.N} ({t_pre:.1f}s); "
-                        f"fwd+bwd compositing on {len(chosen)} of {n_tiles} tiles ({sampled_pairs} of {P} pairs, "
-                        f"{t_comp:.1f}s, scaled x{1 / max(frac, 1e-12):.0f}); projection adjoint on {ns} of "
-                        f"{len(idx)} visible Gaussians, scaled; extrapolated step {t_step:.1f}s"),
-                seconds_measured=round(t_pre + t_comp + (time.perf_counter() - a0), 1))
+    steps = []
+    for rep in range(reps):
+        t0 = time.perf_counter()
+        s, a = O.activate(scene.svec_before_activation, scene.alpha_before_activation)
+        normals, pts = O.frustum(c2w, ocam)
+        mask = O.cull(scene.mean, s, normals, pts, 1.0)
+        idx = np.nonzero(mask)[0]
+        m2d, cov, depth = O.project(mean[idx], scene.qvec.numpy()[idx], s[idx], c2w)
+        P, tl, br = O.aabb(m2d, cov, ocam, 6.0)
+        ic = O.image_constants(ocam)
+        bn = O.bin_gaussians(tl, br, depth, ic["n_tiles_h"], ic["n_tiles_w"])
+        t_pre = time.perf_counter() - t0
+        n_tiles = ic["n_tiles_h"] * ic["n_tiles_w"]
+        counts = np.where(bn["start"] >= 0, bn["end"] - bn["start"], 0).astype(np.int64)
+        nonempty = np.nonzero(counts)[0]
+        by_len = nonempty[np.argsort(counts[nonempty], kind="stable")]
+        k = max(1, int(round(1.0 / tile_frac)))
+        chosen = by_len[k // 2::k]
+        st = -np.ones(n_tiles, np.int32)
+        en = -np.ones(n_tiles, np.int32)
+        st[chosen] = bn["start"][chosen]
+        en[chosen] = bn["end"][chosen]
+        sampled_pairs = int(counts[chosen].sum())
+        sh_v = np.ascontiguousarray(scene.sh_coeffs.numpy()[idx][..., : C * C])
+        al = a[idx]
+        c0 = time.perf_counter()
+        out, T, nd, _ = O.render_sh_fwd(m2d, cov, sh_v, al, st, en, bn["ids"], ic["topleft"], c2w, 16,
+                                        ic["n_tiles_h"], ic["n_tiles_w"], ic["psx"], ic["psy"], ic["H"], ic["W"], C,
+                                        1e-4)
+        gout = np.sign(out - 0.5).astype(np.float32) / out.size
+        O.render_sh_bwd(m2d, cov, sh_v, al, st, en, bn["ids"], out, gout, ic["topleft"], c2w, 16, ic["n_tiles_h"],
+                        ic["n_tiles_w"], ic["psx"], ic["psy"], ic["H"], ic["W"], C, 1e-4)
+        t_comp = time.perf_counter() - c0
+        scale = nonempty.size / max(chosen.size, 1)
+        ns = min(len(idx), 100_000)
+        sub = idx[:ns]
+        a0 = time.perf_counter()
+        O.projection_adjoint(mean[sub], scene.qvec.numpy()[sub], scene.svec_before_activation.numpy()[sub],
+                             scene.alpha_before_activation.numpy()[sub], scene.sh_coeffs.numpy()[sub], c2w,
+                             np.arange(ns), np.zeros((ns, 2), np.float32), np.zeros((ns, 4), np.float32),
+                             np.zeros((ns, 3, C * C), np.float32), np.zeros(ns, np.float32), C)
+        t_adj = (time.perf_counter() - a0) * len(idx) / max(ns, 1)
+        steps.append((t_pre + t_comp * scale + t_adj, t_pre, t_comp, t_adj))
+        print(f"[bench] cpu baseline rep {rep}: step {steps[-1][0]:.1f}s (preprocess+binning {t_pre:.1f}s, "
+              f"compositing sample {t_comp:.2f}s x{scale:.1f}, adjoint {t_adj:.1f}s)", file=sys.stderr, flush=True)
+    med = sorted(steps)[len(steps) // 2]
+    t_step = med[0]
+    return dict(value=ic["H"] * ic["W"] / t_step / 1e6, unit="Mpix/s", cores=threads, kind="port",
+                cpu_model=cpu_model(), step_s=round(t_step, 2),
+                sample=(f"CPU oracle on {threads} threads ({cpu_model()}), median of {reps}: full "
+                        f"preprocess+binning of N={scene.N} ({med[1]:.1f}s); fwd+bwd compositing on {chosen.size} "
+                        f"of {nonempty.size} non-empty tiles (every {k}th by list length, {sampled_pairs} of {P} "
+                        f"pairs, {med[2]:.2f}s, scaled x{scale:.1f}); projection adjoint on {ns} of {len(idx)} "
+                        f"visible Gaussians ({med[3]:.1f}s scaled); extrapolated step {t_step:.1f}s"))
 
 
 KERNEL_PREFIX = {"composite_fwd": "gsmi::k_fwd_sh", "composite_bwd": "gsmi::k_bwd_sh"}
@@ -315,11 +353,15 @@ def _pmc_pass(counter, child_args, timeout_s=90):
 VALU_PEAK_INSTR_S = 256 * 4 * 2.4e9 / 4
 
 
-def measure_valu(workload, exact, kernel, kernel_ms):
+def _child_args(workload, exact, split):
+    return ["--pmc-child", "--workload", workload] + (["--exact"] if exact else []) + (["--bwd-split"] if split else [])
+
+
+def measure_valu(workload, exact, kernel, kernel_ms, split=False):
     """Wave-level VALU instructions per launch of `kernel` (rocprofv3
     SQ_INSTS_VALU, summed over the chip) and their issue rate over the
     kernel's HIP-event time, against the non-packed issue peak."""
-    child = ["--pmc-child", "--workload", workload] + (["--exact"] if exact else [])
+    child = _child_args(workload, exact, split)
     run = _pmc_pass("SQ_INSTS_VALU", child)
     vals = [v for k, (v, n) in run.items() if k.split("void ", 1)[-1].startswith(KERNEL_PREFIX[kernel])]
     if not vals:
@@ -329,13 +371,13 @@ def measure_valu(workload, exact, kernel, kernel_ms):
                 peak_g_instr_s=round(VALU_PEAK_INSTR_S / 1e9, 1), frac=round(rate / VALU_PEAK_INSTR_S, 4))
 
 
-def measure_traffic(workload, exact, kernel):
+def measure_traffic(workload, exact, kernel, split=False):
     """HBM bytes per launch of `kernel` from rocprofv3 FETCH_SIZE and
     WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
     for a 1 GiB device-to-device copy (bytes moved / counter KiB)."""
     copy_bytes = float(1 << 30)
     res = {}
-    child = ["--pmc-child", "--workload", workload] + (["--exact"] if exact else [])
+    child = _child_args(workload, exact, split)
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         cal = _pmc_pass(counter, ["--copy-calibration"])
         copy_kib = [v for k, (v, n) in cal.items() if "copyBuffer" in k]
@@ -361,6 +403,9 @@ def main():
     ap.add_argument("--workload", default="dense_3m", choices=WORKLOADS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact", action="store_true", help="exact colour arithmetic (bit-reproducible images)")
+    ap.add_argument("--bwd-split", action="store_true",
+                    help="opt-in: SH-gradient products on bf16 matrix cores as a three-term split (not fp32; the "
+                         "line's dtype says so)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (scales rocprof byte counters)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -389,7 +434,7 @@ def main():
         if len(sc.cameras) > 1:
             gpu_bench_views(sc, 1, 1)
         else:
-            gpu_bench(sc, 1, 1, exact=args.exact)
+            gpu_bench(sc, 1, 1, exact=args.exact, split=args.bwd_split)
         return
     if args.workload == "serve":
         serve_main(args)
@@ -403,10 +448,12 @@ def main():
     if views > 1:
         r = gpu_bench_views(scene, args.steps, args.warmup)
     else:
-        r = gpu_bench(scene, args.steps, args.warmup, exact=args.exact)
+        r = gpu_bench(scene, args.steps, args.warmup, exact=args.exact, split=args.bwd_split)
         r["n_pix"], r["views"] = r["H"] * r["W"], 1
     print(f"[bench] gpu done: {r}", file=sys.stderr, flush=True)
-    ab = alg_bytes(r["P"], r["n_pix"], r["C"], r["V"], r["N"])
+    # compositing bytes are counted over the pairs the launches process (the
+    # list entries up to each tile's saturation), not over every listed pair
+    ab = alg_bytes(r["P_exam"], r["n_pix"], r["C"], r["V"], r["N"])
     dom = "composite_bwd" if r["ms_composite_bwd"] >= r["ms_composite_fwd"] else "composite_fwd"
     t_dom = r["ms_" + dom] / 1e3
     achieved = ab[dom] / t_dom / 1e9
@@ -425,7 +472,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+bf16x3-split" if args.bwd_split else "f32",
         "data": "synthetic (seeded scene and target, random-init Gaussians)",
         "config": {"workload": scene.name, "gaussians": r["N"], "visible": r["V"], "pairs": r["P"],
                    "width": r["W"], "height": r["H"], "sh_degree": r["C"] - 1, "views": r["views"]},
@@ -434,7 +481,7 @@ def main():
                        "composite_bwd": round(r["ms_composite_bwd"], 4)},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
-                     "alg_bytes_per_launch": int(ab[dom])},
+                     "alg_bytes_per_launch": int(ab[dom]), "pairs_processed": r["P_exam"], "pairs_listed": r["P"]},
     }
     if r.get("block_pairs") is not None:
         line["config"]["block_pairs"] = r["block_pairs"]  # (Gaussian, 64-px block) pairs, what is sorted
@@ -442,16 +489,20 @@ def main():
         line["fwd_only_mpix_s"] = round(r["n_pix"] / (r["ms_fwd_only"] / 1e3) / 1e6, 3)
     if not args.no_traffic:
         try:
-            tr = measure_traffic(args.workload, args.exact, dom)
+            tr = measure_traffic(args.workload, args.exact, dom, args.bwd_split)
             line["roofline"]["traffic"] = int(tr["bytes"])
             line["roofline"]["traffic_detail"] = {k: (round(v, 3) if isinstance(v, float) else v)
                                                   for k, v in tr.items()}
+            # the DRAM bytes the counters measured, over the same launch time
+            line["roofline"]["dram_achieved"] = round(tr["bytes"] / t_dom / 1e9, 2)
+            line["roofline"]["dram_frac"] = round(tr["bytes"] / t_dom / HBM_PEAK, 4)
         except Exception as e:  # counters unavailable: traffic stays null, say why
             line["roofline"]["traffic_error"] = str(e)
         print(f"[bench] traffic: {line['roofline'].get('traffic')}", file=sys.stderr, flush=True)
         # the compositing kernels are VALU-issue bound, not HBM bound: report that rate too
         try:
-            line["roofline"]["valu_issue"] = measure_valu(args.workload, args.exact, dom, line["kernels_ms"][dom])
+            line["roofline"]["valu_issue"] = measure_valu(args.workload, args.exact, dom, line["kernels_ms"][dom],
+                                                          args.bwd_split)
         except Exception as e:
             line["roofline"]["valu_issue"] = {"error": str(e)}
         print(f"[bench] valu: {line['roofline']['valu_issue']}", file=sys.stderr, flush=True)
@@ -464,7 +515,8 @@ def main():
         line["serving"] = sv
     if not args.no_cpu_baseline and views == 1:
         cb = cpu_baseline(scene)
-        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "cpu_model",
+                                                   "step_s")}
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -45,26 +45,38 @@ SIGNATURES = {
     "gsmi_preprocess_emit": (_I32, [_I64, _P, _P, _P, _P, _P, _F, _F, _F, _F, _I32, _I32, _F, _I32, _P, _P, _P,
                                     _P, _P, _P, _P, _P, _P, _P]),
     "gsmi_tile_ranges": (_I32, [_I64, _P, _I32, _P, _P, _P]),
-    "gsmi_tile_based_vol_rendering_sh": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P,
-                                                _P, _I32, _I32, _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P, _P, _P,
-                                                _I32, _I32, _P]),
-    "gsmi_tile_based_vol_rendering_backward_sh": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P,
-                                                         _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32,
-                                                         _F, _F, _I32, _I32, _I32, _F, _P, _P]),
+    # the reference's own compositing signatures (gs/src/render.h:70-133)
+    "gsmi_tile_based_vol_rendering_sh": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32,
+                                                _F, _F, _I32, _I32, _I32, _F, _P]),
+    "gsmi_tile_based_vol_rendering_backward_sh": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                                         _P, _P, _P, _I32, _I32, _I32, _F, _F, _I32, _I32, _I32, _F,
+                                                         _P]),
+    "gsmi_tile_based_vol_rendering_sh_with_bg": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32,
+                                                        _I32, _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P]),
+    "gsmi_tile_based_vol_rendering_backward_sh_with_bg": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                                                 _P, _P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _I32,
+                                                                 _I32, _I32, _F, _P, _P]),
+    "gsmi_tile_based_vol_rendering_start_end": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
+                                                       _I32, _F, _F, _I32, _I32, _F, _P]),
+    "gsmi_tile_based_vol_rendering_backward_start_end": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                                                _P, _P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _I32,
+                                                                _I32, _F, _P]),
+    # extended compositing entries
+    "gsmi_render_sh_forward": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I32,
+                                      _I32, _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P, _P, _P, _I32, _I32, _P]),
     "gsmi_backward_order_workspace_size": (_SZ, [_I32]),
-    "gsmi_tile_based_vol_rendering_backward_sh_ws": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P,
-                                                            _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
-                                                            _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P, _SZ, _P]),
+    "gsmi_render_sh_backward": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                       _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _I32, _I32, _I32, _F, _P,
+                                       _I32, _P, _SZ, _P]),
+    "gsmi_render_rgb_forward": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _F, _F,
+                                       _I32, _I32, _F, _P, _P, _I32, _P]),
+    "gsmi_render_rgb_backward": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                        _P, _I32, _I32, _I32, _F, _F, _I32, _I32, _F, _P]),
     "gsmi_shift_rects": (_I32, [_I64, _P, _P, _I32, _P, _P, _P]),
     "gsmi_two_level_offsets_workspace_size": (_SZ, [_I32]),
     "gsmi_two_level_offsets": (_I32, [_I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
     "gsmi_render_sh_two_level": (_I32, [_I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P,
                                         _P, _P, _P, _I32, _I32, _F, _F, _I32, _I32, _I32, _F, _P, _P, _I32, _P]),
-    "gsmi_tile_based_vol_rendering_start_end": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32,
-                                                       _I32, _F, _F, _I32, _I32, _F, _P, _P, _I32, _P]),
-    "gsmi_tile_based_vol_rendering_backward_start_end": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                                                _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _F,
-                                                                _F, _I32, _I32, _F, _P]),
     "gsmi_project_backward": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gsmi_preprocess_views_workspace_size": (_SZ, [_I64, _I32]),
     "gsmi_preprocess_views_count": (_I32, [_I64, _I32, _P, _P, _P, _P, _F, _F, _I32, _P, _P, _P, _P, _SZ, _P]),
@@ -78,7 +90,8 @@ SIGNATURES = {
     "gsmi_render_views_sh": (_I32, [_I64, _I64, _I32, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P,
                                     _I32, _F, _P, _P, _P, _P, _I32, _I32, _P]),
     "gsmi_render_views_backward_sh": (_I32, [_I64, _I64, _I32, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P, _P,
-                                             _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _P, _P, _SZ, _P]),
+                                             _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _P, _I32, _P, _SZ,
+                                             _P]),
     "gsmi_project_backward_views": (_I32, [_I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                            _P]),
     "gsmi_loss_workspace_size": (_SZ, [_I64]),

@@ -15,6 +15,7 @@ from . import _lib
 from ._lib import check, ptr, stream_of
 
 TILE = 16
+GSMI_BWD_SPLIT_BF16 = 1  # include/gsmi.h
 
 
 def _check_cuda(name, t):
@@ -109,12 +110,12 @@ class _Backend:
         C = int(C)
         sg = 3 * C * C if sh_stride_g is None else int(sh_stride_g)
         sc = C * C if sh_stride_c is None else int(sh_stride_c)
-        check(L.gsmi_tile_based_vol_rendering_sh(
+        check(L.gsmi_render_sh_forward(
             mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(sh_coeffs), sg, sc, ptr(sh_index),
             ptr(alpha), ptr(start), ptr(end), ptr(gaussian_ids), ptr(out), ptr(topleft), ptr(c2w), int(tile_size),
             int(n_tiles_h), int(n_tiles_w), float(pixel_size_x), float(pixel_size_y), int(H), int(W), C,
             float(thresh), ptr(bg_rgb), ptr(T_final), ptr(n_done), ptr(pre_bg), 1 if write_empty else 0,
-            1 if exact else 0, stream_of(out.device)), "tile_based_vol_rendering_sh")
+            1 if exact else 0, stream_of(out.device)), "render_sh_forward")
 
     def sh_forward_two_level(self, mean, cov, sh_coeffs, alpha, tl, br, block_start, block_end, block_ids, shift,
                              tile_off, tile_ids, tile_end, out, topleft, c2w, n_tiles_h, n_tiles_w, pixel_size_x,
@@ -137,23 +138,27 @@ class _Backend:
     def sh_backward(self, mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, T_final, n_done, pre_bg,
                     grad_out, grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, topleft, c2w, tile_size, n_tiles_h,
                     n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb=None, sh_stride_g=None,
-                    sh_stride_c=None, sh_index=None):
-        """Reverse-order adjoint from saved per-pixel state (accumulates).
-        The tiles are launched most expensive first (a small workspace holds
-        the order)."""
+                    sh_stride_c=None, sh_index=None, split_bf16=False):
+        """Reverse-order adjoint (accumulates).  From the saved per-pixel
+        state, with the tiles launched most expensive first (a small
+        workspace holds the order); with T_final = n_done = None the launch
+        recomputes that state itself.  split_bf16: the SH-gradient products
+        as a three-term bf16 split (opt-in, not fp32)."""
         L = _lib.load()
         C = int(C)
         sg = 3 * C * C if sh_stride_g is None else int(sh_stride_g)
         sc = C * C if sh_stride_c is None else int(sh_stride_c)
-        nws = int(L.gsmi_backward_order_workspace_size(int(n_tiles_h) * int(n_tiles_w)))
-        ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=grad_out.device)
-        check(L.gsmi_tile_based_vol_rendering_backward_sh_ws(
+        ws, nws = None, 0
+        if n_done is not None:
+            nws = int(L.gsmi_backward_order_workspace_size(int(n_tiles_h) * int(n_tiles_w)))
+            ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=grad_out.device)
+        check(L.gsmi_render_sh_backward(
             mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(sh_coeffs), sg, sc, ptr(sh_index),
             ptr(alpha), ptr(start), ptr(end), ptr(gaussian_ids), ptr(out), ptr(T_final), ptr(n_done), ptr(pre_bg),
             ptr(grad_out), ptr(grad_mean), ptr(grad_cov), ptr(grad_sh_coeffs), ptr(grad_alpha), ptr(topleft),
             ptr(c2w), int(tile_size), int(n_tiles_h), int(n_tiles_w), float(pixel_size_x), float(pixel_size_y),
-            int(H), int(W), C, float(thresh), ptr(bg_rgb), ptr(ws), nws, stream_of(grad_out.device)),
-            "tile_based_vol_rendering_backward_sh")
+            int(H), int(W), C, float(thresh), ptr(bg_rgb), GSMI_BWD_SPLIT_BF16 if split_bf16 else 0, ptr(ws), nws,
+            stream_of(grad_out.device)), "render_sh_backward")
 
     # gs/src/render.cu:483-544
     def tile_based_vol_rendering_sh(self, mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, topleft,
@@ -162,20 +167,16 @@ class _Backend:
         self._sh_checks(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, topleft, c2w)
         if not 1 <= int(C) <= 4:
             return  # the reference's switch has no default case (render.cu:506-543)
-        self.sh_forward(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, topleft, c2w, tile_size,
-                        n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh)
+        check(_lib.load().gsmi_tile_based_vol_rendering_sh(
+            mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(sh_coeffs), ptr(alpha), ptr(start),
+            ptr(end), ptr(gaussian_ids), ptr(out), ptr(topleft), ptr(c2w), int(tile_size), int(n_tiles_h),
+            int(n_tiles_w), float(pixel_size_x), float(pixel_size_y), int(H), int(W), int(C), float(thresh),
+            stream_of(out.device)), "tile_based_vol_rendering_sh")
 
-    def _state_for_backward(self, mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, topleft, c2w, tile_size,
-                            n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb=None):
-        dev = mean.device
-        scratch = torch.zeros(int(H) * int(W) * 3, dtype=torch.float32, device=dev)
-        T = torch.empty(int(H) * int(W), dtype=torch.float32, device=dev)
-        nd = torch.empty(int(H) * int(W), dtype=torch.int32, device=dev)
-        pre = torch.empty_like(scratch) if bg_rgb is not None else None
-        self.sh_forward(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, scratch, topleft, c2w, tile_size,
-                        n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb=bg_rgb,
-                        T_final=T, n_done=nd, pre_bg=pre, write_empty=True)
-        return T, nd, pre
+    def _grad_checks(self, grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, grad_out, sh_name="grad_sh_coeffs"):
+        for n, t in (("grad_mean", grad_mean), ("grad_cov", grad_cov), (sh_name, grad_sh_coeffs),
+                     ("grad_alpha", grad_alpha), ("grad_out", grad_out)):
+            check_float(n, t)
 
     # gs/src/render.cu:546-624
     def tile_based_vol_rendering_backward_sh(self, mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out,
@@ -183,17 +184,15 @@ class _Backend:
                                              c2w, tile_size, n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H,
                                              W, C, thresh):
         self._sh_checks(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, topleft, c2w)
-        for n, t in (("grad_mean", grad_mean), ("grad_cov", grad_cov), ("grad_sh_coeffs", grad_sh_coeffs),
-                     ("grad_alpha", grad_alpha), ("grad_out", grad_out)):
-            check_float(n, t)
+        self._grad_checks(grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, grad_out)
         if not 1 <= int(C) <= 4:
             return
-        T, nd, _ = self._state_for_backward(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, topleft, c2w,
-                                            tile_size, n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C,
-                                            thresh)
-        self.sh_backward(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, T, nd, None, grad_out,
-                         grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, topleft, c2w, tile_size, n_tiles_h,
-                         n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh)
+        check(_lib.load().gsmi_tile_based_vol_rendering_backward_sh(
+            mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(sh_coeffs), ptr(alpha), ptr(start),
+            ptr(end), ptr(gaussian_ids), ptr(out), ptr(grad_mean), ptr(grad_cov), ptr(grad_sh_coeffs),
+            ptr(grad_alpha), ptr(grad_out), ptr(topleft), ptr(c2w), int(tile_size), int(n_tiles_h), int(n_tiles_w),
+            float(pixel_size_x), float(pixel_size_y), int(H), int(W), int(C), float(thresh),
+            stream_of(grad_out.device)), "tile_based_vol_rendering_backward_sh")
 
     # gs/src/render.cu:780-842 (vol_render_bg.h:12-100)
     def tile_based_vol_rendering_sh_with_bg(self, mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out,
@@ -203,8 +202,11 @@ class _Backend:
         check_float("bg_rgb", bg_rgb)
         if not 1 <= int(C) <= 4:
             return
-        self.sh_forward(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, topleft, c2w, tile_size,
-                        n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb=bg_rgb)
+        check(_lib.load().gsmi_tile_based_vol_rendering_sh_with_bg(
+            mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(sh_coeffs), ptr(alpha), ptr(start),
+            ptr(end), ptr(gaussian_ids), ptr(out), ptr(topleft), ptr(c2w), int(tile_size), int(n_tiles_h),
+            int(n_tiles_w), float(pixel_size_x), float(pixel_size_y), int(H), int(W), int(C), float(thresh),
+            ptr(bg_rgb), stream_of(out.device)), "tile_based_vol_rendering_sh_with_bg")
 
     # gs/src/render.cu:844-925 (vol_render_bg.h:121-234)
     def tile_based_vol_rendering_backward_sh_with_bg(self, mean, cov, sh_coeffs, alpha, start, end, gaussian_ids,
@@ -212,37 +214,40 @@ class _Backend:
                                                      topleft, c2w, tile_size, n_tiles_h, n_tiles_w, pixel_size_x,
                                                      pixel_size_y, H, W, C, thresh, bg_rgb):
         self._sh_checks(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, topleft, c2w)
+        self._grad_checks(grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, grad_out)
         check_float("bg_rgb", bg_rgb)
         if not 1 <= int(C) <= 4:
             return
-        T, nd, pre = self._state_for_backward(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, topleft,
-                                              c2w, tile_size, n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H,
-                                              W, C, thresh, bg_rgb=bg_rgb)
-        self.sh_backward(mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, T, nd, pre, grad_out,
-                         grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, topleft, c2w, tile_size, n_tiles_h,
-                         n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb=bg_rgb)
+        check(_lib.load().gsmi_tile_based_vol_rendering_backward_sh_with_bg(
+            mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(sh_coeffs), ptr(alpha), ptr(start),
+            ptr(end), ptr(gaussian_ids), ptr(out), ptr(grad_mean), ptr(grad_cov), ptr(grad_sh_coeffs),
+            ptr(grad_alpha), ptr(grad_out), ptr(topleft), ptr(c2w), int(tile_size), int(n_tiles_h), int(n_tiles_w),
+            float(pixel_size_x), float(pixel_size_y), int(H), int(W), int(C), float(thresh), ptr(bg_rgb),
+            stream_of(grad_out.device)), "tile_based_vol_rendering_backward_sh_with_bg")
 
     # ------------------------------------------------------------------ RGB
     def rgb_forward(self, mean, cov, color, alpha, start, end, gaussian_ids, out, topleft, tile_size, n_tiles_h,
                     n_tiles_w, pixel_size_x, pixel_size_y, H, W, thresh, T_final=None, n_done=None,
                     write_empty=False):
         L = _lib.load()
-        check(L.gsmi_tile_based_vol_rendering_start_end(
+        check(L.gsmi_render_rgb_forward(
             mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(color), ptr(alpha), ptr(start),
             ptr(end), ptr(gaussian_ids), ptr(out), ptr(topleft), int(tile_size), int(n_tiles_h), int(n_tiles_w),
             float(pixel_size_x), float(pixel_size_y), int(H), int(W), float(thresh), ptr(T_final), ptr(n_done),
-            1 if write_empty else 0, stream_of(out.device)), "tile_based_vol_rendering_start_end")
+            1 if write_empty else 0, stream_of(out.device)), "render_rgb_forward")
 
     def rgb_backward(self, mean, cov, color, alpha, start, end, gaussian_ids, out, T_final, n_done, grad_out,
                      grad_mean, grad_cov, grad_color, grad_alpha, topleft, tile_size, n_tiles_h, n_tiles_w,
                      pixel_size_x, pixel_size_y, H, W, thresh):
+        """RGB adjoint from saved T_final / n_done, or (both None) with the
+        forward state recomputed inside the launch."""
         L = _lib.load()
-        check(L.gsmi_tile_based_vol_rendering_backward_start_end(
+        check(L.gsmi_render_rgb_backward(
             mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(color), ptr(alpha), ptr(start),
             ptr(end), ptr(gaussian_ids), ptr(out), ptr(T_final), ptr(n_done), ptr(grad_out), ptr(grad_mean),
             ptr(grad_cov), ptr(grad_color), ptr(grad_alpha), ptr(topleft), int(tile_size), int(n_tiles_h),
             int(n_tiles_w), float(pixel_size_x), float(pixel_size_y), int(H), int(W), float(thresh),
-            stream_of(grad_out.device)), "tile_based_vol_rendering_backward_start_end")
+            stream_of(grad_out.device)), "render_rgb_backward")
 
     # gs/src/render.cu:399-441
     def tile_based_vol_rendering_start_end(self, mean, cov, color, alpha, start, end, gaussian_ids, out, topleft,
@@ -253,8 +258,11 @@ class _Backend:
             check_float(n, t)
         for n, t in (("start", start), ("end", end), ("gaussian_ids", gaussian_ids)):
             check_int(n, t)
-        self.rgb_forward(mean, cov, color, alpha, start, end, gaussian_ids, out, topleft, tile_size, n_tiles_h,
-                         n_tiles_w, pixel_size_x, pixel_size_y, H, W, thresh)
+        check(_lib.load().gsmi_tile_based_vol_rendering_start_end(
+            mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(color), ptr(alpha), ptr(start),
+            ptr(end), ptr(gaussian_ids), ptr(out), ptr(topleft), int(tile_size), int(n_tiles_h), int(n_tiles_w),
+            float(pixel_size_x), float(pixel_size_y), int(H), int(W), float(thresh), stream_of(out.device)),
+            "tile_based_vol_rendering_start_end")
 
     # gs/src/render.cu:443-481
     def tile_based_vol_rendering_backward_start_end(self, mean, cov, color, alpha, start, end, gaussian_ids, out,
@@ -267,16 +275,12 @@ class _Backend:
             check_float(n, t)
         for n, t in (("start", start), ("end", end), ("gaussian_ids", gaussian_ids)):
             check_int(n, t)
-        dev = mean.device
-        scratch = torch.zeros(int(H) * int(W) * 3, dtype=torch.float32, device=dev)
-        T = torch.empty(int(H) * int(W), dtype=torch.float32, device=dev)
-        nd = torch.empty(int(H) * int(W), dtype=torch.int32, device=dev)
-        self.rgb_forward(mean, cov, color, alpha, start, end, gaussian_ids, scratch, topleft, tile_size, n_tiles_h,
-                         n_tiles_w, pixel_size_x, pixel_size_y, H, W, thresh, T_final=T, n_done=nd,
-                         write_empty=True)
-        self.rgb_backward(mean, cov, color, alpha, start, end, gaussian_ids, out, T, nd, grad_out, grad_mean,
-                          grad_cov, grad_color, grad_alpha, topleft, tile_size, n_tiles_h, n_tiles_w, pixel_size_x,
-                          pixel_size_y, H, W, thresh)
+        check(_lib.load().gsmi_tile_based_vol_rendering_backward_start_end(
+            mean.shape[0], gaussian_ids.shape[0], ptr(mean), ptr(cov), ptr(color), ptr(alpha), ptr(start),
+            ptr(end), ptr(gaussian_ids), ptr(out), ptr(grad_mean), ptr(grad_cov), ptr(grad_color), ptr(grad_alpha),
+            ptr(grad_out), ptr(topleft), int(tile_size), int(n_tiles_h), int(n_tiles_w), float(pixel_size_x),
+            float(pixel_size_y), int(H), int(W), float(thresh), stream_of(grad_out.device)),
+            "tile_based_vol_rendering_backward_start_end")
 
 
 _backend = _Backend()

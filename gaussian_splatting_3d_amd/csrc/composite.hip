@@ -261,8 +261,9 @@ __device__ __forceinline__ float dot_fast(const float* a, const float* b) {
   return s;
 }
 
-// Stage entries [s0+base, s0+base+cnt) of the tile list into LDS.
-template <int SHN, int CC, bool IDS_IN_LDS = false>
+// Stage entries [s0+base, s0+base+cnt) of the tile list into LDS (without
+// WITH_SH only the geometry and opacity).
+template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true>
 __device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
                                             GeoB* s_gb, float2* s_gc, float* s_sh) {
   const int t = threadIdx.x;
@@ -277,6 +278,7 @@ __device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int bas
     s_gb[t] = GeoB{c.z, c.w, det, 1.0f / det};
     s_gc[t] = make_float2(al, reject_radius(al));
   }
+  if constexpr (!WITH_SH) return;
   __syncthreads();
   for (int e = t; e < cnt * SHN; e += kTilePixels) {
     const int j = e / SHN;
@@ -513,6 +515,94 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   if (a.n_done) a.n_done[pix] = ndone;
 }
 
+__device__ __forceinline__ void stage_rgb(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
+                                          GeoB* s_gb, float2* s_gc, float* s_col);
+
+// ---------------------------------------------------------------------------
+// Forward state recomputed inside a backward launch, for the backward entry
+// points that take only the reference's arguments (render.h:78-84, 95-101,
+// 126-133: no saved transmittance).  Each thread walks its pixel's list front
+// to back with the forward's own decisions (k_fwd_sh, k_fwd_rgb: the same
+// rejection test, exact Gaussian value, 1/255 skip and transmittance break),
+// so it ends with the bits the forward stores as T_final and n_done; with
+// COLOUR (background variant) also the colour before the blend, in the
+// forward's fast arithmetic.  Pixels outside the image get T = 1, nd = 0, as
+// a backward reads them.  Batches of NB entries go through the calling
+// kernel's staging arrays.  The reference recomputes the same front-to-back
+// state in its backward (vol_render_sh.h:268-351).
+// ---------------------------------------------------------------------------
+template <int NB, int C, bool COLOUR, bool RGB>
+__device__ __forceinline__ void forward_state(const RasterArgs& a, int s0, int n, const float* Y, float px,
+                                              float py, bool inside, int* s_id, GeoA* s_ga, GeoB* s_gb,
+                                              float2* s_gc, float* s_sh, float& T, int& nd, float& p0, float& p1,
+                                              float& p2) {
+  constexpr int CC = C * C;
+  constexpr int SHN = 3 * CC;
+  static_assert(NB <= 64, "one batch entry per lane in the relevance mask");
+  const float thresh = a.thresh;
+  T = 1.0f;
+  p0 = p1 = p2 = 0.0f;
+  bool done = !inside;
+  nd = n;
+  if (inside && T < thresh) {
+    done = true;
+    nd = 0;
+  }
+  const float4 wr = wave_pixel_rect(px, py);
+  const int lane = threadIdx.x & 63;
+  for (int base = 0; base < n; base += NB) {
+    const int cnt = min(NB, n - base);
+    __syncthreads();
+    if constexpr (RGB)
+      stage_rgb(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    else
+      stage_batch<SHN, CC, false, COLOUR>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    __syncthreads();
+    const int le = lane < cnt ? lane : 0;
+    uint64_t rel;
+    if constexpr (RGB)
+      rel = __ballot(lane < cnt);  // the RGB forward walks every entry
+    else
+      rel = __ballot(lane < cnt && may_reach(s_ga[le], s_gb[le], s_gc[le].y, wr.x, wr.y, wr.z, wr.w));
+    while (rel) {
+      const int j = __builtin_ctzll(rel);
+      rel &= rel - 1;
+      if (__all(done)) break;
+      if (done) continue;
+      const GeoA ga = s_ga[j];
+      const GeoB gb = s_gb[j];
+      const float2 gc = s_gc[j];
+      const float dx = px - ga.mx, dy = py - ga.my;
+      const float tmpx = dx * gb.c3 - dy * gb.c2;
+      const float tmpy = -dx * ga.c1 + dy * ga.c0;
+      const float q = tmpx * dx + tmpy * dy;
+      if (q * gb.inv_det > gc.y) continue;
+      const float G = RGB ? gauss_exact_d(dx, dy, ga.c0, ga.c1, gb.c2, gb.c3) : gauss_exact(q, gb.det);
+      const float al = gc.x;
+      float coeff = al * T;
+      coeff *= G;
+      if (al * G < kMinRenderAlpha) continue;
+      if constexpr (COLOUR && !RGB) {
+        if (isnan(coeff)) coeff = 0.0f;
+        const float* shj = s_sh + j * SHN;
+        p0 = fmaf(coeff, fmaxf(sigmoid_fast(dot_fast<CC>(shj, Y)), 0.0f), p0);
+        p1 = fmaf(coeff, fmaxf(sigmoid_fast(dot_fast<CC>(shj + CC, Y)), 0.0f), p1);
+        p2 = fmaf(coeff, fmaxf(sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y)), 0.0f), p2);
+      }
+      T = T * (1.0f - al * G);
+      if (T < thresh) {
+        done = true;
+        nd = base + j + 1;
+      }
+    }
+    if (!__syncthreads_or(!done)) break;
+  }
+  if (!inside) {
+    T = 1.0f;
+    nd = 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Wave64 reduce-scatter: NV values per lane in, one column sum per lane out.
 // After log2(NV) halving rounds (partner = lane ^ m, m = 32, 16, ...) each
@@ -663,7 +753,7 @@ __device__ __forceinline__ int block_max(int v, int* s_tmp) {
 // s = (dx c3 - dy c2)/det, u = (-dx c1 + dy c0)/det.  T_j is recovered from
 // the stored final transmittance as T_j = T_{j+1} / (1 - a G).
 // ---------------------------------------------------------------------------
-template <int C, bool BG>
+template <int C, bool BG, bool RECOMP = false>
 __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
@@ -700,16 +790,27 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
 
   float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
   int nd = 0;
+  float pre0 = 0.0f, pre1 = 0.0f, pre2 = 0.0f;
+  if constexpr (RECOMP)
+    forward_state<kBatch, C, BG, false>(a, s0, e0 - s0, Y, px, py, inside, s_id, s_ga, s_gb, s_gc, s_sh, T, nd,
+                                        pre0, pre1, pre2);
   if (inside) {
-    T = a.T_saved[pix];
-    nd = a.n_done_saved[pix];
+    if constexpr (!RECOMP) {
+      T = a.T_saved[pix];
+      nd = a.n_done_saved[pix];
+      if (BG) {
+        pre0 = a.pre_bg_saved[3 * pix + 0];
+        pre1 = a.pre_bg_saved[3 * pix + 1];
+        pre2 = a.pre_bg_saved[3 * pix + 2];
+      }
+    }
     g0 = a.grad_out[3 * pix + 0];
     g1 = a.grad_out[3 * pix + 1];
     g2 = a.grad_out[3 * pix + 2];
     if (BG) {
-      S0 = a.out_saved[3 * pix + 0] - a.pre_bg_saved[3 * pix + 0];
-      S1 = a.out_saved[3 * pix + 1] - a.pre_bg_saved[3 * pix + 1];
-      S2 = a.out_saved[3 * pix + 2] - a.pre_bg_saved[3 * pix + 2];
+      S0 = a.out_saved[3 * pix + 0] - pre0;
+      S1 = a.out_saved[3 * pix + 1] - pre1;
+      S2 = a.out_saved[3 * pix + 2] - pre2;
     }
   }
   const int nmax = block_max(nd, s_tmp);
@@ -808,35 +909,38 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
 // pixel are staged in LDS, 5 entries at a time, as a 15 x 64 matrix
 // (row = ch * 5 + entry slot, columns = the wave's 64 pixels), and
 //   grad_sh[e][ch][k] += sum_p cc[ch,e](p) * Y_k(p)
-// is one 16x64 by 64x16 product (row 15 unused) on bf16 matrix cores with
-// fp32 accumulation, in three terms: each fp32 operand x is split into
-// hi = x with the low 16 bits cleared (exactly a bf16) and lo = x - hi (exact
-// in fp32, then truncated to bf16), and
-//   A B ~= Ah Bh + Ah Bl + Al Bh,
-// dropping Al Bl and the truncation of lo: relative error below 2^-15 per
-// product, accumulated in fp32.  That is 6 v_mfma_f32_16x16x32_bf16 (2
-// K-steps of 32 pixels x 3 terms, 16 cycles each) per chunk instead of 16
-// fp32 16x16x4 steps of 32 cycles.  The Y operand is constant per wave and
-// lives in 16 VGPRs (hi and lo, 2 K-steps, 8 bf16 each).  Entries without a
-// contributing pixel take no slot.  The six scalar gradient components (mean,
-// cov, alpha) are reduced with an 8-wide wave reduce-scatter.
+// is one 16x64 by 64x16 product (row 15 unused) with fp32 accumulation.
 //
-// Operand layout of the 16x16x32 bf16 MFMA: lane i + 16g holds 8 values of
-// row i of A (and of column i of B) for the same 8 K indices; D[r][c] is in
-// lane c + 16 (r / 4), accumulator register r % 4.  Both operands here give
-// lane group g pixels 32t + 8g .. 32t + 8g + 7 for K-step t, so the product
-// sums every pixel once whatever order the hardware assigns to the 8 K
-// indices of a lane.  A staged row holds the 64 pixels in 8 blocks of 8
-// (bf16), block b at position b ^ (row & 7), so each lane's A values of one
-// K-step are one 16-byte load and the 8 rows a load touches fall on distinct
-// banks.
+// Default: fp32 products, as the reference forms them (vol_render_sh.h:28-36,
+// :328-333): 16 v_mfma_f32_16x16x4_f32 per chunk.  Operand layout: A[i][k]
+// from lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane j + 16 (i / 4),
+// accumulator register i % 4.  K-step t takes pixel 16 g + t from lane group
+// g in both operands, so the product sums every pixel once.  A staged row
+// keeps its 64 floats in 16-byte blocks, block b at position b ^ (row & 15),
+// so a lane's 16 A values are four 16-byte loads and the rows of one load
+// fall on distinct banks; the Y operand is constant per wave (16 VGPRs).
 //
-// The per-entry sums of the 48 SH components go to per-wave LDS accumulators
-// with plain stores: an entry takes at most one slot per wave and batch, so
-// each (entry, component) is written at most once per wave, and the four
-// waves' values are added when the batch is flushed to the global gradients.
-// (LDS float atomics from every lane of every flush were the kernel's largest
-// cost.)  The six scalar components keep LDS atomics (6 lanes per entry).
+// SPLIT (opt-in; not fp32 arithmetic): the same product on bf16 matrix cores
+// as a three-term split.  Each fp32 operand x is split into hi = x with the
+// low 16 bits cleared (exactly a bf16) and lo = x - hi (exact in fp32, then
+// truncated to bf16), and A B ~= Ah Bh + Ah Bl + Al Bh, dropping Al Bl and
+// the truncation of lo: relative error up to 2^-15 per product, biased toward
+// zero.  6 v_mfma_f32_16x16x32_bf16 (16 cycles each) per chunk instead of 16
+// fp32 steps of 32 cycles.  Layout of the 16x16x32 bf16 MFMA: lane i + 16g
+// holds 8 values of row i of A (and of column i of B) for the same 8 K
+// indices; both operands give lane group g pixels 32t + 8g .. 32t + 8g + 7
+// for K-step t.  A staged row holds the 64 pixels in 8 blocks of 8 (bf16),
+// block b at position b ^ (row & 7).
+//
+// Entries without a contributing pixel take no slot.  The per-entry sums of
+// the SH components go to per-wave LDS accumulators with plain stores: an
+// entry takes at most one slot per wave and batch, so each (entry,
+// component) is written at most once per wave, and the four waves' values are
+// added when the batch is flushed to the global gradients.  (LDS float
+// atomics from every lane of every flush were the kernel's largest cost.)
+// The six scalar components (mean, cov, alpha) use an 8-wide wave
+// reduce-scatter and LDS atomics (6 lanes per entry).  RECOMP: the forward
+// state comes from forward_state instead of saved T_final / n_done.
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -849,14 +953,21 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMfmaBatch = GSMI_BWD_BATCH;
 constexpr int kChunk = 5;        // entry slots per MFMA reduction (3 * 5 = 15 of 16 rows)
 constexpr int kCcRows = 3 * kChunk;
-constexpr int kCcStride = 64;    // bf16 per staged row (128 B)
-constexpr int kCcPlane = kCcRows * kCcStride;  // bf16 per plane (hi or lo)
+constexpr int kCcStride = 64;    // bf16 per staged row (128 B), SPLIT
+constexpr int kCcPlane = kCcRows * kCcStride;  // bf16 per plane (hi or lo), SPLIT
+constexpr int kCcWords = kCcRows * 64;          // 32-bit words per wave: 15 fp32 rows, or the two bf16 planes
 constexpr int kSlotBits = 6;     // batch positions of the chunk's slots, packed in one word
 static_assert(kMfmaBatch <= (1 << kSlotBits) && kChunk * kSlotBits <= 32, "slot packing");
+static_assert(2 * kCcPlane * 2 == kCcWords * 4, "the two bf16 planes take the fp32 rows' bytes");
 
-// Position (bf16 units) of pixel p in staged row `row`: 8-pixel blocks XOR-swizzled by row.
+// Position (bf16 units) of pixel p in staged bf16 row `row`: 8-pixel blocks XOR-swizzled by row.
 __device__ __forceinline__ int cc_pos(int row, int p) {
   return row * kCcStride + (p ^ ((row & 7) << 3));  // p < 64: block p / 8 -> (p / 8) ^ (row & 7)
+}
+
+// Position (floats) of pixel p in staged fp32 row `row`: 4-float blocks XOR-swizzled by row.
+__device__ __forceinline__ int cc_pos_f32(int row, int p) {
+  return row * 64 + (p ^ ((row & 15) << 2));  // p < 64: block p / 4 -> (p / 4) ^ (row & 15)
 }
 
 // Upper 16 bits of x (x truncated to bf16) and of the remainder x - hi.
@@ -866,7 +977,7 @@ __device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) 
   lo = __float_as_uint(x - __uint_as_float(b & 0xffff0000u)) >> 16;
 }
 
-template <int C, bool BG>
+template <int C, bool BG, bool SPLIT, bool RECOMP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bwd_sh_mfma(RasterArgs a) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
@@ -879,7 +990,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   __shared__ float s_sh[kMfmaBatch * SHN];
   __shared__ float s_acc[kMfmaBatch * 6];                      // scalar components, shared
   __shared__ __align__(16) float s_shacc[4][kMfmaBatch * SHN];  // SH components, per wave
-  __shared__ __align__(16) uint16_t s_cc[4][2 * kCcPlane];     // per wave: hi plane, lo plane
+  __shared__ __align__(16) uint32_t s_cc[4][kCcWords];         // per wave: staged coefficient rows
   __shared__ int s_tmp[4];
   __shared__ uint32_t s_touch;          // batch entries with a contribution in some wave
   __shared__ int s_tlist[kMfmaBatch];   // their positions, ascending
@@ -908,13 +1019,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   static_assert(kMfmaBatch <= 32, "one batch entry per lane in the relevance mask");
   const float4 wr = wave_pixel_rect(px, py);
 
-  // B operand: lane j + 16g holds Y_j at pixels 32t + 8g + e (e = 0..7) of
-  // this wave for K-step t, split into hi and lo bf16.  Transposed through
-  // this wave's staging area as fp32, 8 basis functions at a time.
-  static_assert(2 * kCcPlane * 2 >= 64 * 8 * 4, "transpose does not fit the staging area");
-  uint16_t* cc_w = s_cc[wv];
-  float* tr = reinterpret_cast<float*>(cc_w);
+  float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
+  int nd = 0;
+  float pre0 = 0.0f, pre1 = 0.0f, pre2 = 0.0f;
+  if constexpr (RECOMP)
+    forward_state<kMfmaBatch, C, BG, false>(a, s0, e0 - s0, Y, px, py, inside, s_id, s_ga, s_gb, s_gc, s_sh, T,
+                                            nd, pre0, pre1, pre2);
+  if (inside) {
+    if constexpr (!RECOMP) {
+      T = a.T_saved[pix];
+      nd = a.n_done_saved[pix];
+      if (BG) {
+        pre0 = a.pre_bg_saved[3 * pix + 0];
+        pre1 = a.pre_bg_saved[3 * pix + 1];
+        pre2 = a.pre_bg_saved[3 * pix + 2];
+      }
+    }
+    g0 = a.grad_out[3 * pix + 0];
+    g1 = a.grad_out[3 * pix + 1];
+    g2 = a.grad_out[3 * pix + 2];
+    if (BG) {
+      S0 = a.out_saved[3 * pix + 0] - pre0;
+      S1 = a.out_saved[3 * pix + 1] - pre1;
+      S2 = a.out_saved[3 * pix + 2] - pre2;
+    }
+  }
+
+  // B operand (the per-pixel SH basis), transposed through this wave's
+  // staging area as fp32, 8 basis functions at a time.  FP32: bY[t] = Y_j at
+  // pixel 16g + t for lane j + 16g.  SPLIT: lane j + 16g holds Y_j at pixels
+  // 32t + 8g + e (e = 0..7) for K-step t, split into hi and lo bf16.
+  static_assert(kCcWords >= 64 * 8, "transpose does not fit the staging area");
+  uint32_t* cc_raw = s_cc[wv];
+  float* tr = reinterpret_cast<float*>(cc_raw);
   u32x4 bh[2], bl[2];
+  float bY[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) bY[k] = 0.0f;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
 #pragma unroll
@@ -922,44 +1063,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     __builtin_amdgcn_wave_barrier();
     const int col = (lane & 15) - 8 * h;
     if (col >= 0 && col < 8) {
+      if constexpr (SPLIT) {
 #pragma unroll
-      for (int st = 0; st < 2; ++st) {
+        for (int st = 0; st < 2; ++st) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int p = 32 * st + 8 * (lane >> 4) + 2 * q;
-          uint32_t h0, l0, h1, l1;
-          split_bf16(tr[p * 8 + col], h0, l0);
-          split_bf16(tr[(p + 1) * 8 + col], h1, l1);
-          bh[st][q] = h0 | (h1 << 16);
-          bl[st][q] = l0 | (l1 << 16);
+          for (int q = 0; q < 4; ++q) {
+            const int p = 32 * st + 8 * (lane >> 4) + 2 * q;
+            uint32_t h0, l0, h1, l1;
+            split_bf16(tr[p * 8 + col], h0, l0);
+            split_bf16(tr[(p + 1) * 8 + col], h1, l1);
+            bh[st][q] = h0 | (h1 << 16);
+            bl[st][q] = l0 | (l1 << 16);
+          }
         }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) bY[k] = tr[(16 * (lane >> 4) + k) * 8 + col];
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
 
-  float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
-  int nd = 0;
-  if (inside) {
-    T = a.T_saved[pix];
-    nd = a.n_done_saved[pix];
-    g0 = a.grad_out[3 * pix + 0];
-    g1 = a.grad_out[3 * pix + 1];
-    g2 = a.grad_out[3 * pix + 2];
-    if (BG) {
-      S0 = a.out_saved[3 * pix + 0] - a.pre_bg_saved[3 * pix + 0];
-      S1 = a.out_saved[3 * pix + 1] - a.pre_bg_saved[3 * pix + 1];
-      S2 = a.out_saved[3 * pix + 2] - a.pre_bg_saved[3 * pix + 2];
-    }
-  }
   const int nmax = block_max(nd, s_tmp);
   const int my_comp = lane >> 3;                      // scalar reduce-scatter (8 wide)
   const bool writer = (lane & 7) == 0 && my_comp < 6;
   // MFMA A operand: lane i + 16g reads row i (row 15 repeats row 0; its
-  // output row is discarded) at pixels 32t + 8g .. +7 of each plane.
+  // output row is discarded).  SPLIT: pixels 32t + 8g .. +7 of each plane.
+  // FP32: pixels 16g .. 16g + 15, blocks 4g + q (q = 0..3) of the row.
   const int arow = (lane & 15) < kCcRows ? (lane & 15) : 0;
+  uint16_t* cc_w = reinterpret_cast<uint16_t*>(cc_raw);
+  float* cc_f = reinterpret_cast<float*>(cc_raw);
   const u32x4* a_src0 = reinterpret_cast<const u32x4*>(cc_w + cc_pos(arow, 8 * (lane >> 4)));
   const u32x4* a_src1 = reinterpret_cast<const u32x4*>(cc_w + cc_pos(arow, 32 + 8 * (lane >> 4)));
+  const floatx4* a_row = reinterpret_cast<const floatx4*>(cc_f + arow * 64);
+  const int a_blk = (4 * (lane >> 4)) ^ (arow & 15);  // block 4g + q sits at a_blk ^ q
   float* shacc_w = s_shacc[wv];
   const int d_ch = lane >> 4;                          // channel of this lane's MFMA output rows
   const int d_k = lane & 15;                           // basis function of this lane's output column
@@ -1044,16 +1181,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
           v[6] = 0.0f;
           v[7] = 0.0f;
           const int m = n_slots;
-          uint32_t h, l;
-          split_bf16(cc0, h, l);
-          cc_w[cc_pos(0 * kChunk + m, lane)] = h;
-          cc_w[kCcPlane + cc_pos(0 * kChunk + m, lane)] = l;
-          split_bf16(cc1, h, l);
-          cc_w[cc_pos(1 * kChunk + m, lane)] = h;
-          cc_w[kCcPlane + cc_pos(1 * kChunk + m, lane)] = l;
-          split_bf16(cc2, h, l);
-          cc_w[cc_pos(2 * kChunk + m, lane)] = h;
-          cc_w[kCcPlane + cc_pos(2 * kChunk + m, lane)] = l;
+          if constexpr (SPLIT) {
+            uint32_t h, l;
+            split_bf16(cc0, h, l);
+            cc_w[cc_pos(0 * kChunk + m, lane)] = h;
+            cc_w[kCcPlane + cc_pos(0 * kChunk + m, lane)] = l;
+            split_bf16(cc1, h, l);
+            cc_w[cc_pos(1 * kChunk + m, lane)] = h;
+            cc_w[kCcPlane + cc_pos(1 * kChunk + m, lane)] = l;
+            split_bf16(cc2, h, l);
+            cc_w[cc_pos(2 * kChunk + m, lane)] = h;
+            cc_w[kCcPlane + cc_pos(2 * kChunk + m, lane)] = l;
+          } else {
+            cc_f[cc_pos_f32(0 * kChunk + m, lane)] = cc0;
+            cc_f[cc_pos_f32(1 * kChunk + m, lane)] = cc1;
+            cc_f[cc_pos_f32(2 * kChunk + m, lane)] = cc2;
+          }
           slots |= static_cast<uint32_t>(j) << (kSlotBits * m);
           n_slots = m + 1;
           if (lane == 0) atomicOr(&s_touch, 1u << j);
@@ -1065,21 +1208,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         // LDS operations of one wave complete in order; the barrier keeps the
         // compiler from moving the loads above the stores.
         __builtin_amdgcn_wave_barrier();
-        const u32x4 ah0 = a_src0[0], ah1 = a_src1[0];  // pixels 8g.., 32 + 8g..
-        const u32x4 al0 = a_src0[kCcPlane / 8], al1 = a_src1[kCcPlane / 8];
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah0),
-                                                      __builtin_bit_cast(bf16x8, bh[0]), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah0),
-                                                      __builtin_bit_cast(bf16x8, bl[0]), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al0),
-                                                      __builtin_bit_cast(bf16x8, bh[0]), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah1),
-                                                      __builtin_bit_cast(bf16x8, bh[1]), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah1),
-                                                      __builtin_bit_cast(bf16x8, bl[1]), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al1),
-                                                      __builtin_bit_cast(bf16x8, bh[1]), acc, 0, 0, 0);
+        if constexpr (SPLIT) {
+          const u32x4 ah0 = a_src0[0], ah1 = a_src1[0];  // pixels 8g.., 32 + 8g..
+          const u32x4 al0 = a_src0[kCcPlane / 8], al1 = a_src1[kCcPlane / 8];
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah0),
+                                                        __builtin_bit_cast(bf16x8, bh[0]), acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah0),
+                                                        __builtin_bit_cast(bf16x8, bl[0]), acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al0),
+                                                        __builtin_bit_cast(bf16x8, bh[0]), acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah1),
+                                                        __builtin_bit_cast(bf16x8, bh[1]), acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah1),
+                                                        __builtin_bit_cast(bf16x8, bl[1]), acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al1),
+                                                        __builtin_bit_cast(bf16x8, bh[1]), acc, 0, 0, 0);
+        } else {
+          floatx4 av[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) av[q] = a_row[a_blk ^ q];
+#pragma unroll
+          for (int k = 0; k < 16; ++k)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[k >> 2][k & 3], bY[k], acc, 0, 0, 0);
+        }
         if (d_k < CC) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
@@ -1216,6 +1368,7 @@ __global__ __launch_bounds__(256) void k_fwd_rgb(RasterArgs a) {
   if (a.n_done) a.n_done[pix] = ndone;
 }
 
+template <bool RECOMP>
 __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
   constexpr int NCOMP = 9;
   constexpr int NV = 16;
@@ -1239,9 +1392,16 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
   const float py = tc.py;
   float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
   int nd = 0;
+  if constexpr (RECOMP) {
+    float p0, p1, p2;
+    forward_state<kBatch, 1, false, true>(a, s0, e0 - s0, nullptr, px, py, inside, s_id, s_ga, s_gb, s_gc, s_col,
+                                          T, nd, p0, p1, p2);
+  }
   if (inside) {
-    T = a.T_saved[pix];
-    nd = a.n_done_saved[pix];
+    if constexpr (!RECOMP) {
+      T = a.T_saved[pix];
+      nd = a.n_done_saved[pix];
+    }
     g0 = a.grad_out[3 * pix + 0];
     g1 = a.grad_out[3 * pix + 1];
     g2 = a.grad_out[3 * pix + 2];
@@ -1438,15 +1598,37 @@ static int launch_fwd_sh(const RasterArgs& a, int exact, hipStream_t st) {
   return GSMI_OK;
 }
 
+// recomp: no saved forward state (a.T_saved == null); split: bf16 split of
+// the SH-gradient reduction (orders 3 and 4, saved state only).
 template <int C, bool BG>
-static int launch_bwd_sh(const RasterArgs& a, hipStream_t st) {
+static int launch_bwd_sh(const RasterArgs& a, bool split, bool recomp, hipStream_t st) {
   const int grid = xcd_grid(a.n_tiles_total);
-  if constexpr (C >= 3)
-    hipLaunchKernelGGL((k_bwd_sh_mfma<C, BG>), dim3(grid), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_bwd_sh<C, BG>), dim3(grid), dim3(256), 0, st, a);
+  if constexpr (C >= 3) {
+    if (recomp)
+      hipLaunchKernelGGL((k_bwd_sh_mfma<C, BG, false, true>), dim3(grid), dim3(256), 0, st, a);
+    else if (split)
+      hipLaunchKernelGGL((k_bwd_sh_mfma<C, BG, true, false>), dim3(grid), dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_bwd_sh_mfma<C, BG, false, false>), dim3(grid), dim3(256), 0, st, a);
+  } else {
+    if (recomp)
+      hipLaunchKernelGGL((k_bwd_sh<C, BG, true>), dim3(grid), dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_bwd_sh<C, BG, false>), dim3(grid), dim3(256), 0, st, a);
+  }
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
+}
+
+static int dispatch_bwd_sh(const RasterArgs& a, int C, bool split, bool recomp, hipStream_t st) {
+  const bool B = a.bg != nullptr;
+  switch (C) {
+    case 1: return B ? launch_bwd_sh<1, true>(a, split, recomp, st) : launch_bwd_sh<1, false>(a, split, recomp, st);
+    case 2: return B ? launch_bwd_sh<2, true>(a, split, recomp, st) : launch_bwd_sh<2, false>(a, split, recomp, st);
+    case 3: return B ? launch_bwd_sh<3, true>(a, split, recomp, st) : launch_bwd_sh<3, false>(a, split, recomp, st);
+    default:
+      return B ? launch_bwd_sh<4, true>(a, split, recomp, st) : launch_bwd_sh<4, false>(a, split, recomp, st);
+  }
 }
 
 static int check_common(int64_t V, int64_t P, int tile_size, int n_tiles_h, int n_tiles_w, int H, int W) {
@@ -1457,19 +1639,31 @@ static int check_common(int64_t V, int64_t P, int tile_size, int n_tiles_h, int 
   return GSMI_OK;
 }
 
+// Checks shared by the SH backward entry points; sets *recomp when the
+// forward state is to be recomputed (T_final and n_done both null).
+static int check_bwd_state(const float* out, const float* T_final, const int32_t* n_done, const float* pre_bg,
+                           const float* bg, int flags, const void* workspace, bool* recomp) {
+  if ((T_final == nullptr) != (n_done == nullptr)) return GSMI_ERR_ARG;
+  *recomp = T_final == nullptr;
+  if (flags & ~GSMI_BWD_SPLIT_BF16) return GSMI_ERR_ARG;
+  if (*recomp && (flags & GSMI_BWD_SPLIT_BF16)) return GSMI_ERR_UNSUPPORTED;
+  if (*recomp && workspace) return GSMI_ERR_ARG;  // the launch order is derived from the saved n_done
+  if (bg && (!out || (!*recomp && !pre_bg))) return GSMI_ERR_ARG;
+  return GSMI_OK;
+}
+
 }  // namespace gsmi
 
 using namespace gsmi;
 
-extern "C" int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const float* mean, const float* cov,
-                                                const float* sh, int64_t sh_stride_g, int64_t sh_stride_c,
-                                                const int32_t* sh_index, const float* alpha, const int32_t* start,
-                                                const int32_t* end, const int32_t* gaussian_ids, float* out,
-                                                const float* topleft, const float* c2w, int tile_size,
-                                                int n_tiles_h, int n_tiles_w, float pixel_size_x,
-                                                float pixel_size_y, int H, int W, int C, float thresh,
-                                                const float* bg, float* T_final, int32_t* n_done, float* pre_bg,
-                                                int flags, int exact, void* stream) {
+extern "C" int gsmi_render_sh_forward(int64_t V, int64_t P, const float* mean, const float* cov, const float* sh,
+                                      int64_t sh_stride_g, int64_t sh_stride_c, const int32_t* sh_index,
+                                      const float* alpha, const int32_t* start, const int32_t* end,
+                                      const int32_t* gaussian_ids, float* out, const float* topleft,
+                                      const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+                                      float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh,
+                                      const float* bg, float* T_final, int32_t* n_done, float* pre_bg, int flags,
+                                      int exact, void* stream) {
   int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
   if (rc) return rc;
   if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
@@ -1514,22 +1708,28 @@ extern "C" int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const floa
   }
 }
 
-static int bwd_sh_impl(int64_t V, int64_t P, const float* mean, const float* cov, const float* sh,
-                       int64_t sh_stride_g, int64_t sh_stride_c, const int32_t* sh_index, const float* alpha,
-                       const int32_t* start, const int32_t* end, const int32_t* gaussian_ids, const float* out,
-                       const float* T_final, const int32_t* n_done, const float* pre_bg, const float* grad_out,
-                       float* grad_mean, float* grad_cov, float* grad_sh, float* grad_alpha, const float* topleft,
-                       const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x,
-                       float pixel_size_y, int H, int W, int C, float thresh, const float* bg, void* workspace,
-                       size_t workspace_bytes, void* stream) {
+extern "C" size_t gsmi_backward_order_workspace_size(int n_tiles_total) { return order_ws_bytes(n_tiles_total); }
+
+extern "C" int gsmi_render_sh_backward(int64_t V, int64_t P, const float* mean, const float* cov, const float* sh,
+                                       int64_t sh_stride_g, int64_t sh_stride_c, const int32_t* sh_index,
+                                       const float* alpha, const int32_t* start, const int32_t* end,
+                                       const int32_t* gaussian_ids, const float* out, const float* T_final,
+                                       const int32_t* n_done, const float* pre_bg, const float* grad_out,
+                                       float* grad_mean, float* grad_cov, float* grad_sh, float* grad_alpha,
+                                       const float* topleft, const float* c2w, int tile_size, int n_tiles_h,
+                                       int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W, int C,
+                                       float thresh, const float* bg, int flags, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
   int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
   if (rc) return rc;
   if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
-  if (!start || !end || !topleft || !c2w || !T_final || !n_done || !grad_out) return GSMI_ERR_ARG;
+  if (!start || !end || !topleft || !c2w || !grad_out) return GSMI_ERR_ARG;
   if (P > 0 && (!mean || !cov || !sh || !alpha || !gaussian_ids || !grad_mean || !grad_cov || !grad_sh ||
                 !grad_alpha))
     return GSMI_ERR_ARG;
-  if (bg && (!out || !pre_bg)) return GSMI_ERR_ARG;
+  bool recomp;
+  rc = check_bwd_state(out, T_final, n_done, pre_bg, bg, flags, workspace, &recomp);
+  if (rc) return rc;
   if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
   if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
   RasterArgs a{};
@@ -1569,52 +1769,73 @@ static int bwd_sh_impl(int64_t V, int64_t P, const float* mean, const float* cov
     rc = make_tile_order(a, workspace, st);
     if (rc) return rc;
   }
-  const bool B = bg != nullptr;
-  switch (C) {
-    case 1: return B ? launch_bwd_sh<1, true>(a, st) : launch_bwd_sh<1, false>(a, st);
-    case 2: return B ? launch_bwd_sh<2, true>(a, st) : launch_bwd_sh<2, false>(a, st);
-    case 3: return B ? launch_bwd_sh<3, true>(a, st) : launch_bwd_sh<3, false>(a, st);
-    default: return B ? launch_bwd_sh<4, true>(a, st) : launch_bwd_sh<4, false>(a, st);
-  }
+  return dispatch_bwd_sh(a, C, (flags & GSMI_BWD_SPLIT_BF16) != 0, recomp, st);
+}
+
+// The reference's own signatures (render.h:86-101, 119-133): sh_coeffs is the
+// contiguous [V, 3, C*C] slice, out the flat [H*W*3] image the caller zeroed,
+// and the backward recomputes the forward state itself.
+extern "C" int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const float* mean, const float* cov,
+                                                const float* sh_coeffs, const float* alpha, const int32_t* start,
+                                                const int32_t* end, const int32_t* gaussian_ids, float* out,
+                                                const float* topleft, const float* c2w, int tile_size,
+                                                int n_tiles_h, int n_tiles_w, float pixel_size_x,
+                                                float pixel_size_y, int H, int W, int C, float thresh,
+                                                void* stream) {
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  return gsmi_render_sh_forward(V, P, mean, cov, sh_coeffs, 3 * C * C, C * C, nullptr, alpha, start, end,
+                                gaussian_ids, out, topleft, c2w, tile_size, n_tiles_h, n_tiles_w, pixel_size_x,
+                                pixel_size_y, H, W, C, thresh, nullptr, nullptr, nullptr, nullptr, 0, 0, stream);
+}
+
+extern "C" int gsmi_tile_based_vol_rendering_sh_with_bg(int64_t V, int64_t P, const float* mean, const float* cov,
+                                                        const float* sh_coeffs, const float* alpha,
+                                                        const int32_t* start, const int32_t* end,
+                                                        const int32_t* gaussian_ids, float* out,
+                                                        const float* topleft, const float* c2w, int tile_size,
+                                                        int n_tiles_h, int n_tiles_w, float pixel_size_x,
+                                                        float pixel_size_y, int H, int W, int C, float thresh,
+                                                        const float* bg_rgb, void* stream) {
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (!bg_rgb) return GSMI_ERR_ARG;
+  return gsmi_render_sh_forward(V, P, mean, cov, sh_coeffs, 3 * C * C, C * C, nullptr, alpha, start, end,
+                                gaussian_ids, out, topleft, c2w, tile_size, n_tiles_h, n_tiles_w, pixel_size_x,
+                                pixel_size_y, H, W, C, thresh, bg_rgb, nullptr, nullptr, nullptr, 0, 0, stream);
 }
 
 extern "C" int gsmi_tile_based_vol_rendering_backward_sh(
-    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
-    int64_t sh_stride_c, const int32_t* sh_index, const float* alpha, const int32_t* start, const int32_t* end,
-    const int32_t* gaussian_ids, const float* out, const float* T_final, const int32_t* n_done,
-    const float* pre_bg, const float* grad_out, float* grad_mean, float* grad_cov, float* grad_sh,
-    float* grad_alpha, const float* topleft, const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w,
-    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float* bg, void* stream) {
-  return bwd_sh_impl(V, P, mean, cov, sh, sh_stride_g, sh_stride_c, sh_index, alpha, start, end, gaussian_ids, out,
-                     T_final, n_done, pre_bg, grad_out, grad_mean, grad_cov, grad_sh, grad_alpha, topleft, c2w,
-                     tile_size, n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg, nullptr, 0,
-                     stream);
+    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh_coeffs, const float* alpha,
+    const int32_t* start, const int32_t* end, const int32_t* gaussian_ids, const float* out, float* grad_mean,
+    float* grad_cov, float* grad_sh_coeffs, float* grad_alpha, const float* grad_out, const float* topleft,
+    const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x, float pixel_size_y, int H,
+    int W, int C, float thresh, void* stream) {
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  return gsmi_render_sh_backward(V, P, mean, cov, sh_coeffs, 3 * C * C, C * C, nullptr, alpha, start, end,
+                                 gaussian_ids, out, nullptr, nullptr, nullptr, grad_out, grad_mean, grad_cov,
+                                 grad_sh_coeffs, grad_alpha, topleft, c2w, tile_size, n_tiles_h, n_tiles_w,
+                                 pixel_size_x, pixel_size_y, H, W, C, thresh, nullptr, 0, nullptr, 0, stream);
 }
 
-extern "C" size_t gsmi_backward_order_workspace_size(int n_tiles_total) { return order_ws_bytes(n_tiles_total); }
-
-extern "C" int gsmi_tile_based_vol_rendering_backward_sh_ws(
-    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
-    int64_t sh_stride_c, const int32_t* sh_index, const float* alpha, const int32_t* start, const int32_t* end,
-    const int32_t* gaussian_ids, const float* out, const float* T_final, const int32_t* n_done,
-    const float* pre_bg, const float* grad_out, float* grad_mean, float* grad_cov, float* grad_sh,
-    float* grad_alpha, const float* topleft, const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w,
-    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float* bg, void* workspace,
-    size_t workspace_bytes, void* stream) {
-  return bwd_sh_impl(V, P, mean, cov, sh, sh_stride_g, sh_stride_c, sh_index, alpha, start, end, gaussian_ids, out,
-                     T_final, n_done, pre_bg, grad_out, grad_mean, grad_cov, grad_sh, grad_alpha, topleft, c2w,
-                     tile_size, n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg, workspace,
-                     workspace_bytes, stream);
+extern "C" int gsmi_tile_based_vol_rendering_backward_sh_with_bg(
+    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh_coeffs, const float* alpha,
+    const int32_t* start, const int32_t* end, const int32_t* gaussian_ids, const float* out, float* grad_mean,
+    float* grad_cov, float* grad_sh_coeffs, float* grad_alpha, const float* grad_out, const float* topleft,
+    const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x, float pixel_size_y, int H,
+    int W, int C, float thresh, const float* bg_rgb, void* stream) {
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (!bg_rgb) return GSMI_ERR_ARG;
+  return gsmi_render_sh_backward(V, P, mean, cov, sh_coeffs, 3 * C * C, C * C, nullptr, alpha, start, end,
+                                 gaussian_ids, out, nullptr, nullptr, nullptr, grad_out, grad_mean, grad_cov,
+                                 grad_sh_coeffs, grad_alpha, topleft, c2w, tile_size, n_tiles_h, n_tiles_w,
+                                 pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb, 0, nullptr, 0, stream);
 }
 
-extern "C" int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, const float* mean, const float* cov,
-                                                       const float* color, const float* alpha,
-                                                       const int32_t* start, const int32_t* end,
-                                                       const int32_t* gaussian_ids, float* out,
-                                                       const float* topleft, int tile_size, int n_tiles_h,
-                                                       int n_tiles_w, float pixel_size_x, float pixel_size_y,
-                                                       int H, int W, float thresh, float* T_final,
-                                                       int32_t* n_done, int flags, void* stream) {
+extern "C" int gsmi_render_rgb_forward(int64_t V, int64_t P, const float* mean, const float* cov,
+                                       const float* color, const float* alpha, const int32_t* start,
+                                       const int32_t* end, const int32_t* gaussian_ids, float* out,
+                                       const float* topleft, int tile_size, int n_tiles_h, int n_tiles_w,
+                                       float pixel_size_x, float pixel_size_y, int H, int W, float thresh,
+                                       float* T_final, int32_t* n_done, int flags, void* stream) {
   int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
   if (rc) return rc;
   if (!start || !end || !out || !topleft) return GSMI_ERR_ARG;
@@ -1648,19 +1869,22 @@ extern "C" int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, con
   return GSMI_OK;
 }
 
-extern "C" int gsmi_tile_based_vol_rendering_backward_start_end(
-    int64_t V, int64_t P, const float* mean, const float* cov, const float* color, const float* alpha,
-    const int32_t* start, const int32_t* end, const int32_t* gaussian_ids, const float* out, const float* T_final,
-    const int32_t* n_done, const float* grad_out, float* grad_mean, float* grad_cov, float* grad_color,
-    float* grad_alpha, const float* topleft, int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x,
-    float pixel_size_y, int H, int W, float thresh, void* stream) {
+extern "C" int gsmi_render_rgb_backward(int64_t V, int64_t P, const float* mean, const float* cov,
+                                        const float* color, const float* alpha, const int32_t* start,
+                                        const int32_t* end, const int32_t* gaussian_ids, const float* out,
+                                        const float* T_final, const int32_t* n_done, const float* grad_out,
+                                        float* grad_mean, float* grad_cov, float* grad_color, float* grad_alpha,
+                                        const float* topleft, int tile_size, int n_tiles_h, int n_tiles_w,
+                                        float pixel_size_x, float pixel_size_y, int H, int W, float thresh,
+                                        void* stream) {
   int rc = check_common(V, P, tile_size, n_tiles_h, n_tiles_w, H, W);
   if (rc) return rc;
-  if (!start || !end || !topleft || !T_final || !n_done || !grad_out) return GSMI_ERR_ARG;
+  if (!start || !end || !topleft || !grad_out) return GSMI_ERR_ARG;
+  if ((T_final == nullptr) != (n_done == nullptr)) return GSMI_ERR_ARG;
   if (P > 0 && (!mean || !cov || !color || !alpha || !gaussian_ids || !grad_mean || !grad_cov || !grad_color ||
                 !grad_alpha))
     return GSMI_ERR_ARG;
-  (void)out;
+  (void)out;  // the adjoint needs no output colour without a background
   if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
   RasterArgs a{};
   a.mean = mean;
@@ -1688,9 +1912,37 @@ extern "C" int gsmi_tile_based_vol_rendering_backward_start_end(
   a.n_tiles_total = n_tiles_h * n_tiles_w;
   a.H = H;
   a.W = W;
-  hipLaunchKernelGGL(k_bwd_rgb, dim3(xcd_grid(n_tiles_h * n_tiles_w)), dim3(256), 0, (hipStream_t)stream, a);
+  const dim3 grid(xcd_grid(n_tiles_h * n_tiles_w));
+  if (T_final)
+    hipLaunchKernelGGL(k_bwd_rgb<false>, grid, dim3(256), 0, (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL(k_bwd_rgb<true>, grid, dim3(256), 0, (hipStream_t)stream, a);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
+}
+
+// The reference's own signatures (render.h:70-84).
+extern "C" int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, const float* mean, const float* cov,
+                                                       const float* color, const float* alpha,
+                                                       const int32_t* start, const int32_t* end,
+                                                       const int32_t* gaussian_ids, float* out,
+                                                       const float* topleft, int tile_size, int n_tiles_h,
+                                                       int n_tiles_w, float pixel_size_x, float pixel_size_y,
+                                                       int H, int W, float thresh, void* stream) {
+  return gsmi_render_rgb_forward(V, P, mean, cov, color, alpha, start, end, gaussian_ids, out, topleft, tile_size,
+                                 n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, thresh, nullptr, nullptr,
+                                 0, stream);
+}
+
+extern "C" int gsmi_tile_based_vol_rendering_backward_start_end(
+    int64_t V, int64_t P, const float* mean, const float* cov, const float* color, const float* alpha,
+    const int32_t* start, const int32_t* end, const int32_t* gaussian_ids, const float* out, float* grad_mean,
+    float* grad_cov, float* grad_color, float* grad_alpha, const float* grad_out, const float* topleft,
+    int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W,
+    float thresh, void* stream) {
+  return gsmi_render_rgb_backward(V, P, mean, cov, color, alpha, start, end, gaussian_ids, out, nullptr, nullptr,
+                                  grad_out, grad_mean, grad_cov, grad_color, grad_alpha, topleft, tile_size,
+                                  n_tiles_h, n_tiles_w, pixel_size_x, pixel_size_y, H, W, thresh, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -1758,16 +2010,18 @@ extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const 
                                              const float* T_final, const int32_t* n_done, const float* pre_bg,
                                              const float* grad_out, float* grad_mean, float* grad_cov,
                                              float* grad_sh, float* grad_alpha, int C, float thresh,
-                                             const float* bg, void* workspace, size_t workspace_bytes,
+                                             const float* bg, int flags, void* workspace, size_t workspace_bytes,
                                              void* stream) {
   int rc = check_views(V, P, B, views, n_tiles_total);
   if (rc) return rc;
   if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
-  if (!start || !end || !T_final || !n_done || !grad_out) return GSMI_ERR_ARG;
+  if (!start || !end || !grad_out) return GSMI_ERR_ARG;
   if (P > 0 && (!mean || !cov || !sh || !alpha || !gaussian_ids || !grad_mean || !grad_cov || !grad_sh ||
                 !grad_alpha))
     return GSMI_ERR_ARG;
-  if (bg && (!out || !pre_bg)) return GSMI_ERR_ARG;
+  bool recomp;
+  rc = check_bwd_state(out, T_final, n_done, pre_bg, bg, flags, workspace, &recomp);
+  if (rc) return rc;
   if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
   if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15)) return GSMI_ERR_ARG;
   RasterArgs a{};
@@ -1801,13 +2055,7 @@ extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const 
     rc = make_tile_order(a, workspace, st);
     if (rc) return rc;
   }
-  const bool G = bg != nullptr;
-  switch (C) {
-    case 1: return G ? launch_bwd_sh<1, true>(a, st) : launch_bwd_sh<1, false>(a, st);
-    case 2: return G ? launch_bwd_sh<2, true>(a, st) : launch_bwd_sh<2, false>(a, st);
-    case 3: return G ? launch_bwd_sh<3, true>(a, st) : launch_bwd_sh<3, false>(a, st);
-    default: return G ? launch_bwd_sh<4, true>(a, st) : launch_bwd_sh<4, false>(a, st);
-  }
+  return dispatch_bwd_sh(a, C, (flags & GSMI_BWD_SPLIT_BF16) != 0, recomp, st);
 }
 
 // ---------------------------------------------------------------------------

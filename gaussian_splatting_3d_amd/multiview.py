@@ -197,13 +197,13 @@ def backward_views(s: ViewsState, mean, qvec, log_s, logit, sh_coeffs, grad_out)
         raise RuntimeError(f"grad_out has {go.numel()} values, expected {b.n_pix * 3}")
     if b.V == 0:  # no view sees a Gaussian: every gradient is zero (no launch takes empty buffers)
         return torch.zeros_like(mean), torch.zeros_like(qvec), torch.zeros_like(log_s), torch.zeros_like(logit), g_sh
-    nws =int(L.gsmi_backward_order_workspace_size(b.n_tiles))  # launch order, most expensive tiles first
+    nws = int(L.gsmi_backward_order_workspace_size(b.n_tiles))  # launch order, most expensive tiles first
     ows = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
     check(L.gsmi_render_views_backward_sh(b.V, b.P, b.B, ptr(b.views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d),
                                           ptr(sh_coeffs), 3 * cmax2, cmax2, ptr(b.orig), ptr(b.alpha), ptr(b.start),
                                           ptr(b.end), ptr(b.ids), ptr(s.out), ptr(s.T), ptr(s.n_done),
                                           ptr(s.pre_bg), ptr(go), ptr(g_m2d), ptr(g_cov), ptr(g_sh), ptr(g_al), s.C,
-                                          float(s.cfg.T_thresh), ptr(s.bg), ptr(ows), nws, st),
+                                          float(s.cfg.T_thresh), ptr(s.bg), 0, ptr(ows), nws, st),
           "render_views_backward_sh")
     g_mean = torch.empty_like(mean)
     g_q = torch.empty_like(qvec)

@@ -12,7 +12,7 @@ Forward (all on the current torch stream):
      the visible Gaussians by depth, pair expansion in that order, stable sort
      of the pairs by tile, start/end per tile (the order of the reference's
      (tile<<32 | depth) key sort);
-  6. gsmi_tile_based_vol_rendering_sh — compositing; SH rows are read from the
+  6. gsmi_render_sh_forward — compositing; SH rows are read from the
      raw [N,3,C_max^2] tensor through the original index (no gather copy).
   Two-level form (cfg.two_level, the default without background): steps 1-5
   count, sort and range the pairs per 64-px block of 4 x 4 tiles (about a
@@ -21,7 +21,7 @@ Forward (all on the current torch stream):
   depth-ordered list as it composites (the same list, so the same image, T and
   n_done) and records it for the backward.
 Backward:
-  7. gsmi_tile_based_vol_rendering_backward_sh — reverse-order adjoint; SH
+  7. gsmi_render_sh_backward — reverse-order adjoint; SH
      gradients land directly in the raw-layout [N,3,C_max^2] buffer;
   8. gsmi_project_backward  — projection/activation adjoint with the
      reference's detaches, zero for culled Gaussians.
@@ -246,12 +246,13 @@ def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact
     return out, ViewState(b, int(C), cfg, topleft, c2w_d, bg, out, T, nd, pre, early.get("g_sh"), early.get("bnd"))
 
 
-def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g_sh=None):
+def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g_sh=None, split_bf16=False):
     """Steps 7-8: gradients w.r.t. the raw parameters.  SH gradients are
     accumulated into g_sh when given (several views), else a new buffer.
     Returns (g_mean, g_qvec, g_log_s, g_logit, g_sh, boundary) where boundary
     holds the compositing gradients (mean2d, cov2d, alpha) of visible
-    Gaussians."""
+    Gaussians.  split_bf16: SH-gradient products on bf16 matrix cores as a
+    three-term split (opt-in; the default forms them in fp32)."""
     b, cfg, ic = s.b, s.cfg, s.b.consts
     dev = mean.device
     L = _lib.load()
@@ -274,7 +275,7 @@ def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g
                          s.pre_bg, grad_out.contiguous().view(-1), g_m2d, g_cov, g_sh, g_al, s.topleft, s.c2w_d,
                          cfg.tile_size, ic["n_tiles_h"], ic["n_tiles_w"], ic["pixel_size_x"], ic["pixel_size_y"],
                          ic["H"], ic["W"], s.C, cfg.T_thresh, bg_rgb=s.bg, sh_stride_g=3 * cmax2, sh_stride_c=cmax2,
-                         sh_index=b.orig)
+                         sh_index=b.orig, split_bf16=split_bf16)
     g_mean = torch.empty_like(mean)
     g_q = torch.empty_like(qvec)
     g_ls = torch.empty_like(log_s)

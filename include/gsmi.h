@@ -179,74 +179,150 @@ int gsmi_tile_ranges(int64_t P, const uint64_t *sorted_keys, int n_tiles, int32_
 /* ------------------------------------------------------------------------ */
 
 /*
- * Replaces tile_based_vol_rendering_sh (gs/src/render.cu:483-544,
- * gs/src/include/vol_render_sh.h:97-262; bound at bindings.cpp:42) and, with
- * bg != NULL, tile_based_vol_rendering_sh_with_bg (render.cu:780-842,
- * vol_render_bg.h:12-119).
+ * The reference's compositing functions with the reference's arguments (V =
+ * mean.size(0), P = gaussian_ids.size(0), then the tensors' device pointers
+ * in the order of gs/src/render.h, then the stream).  sh_coeffs is the
+ * contiguous [V, 3, C*C] slice (gs/sh_renderer.py:273), color [V,3], out the
+ * flat [H*W*3] image (HWC) that the caller zeroed (gs/renderer.py:695), and
+ * the grad_* arrays are zeroed or hold values to accumulate onto
+ * (renderer.py:760-763).  grad_cov receives 0.5 * dL/dcov_offdiag in both
+ * off-diagonals and grad_alpha the gradient w.r.t. the post-sigmoid alpha
+ * straight through the 0.99 clamp, as the reference.  The backward entries
+ * recompute the forward's transmittance inside the launch (the reference's
+ * backward does the same, vol_render_sh.h:268-351); for SH without a
+ * background and for RGB `out` is not read.  C outside 1..4 returns
+ * GSMI_ERR_UNSUPPORTED (the reference's switch silently does nothing).
  *
- * mean [V,2], cov [V,4], alpha [V] (post-sigmoid), start/end [n_tiles],
- * gaussian_ids [P], topleft [2] and c2w [3,4] are device arrays as in the
- * reference.  SH coefficient (g, ch, k) is read at
+ * tile_based_vol_rendering_sh: render.h:86-93, render.cu:483-544,
+ *   vol_render_sh.h:97-262; bound at bindings.cpp:42.
+ * tile_based_vol_rendering_backward_sh: render.h:95-101, render.cu:546-624,
+ *   vol_render_sh.h:268-476; bindings.cpp:44.
+ * tile_based_vol_rendering_sh_with_bg: render.h:119-124, render.cu:780-842,
+ *   vol_render_bg.h:12-119 (bg_rgb: device float[3]).
+ * tile_based_vol_rendering_backward_sh_with_bg: render.h:126-133,
+ *   render.cu:844-925, vol_render_bg.h:121-258 (out: the blended output).
+ * tile_based_vol_rendering_start_end: render.h:70-76, render.cu:399-441,
+ *   vol_render.h:716-798; bindings.cpp:31 (Gaussian in double,
+ *   kernels.h:195-224).
+ * tile_based_vol_rendering_backward_start_end: render.h:78-84,
+ *   render.cu:443-481, vol_render.h:800-925; bindings.cpp:35.
+ */
+int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const float *mean, const float *cov,
+                                     const float *sh_coeffs, const float *alpha, const int32_t *start,
+                                     const int32_t *end, const int32_t *gaussian_ids, float *out,
+                                     const float *topleft, const float *c2w, int tile_size, int n_tiles_h,
+                                     int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W, int C,
+                                     float thresh, void *stream);
+int gsmi_tile_based_vol_rendering_backward_sh(int64_t V, int64_t P, const float *mean, const float *cov,
+                                              const float *sh_coeffs, const float *alpha, const int32_t *start,
+                                              const int32_t *end, const int32_t *gaussian_ids, const float *out,
+                                              float *grad_mean, float *grad_cov, float *grad_sh_coeffs,
+                                              float *grad_alpha, const float *grad_out, const float *topleft,
+                                              const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+                                              float pixel_size_x, float pixel_size_y, int H, int W, int C,
+                                              float thresh, void *stream);
+int gsmi_tile_based_vol_rendering_sh_with_bg(int64_t V, int64_t P, const float *mean, const float *cov,
+                                             const float *sh_coeffs, const float *alpha, const int32_t *start,
+                                             const int32_t *end, const int32_t *gaussian_ids, float *out,
+                                             const float *topleft, const float *c2w, int tile_size, int n_tiles_h,
+                                             int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W,
+                                             int C, float thresh, const float *bg_rgb, void *stream);
+int gsmi_tile_based_vol_rendering_backward_sh_with_bg(
+    int64_t V, int64_t P, const float *mean, const float *cov, const float *sh_coeffs, const float *alpha,
+    const int32_t *start, const int32_t *end, const int32_t *gaussian_ids, const float *out, float *grad_mean,
+    float *grad_cov, float *grad_sh_coeffs, float *grad_alpha, const float *grad_out, const float *topleft,
+    const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x, float pixel_size_y, int H,
+    int W, int C, float thresh, const float *bg_rgb, void *stream);
+int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, const float *mean, const float *cov,
+                                            const float *color, const float *alpha, const int32_t *start,
+                                            const int32_t *end, const int32_t *gaussian_ids, float *out,
+                                            const float *topleft, int tile_size, int n_tiles_h, int n_tiles_w,
+                                            float pixel_size_x, float pixel_size_y, int H, int W, float thresh,
+                                            void *stream);
+int gsmi_tile_based_vol_rendering_backward_start_end(
+    int64_t V, int64_t P, const float *mean, const float *cov, const float *color, const float *alpha,
+    const int32_t *start, const int32_t *end, const int32_t *gaussian_ids, const float *out, float *grad_mean,
+    float *grad_cov, float *grad_color, float *grad_alpha, const float *grad_out, const float *topleft,
+    int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W,
+    float thresh, void *stream);
+
+/*
+ * Extended SH forward (the entries above call it; this build's pipeline
+ * calls it directly).  Same arithmetic as gsmi_tile_based_vol_rendering_sh
+ * (bg == NULL) or _with_bg (bg = device float[3]), plus:
+ * SH coefficient (g, ch, k) is read at
  * sh[sh_row(g) * sh_stride_g + ch * sh_stride_c + k], sh_row(g) =
  * sh_index ? sh_index[g] : g (the reference layout is sh_stride_g = 3*C*C,
- * sh_stride_c = C*C, sh_index = NULL).
- * out [H*W*3] (HWC).  flags bit 0: also write pixels of empty tiles (zeros;
- * the reference leaves them untouched).  Optional per-pixel state for the
- * backward pass (NULL to skip): T_final [H*W], n_done [H*W] (list entries
- * examined before the transmittance break), pre_bg [H*W*3] (colour before
- * the background blend; bg mode only).  bg: NULL or device float[3].
- * exact != 0 evaluates colours with the oracle's exact operation sequence
- * (bit-reproducible output); exact == 0 uses fused multiply-adds for the SH
- * dot product and the hardware exponential for the colour sigmoid.  All
- * compositing decisions are exact in both modes.
+ * sh_stride_c = C*C, sh_index = NULL), so the raw [N,3,C_max^2] tensor is
+ * read through the compacted records' original index without a copy.
+ * flags bit 0: also write pixels of empty tiles (zeros; the reference leaves
+ * them untouched).  Optional per-pixel state for the backward pass (NULL to
+ * skip): T_final [H*W], n_done [H*W] (list entries examined before the
+ * transmittance break), pre_bg [H*W*3] (colour before the background blend;
+ * bg mode only).  exact != 0 evaluates colours with the oracle's exact
+ * operation sequence (bit-reproducible output); exact == 0 uses fused
+ * multiply-adds for the SH dot product and the hardware exponential for the
+ * colour sigmoid.  All compositing decisions are exact in both modes.
  */
-int gsmi_tile_based_vol_rendering_sh(int64_t V, int64_t P, const float *mean, const float *cov, const float *sh,
-                                     int64_t sh_stride_g, int64_t sh_stride_c, const int32_t *sh_index,
-                                     const float *alpha, const int32_t *start, const int32_t *end,
-                                     const int32_t *gaussian_ids, float *out, const float *topleft,
-                                     const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
-                                     float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh,
-                                     const float *bg, float *T_final, int32_t *n_done, float *pre_bg, int flags,
-                                     int exact, void *stream);
+int gsmi_render_sh_forward(int64_t V, int64_t P, const float *mean, const float *cov, const float *sh,
+                           int64_t sh_stride_g, int64_t sh_stride_c, const int32_t *sh_index, const float *alpha,
+                           const int32_t *start, const int32_t *end, const int32_t *gaussian_ids, float *out,
+                           const float *topleft, const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+                           float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh,
+                           const float *bg, float *T_final, int32_t *n_done, float *pre_bg, int flags, int exact,
+                           void *stream);
+
+/* flags bit of the SH backward entries: reduce the SH gradients (orders 3
+ * and 4) on bf16 matrix cores as a three-term split (products with relative
+ * error up to 2^-15, biased toward zero) instead of fp32.  Off by default;
+ * not fp32 arithmetic. */
+#define GSMI_BWD_SPLIT_BF16 1
 
 /*
- * Replaces tile_based_vol_rendering_backward_sh (render.cu:546-624,
- * vol_render_sh.h:268-476; bound at bindings.cpp:44) and its _with_bg
- * variant (vol_render_bg.h:121-258).  Reverse-order adjoint: starts from the
- * forward's T_final and n_done, recomputes transmittance backwards, reduces
- * per-Gaussian gradients across each wavefront, then accumulates them
- * (float atomics) into grad_mean [V,2], grad_cov [V,4] (both off-diagonals
- * receive 0.5 * dL/dcov_offdiag, as the reference), grad_sh (same addressing
- * as sh, strides sh_stride_g / sh_stride_c through sh_index) and grad_alpha
- * [V] (w.r.t. the post-sigmoid alpha, straight through the 0.99 clamp).
- * out is the forward's output (for bg, the blended colour) and pre_bg the
- * colour before blending (bg mode only).  grad_* must be zero-initialised or
- * hold values to accumulate onto.
- */
-int gsmi_tile_based_vol_rendering_backward_sh(
-    int64_t V, int64_t P, const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
-    int64_t sh_stride_c, const int32_t *sh_index, const float *alpha, const int32_t *start, const int32_t *end,
-    const int32_t *gaussian_ids, const float *out, const float *T_final, const int32_t *n_done,
-    const float *pre_bg, const float *grad_out, float *grad_mean, float *grad_cov, float *grad_sh,
-    float *grad_alpha, const float *topleft, const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
-    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float *bg, void *stream);
-
-/*
- * Same as gsmi_tile_based_vol_rendering_backward_sh, with a workspace of
+ * Extended SH backward (reverse-order adjoint).  With the forward's T_final
+ * and n_done it recomputes the transmittance backwards from them; with
+ * T_final = n_done = NULL it recomputes the forward state front to back
+ * inside the launch first (the reference-shaped entries above).  Per-Gaussian
+ * gradients are reduced within each wavefront, then accumulated (float
+ * atomics) into grad_mean [V,2], grad_cov [V,4], grad_sh (same addressing as
+ * sh) and grad_alpha [V].  out is the forward's output (read with bg only:
+ * the blended colour) and pre_bg the colour before blending (bg with saved
+ * state only).  flags: 0 or GSMI_BWD_SPLIT_BF16 (saved state only).
+ * workspace (saved state only; NULL for none) of
  * gsmi_backward_order_workspace_size(n_tiles_h * n_tiles_w) bytes: the tiles
  * are launched most expensive first (cost from n_done), which shortens the
- * launch's tail.  No counterpart in the reference (a scheduling variant of
- * the entry above; the gradients are the same).
+ * launch's tail; the gradients are the same.  No counterpart in the
+ * reference beyond the entries above.
  */
 size_t gsmi_backward_order_workspace_size(int n_tiles_total);
-int gsmi_tile_based_vol_rendering_backward_sh_ws(
-    int64_t V, int64_t P, const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
-    int64_t sh_stride_c, const int32_t *sh_index, const float *alpha, const int32_t *start, const int32_t *end,
-    const int32_t *gaussian_ids, const float *out, const float *T_final, const int32_t *n_done,
-    const float *pre_bg, const float *grad_out, float *grad_mean, float *grad_cov, float *grad_sh,
-    float *grad_alpha, const float *topleft, const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
-    float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh, const float *bg, void *workspace,
-    size_t workspace_bytes, void *stream);
+int gsmi_render_sh_backward(int64_t V, int64_t P, const float *mean, const float *cov, const float *sh,
+                            int64_t sh_stride_g, int64_t sh_stride_c, const int32_t *sh_index, const float *alpha,
+                            const int32_t *start, const int32_t *end, const int32_t *gaussian_ids, const float *out,
+                            const float *T_final, const int32_t *n_done, const float *pre_bg,
+                            const float *grad_out, float *grad_mean, float *grad_cov, float *grad_sh,
+                            float *grad_alpha, const float *topleft, const float *c2w, int tile_size,
+                            int n_tiles_h, int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W,
+                            int C, float thresh, const float *bg, int flags, void *workspace,
+                            size_t workspace_bytes, void *stream);
+
+/*
+ * Extended RGB compositing (render_start_end), called by the reference-shaped
+ * entries above: the forward with optional T_final / n_done outputs and flags
+ * bit 0 (write empty tiles' pixels); the backward from saved T_final / n_done
+ * or, with both NULL, recomputing them inside the launch.
+ */
+int gsmi_render_rgb_forward(int64_t V, int64_t P, const float *mean, const float *cov, const float *color,
+                            const float *alpha, const int32_t *start, const int32_t *end,
+                            const int32_t *gaussian_ids, float *out, const float *topleft, int tile_size,
+                            int n_tiles_h, int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W,
+                            float thresh, float *T_final, int32_t *n_done, int flags, void *stream);
+int gsmi_render_rgb_backward(int64_t V, int64_t P, const float *mean, const float *cov, const float *color,
+                             const float *alpha, const int32_t *start, const int32_t *end,
+                             const int32_t *gaussian_ids, const float *out, const float *T_final,
+                             const int32_t *n_done, const float *grad_out, float *grad_mean, float *grad_cov,
+                             float *grad_color, float *grad_alpha, const float *topleft, int tile_size,
+                             int n_tiles_h, int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W,
+                             float thresh, void *stream);
 
 /*
  * Two-level lists (no counterpart in the reference; single view, no
@@ -256,7 +332,7 @@ int gsmi_tile_based_vol_rendering_backward_sh_ws(
  * flags, or over bounds shifted by gsmi_shift_rects),
  * and the forward filters each tile's list from its block's depth-ordered
  * list on the fly: the filtered list equals the tile's own list, so images,
- * T_final and n_done are those of gsmi_tile_based_vol_rendering_sh.  The
+ * T_final and n_done are those of gsmi_render_sh_forward.  The
  * forward records each tile's filtered entries (up to where its pixels
  * saturated) in fids[foff[t] .. fend[t]), so the backward takes (foff, fend,
  * fids) as its start / end / gaussian_ids.  foff: exclusive prefix of the
@@ -275,27 +351,6 @@ int gsmi_render_sh_two_level(int64_t V, const float *mean, const float *cov, con
                              const float *topleft, const float *c2w, int n_tiles_h, int n_tiles_w,
                              float pixel_size_x, float pixel_size_y, int H, int W, int C, float thresh,
                              float *T_final, int32_t *n_done, int exact, void *stream);
-
-/*
- * Replaces tile_based_vol_rendering_start_end (render.cu:399-441,
- * vol_render.h:716-798; bound at bindings.cpp:31): RGB compositing with the
- * Gaussian evaluated in double (kernels.h:195-224).  color [V,3].
- */
-int gsmi_tile_based_vol_rendering_start_end(int64_t V, int64_t P, const float *mean, const float *cov,
-                                            const float *color, const float *alpha, const int32_t *start,
-                                            const int32_t *end, const int32_t *gaussian_ids, float *out,
-                                            const float *topleft, int tile_size, int n_tiles_h, int n_tiles_w,
-                                            float pixel_size_x, float pixel_size_y, int H, int W, float thresh,
-                                            float *T_final, int32_t *n_done, int flags, void *stream);
-
-/* Replaces tile_based_vol_rendering_backward_start_end (render.cu:443-481,
- * vol_render.h:800-925; bound at bindings.cpp:35). */
-int gsmi_tile_based_vol_rendering_backward_start_end(
-    int64_t V, int64_t P, const float *mean, const float *cov, const float *color, const float *alpha,
-    const int32_t *start, const int32_t *end, const int32_t *gaussian_ids, const float *out,
-    const float *T_final, const int32_t *n_done, const float *grad_out, float *grad_mean, float *grad_cov,
-    float *grad_color, float *grad_alpha, const float *topleft, int tile_size, int n_tiles_h, int n_tiles_w,
-    float pixel_size_x, float pixel_size_y, int H, int W, float thresh, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Projection adjoint                                                        */
@@ -388,9 +443,10 @@ int gsmi_render_views_sh_two_level(int64_t V, int B, const gsmi_view *views, con
                                    float thresh, float *T_final, int32_t *n_done, int exact, void *stream);
 
 /* Reverse-order adjoint of gsmi_render_views_sh; gradients of records seen
- * in several views accumulate (SH rows through sh_index).  workspace (may be
- * null): gsmi_backward_order_workspace_size(n_tiles_total) bytes to launch
- * the tiles most expensive first. */
+ * in several views accumulate (SH rows through sh_index).  T_final, n_done,
+ * pre_bg, flags and workspace as for gsmi_render_sh_backward (workspace of
+ * gsmi_backward_order_workspace_size(n_tiles_total) bytes: the tiles most
+ * expensive first). */
 int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view *views, int n_tiles_total,
                                   const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
                                   int64_t sh_stride_c, const int32_t *sh_index, const float *alpha,
@@ -398,7 +454,7 @@ int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view *
                                   const float *out, const float *T_final, const int32_t *n_done,
                                   const float *pre_bg, const float *grad_out, float *grad_mean, float *grad_cov,
                                   float *grad_sh, float *grad_alpha, int C, float thresh, const float *bg,
-                                  void *workspace, size_t workspace_bytes, void *stream);
+                                  int flags, void *workspace, size_t workspace_bytes, void *stream);
 
 /* Projection adjoint over all views: per Gaussian the gradients of every
  * view that sees it are summed in view order, then chained to the raw

@@ -25,6 +25,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_MAX_C 4
 
@@ -200,6 +203,8 @@ static void orc_pixel(const float *topleft, float psx, float psy, int gx,
  * Optional outputs (may be NULL): T_final[H*W] (transmittance after the loop),
  * n_done[H*W] (number of list entries examined before the T<thresh break),
  * pre_bg[H*W*3] (colour before the background blend).
+ * Tiles are independent (each writes only its own pixels), so they run in
+ * parallel (OpenMP); the result does not depend on the thread count.
  * Returns 0, or -1 for an unsupported C.
  */
 int orc_render_sh_fwd(int C, const float *mean, const float *cov,
@@ -211,67 +216,67 @@ int orc_render_sh_fwd(int C, const float *mean, const float *cov,
                       float *T_final, int *n_done, float *pre_bg) {
   if (C < 1 || C > ORC_MAX_C) return -1;
   const int CC = C * C;
-  for (int ty = 0; ty < n_tiles_h; ++ty) {
-    for (int tx = 0; tx < n_tiles_w; ++tx) {
-      const int tile = ty * n_tiles_w + tx;
-      const int s0 = start[tile];
-      const int empty = (s0 == -1) || (end[tile] - s0 == 0);
-      for (int ly = 0; ly < tile_size; ++ly) {
-        for (int lx = 0; lx < tile_size; ++lx) {
-          const int gx = tx * tile_size + lx, gy = ty * tile_size + ly;
-          if (gx >= W || gy >= H) continue;
-          const size_t pix = (size_t)gy * W + gx;
-          if (empty) {
-            if (mode == 1 && s0 == -1) {
-              out[3 * pix + 0] = bg[0];
-              out[3 * pix + 1] = bg[1];
-              out[3 * pix + 2] = bg[2];
-            }
-            if (T_final) T_final[pix] = 1.0f;
-            if (n_done) n_done[pix] = 0;
-            if (pre_bg) pre_bg[3 * pix] = pre_bg[3 * pix + 1] = pre_bg[3 * pix + 2] = 0.0f;
-            continue;
+  const int n_tiles = n_tiles_h * n_tiles_w;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int tile = 0; tile < n_tiles; ++tile) {
+    const int ty = tile / n_tiles_w, tx = tile % n_tiles_w;
+    const int s0 = start[tile];
+    const int empty = (s0 == -1) || (end[tile] - s0 == 0);
+    for (int ly = 0; ly < tile_size; ++ly) {
+      for (int lx = 0; lx < tile_size; ++lx) {
+        const int gx = tx * tile_size + lx, gy = ty * tile_size + ly;
+        if (gx >= W || gy >= H) continue;
+        const size_t pix = (size_t)gy * W + gx;
+        if (empty) {
+          if (mode == 1 && s0 == -1) {
+            out[3 * pix + 0] = bg[0];
+            out[3 * pix + 1] = bg[1];
+            out[3 * pix + 2] = bg[2];
           }
-          const int n = end[tile] - s0;
-          float pos[2], dir[3], Y[16];
-          orc_pixel(topleft, psx, psy, gx, gy, pos);
-          orc_direction(c2w, pos[0], pos[1], dir);
-          orc_sh_basis(dir, Y, C);
-          float o[3] = {0.0f, 0.0f, 0.0f};
-          float T = 1.0f;
-          int i = 0;
-          for (; i < n; ++i) {
-            if (T < thresh) break;
-            const int g = ids[s0 + i];
-            const float a = fminf(alpha[g], 0.99f);
-            float coeff = a * T;
-            const float G = orc_gauss2d_f(mean + 2 * g, cov + 4 * g, pos[0], pos[1]);
-            coeff *= G;
-            if (a * G < kMinRenderAlpha) continue;
-            if (isnan(coeff)) coeff = 0.0f;
-            float y[3];
-            for (int ch = 0; ch < 3; ++ch)
-              y[ch] = orc_sigmoid(orc_dot(sh + (size_t)(3 * g + ch) * CC, Y, CC));
-            for (int ch = 0; ch < 3; ++ch)
-              if (isnan(y[ch] * coeff)) y[ch] = 0.0f;
-            for (int ch = 0; ch < 3; ++ch) o[ch] += coeff * y[ch];
-            T *= (1 - a * G);
-          }
-          if (pre_bg) {
-            pre_bg[3 * pix] = o[0];
-            pre_bg[3 * pix + 1] = o[1];
-            pre_bg[3 * pix + 2] = o[2];
-          }
-          if (mode == 1 && T > thresh) {
-            /* vol_render_bg.h:90-94 */
-            for (int ch = 0; ch < 3; ++ch) o[ch] = o[ch] * T + bg[ch] * (1.0f - T);
-          }
-          out[3 * pix + 0] = o[0];
-          out[3 * pix + 1] = o[1];
-          out[3 * pix + 2] = o[2];
-          if (T_final) T_final[pix] = T;
-          if (n_done) n_done[pix] = i;
+          if (T_final) T_final[pix] = 1.0f;
+          if (n_done) n_done[pix] = 0;
+          if (pre_bg) pre_bg[3 * pix] = pre_bg[3 * pix + 1] = pre_bg[3 * pix + 2] = 0.0f;
+          continue;
         }
+        const int n = end[tile] - s0;
+        float pos[2], dir[3], Y[16];
+        orc_pixel(topleft, psx, psy, gx, gy, pos);
+        orc_direction(c2w, pos[0], pos[1], dir);
+        orc_sh_basis(dir, Y, C);
+        float o[3] = {0.0f, 0.0f, 0.0f};
+        float T = 1.0f;
+        int i = 0;
+        for (; i < n; ++i) {
+          if (T < thresh) break;
+          const int g = ids[s0 + i];
+          const float a = fminf(alpha[g], 0.99f);
+          float coeff = a * T;
+          const float G = orc_gauss2d_f(mean + 2 * g, cov + 4 * g, pos[0], pos[1]);
+          coeff *= G;
+          if (a * G < kMinRenderAlpha) continue;
+          if (isnan(coeff)) coeff = 0.0f;
+          float y[3];
+          for (int ch = 0; ch < 3; ++ch)
+            y[ch] = orc_sigmoid(orc_dot(sh + (size_t)(3 * g + ch) * CC, Y, CC));
+          for (int ch = 0; ch < 3; ++ch)
+            if (isnan(y[ch] * coeff)) y[ch] = 0.0f;
+          for (int ch = 0; ch < 3; ++ch) o[ch] += coeff * y[ch];
+          T *= (1 - a * G);
+        }
+        if (pre_bg) {
+          pre_bg[3 * pix] = o[0];
+          pre_bg[3 * pix + 1] = o[1];
+          pre_bg[3 * pix + 2] = o[2];
+        }
+        if (mode == 1 && T > thresh) {
+          /* vol_render_bg.h:90-94 */
+          for (int ch = 0; ch < 3; ++ch) o[ch] = o[ch] * T + bg[ch] * (1.0f - T);
+        }
+        out[3 * pix + 0] = o[0];
+        out[3 * pix + 1] = o[1];
+        out[3 * pix + 2] = o[2];
+        if (T_final) T_final[pix] = T;
+        if (n_done) n_done[pix] = i;
       }
     }
   }
@@ -285,6 +290,12 @@ int orc_render_sh_fwd(int C, const float *mean, const float *cov,
  * the blended output, vol_render_bg.h:183-188).  Gradients are ADDED into the
  * grad_* arrays (callers pass zeros, renderer.py:760-763).  grad_cov receives
  * both off-diagonals, each 0.5 * P*a*G * s*u.
+ * Optional abs_* outputs (double, may be NULL): for every gradient component
+ * the sum of the absolute values of the per-pixel terms that make it up, the
+ * scale of the rounding error any float summation of those terms can have.
+ * Sums are formed in double, per tile over its pixels, and added to the
+ * per-Gaussian totals in tile order, so the result does not depend on the
+ * number of threads the tiles run on.
  * Returns 0, -1 for unsupported C, 1 if the recomputed colour differs from
  * out_saved (the reference asserts equality, vol_render_sh.h:452-454).
  */
@@ -295,93 +306,164 @@ int orc_render_sh_bwd(int C, const float *mean, const float *cov,
                       float *grad_alpha, const float *grad_out,
                       const float *topleft, const float *c2w, int tile_size,
                       int n_tiles_h, int n_tiles_w, float psx, float psy, int H,
-                      int W, float thresh, int mode, const float *bg, int V) {
+                      int W, float thresh, int mode, const float *bg, int V,
+                      double *abs_mean, double *abs_cov, double *abs_sh,
+                      double *abs_alpha) {
   if (C < 1 || C > ORC_MAX_C) return -1;
   const int CC = C * C;
-  double *acc_mean = calloc((size_t)V * 2, sizeof(double));
-  double *acc_cov = calloc((size_t)V * 4, sizeof(double));
-  double *acc_sh = calloc((size_t)V * 3 * CC, sizeof(double));
-  double *acc_alpha = calloc((size_t)V, sizeof(double));
+  /* per entry: mean 2, cov 4, sh 3*CC, alpha 1 */
+  const int NC = 7 + 3 * CC;
+  const int n_tiles = n_tiles_h * n_tiles_w;
+  int max_n = 0;
+  for (int t = 0; t < n_tiles; ++t)
+    if (start[t] != -1 && end[t] - start[t] > max_n) max_n = end[t] - start[t];
+  double *acc = calloc((size_t)V * NC, sizeof(double));
+  double *acc_abs = calloc((size_t)V * NC, sizeof(double));
   int mismatch = 0;
-  for (int ty = 0; ty < n_tiles_h; ++ty) {
-    for (int tx = 0; tx < n_tiles_w; ++tx) {
-      const int tile = ty * n_tiles_w + tx;
+#pragma omp parallel reduction(| : mismatch)
+  {
+    /* this thread's per-tile sums, indexed by list position */
+    double *loc = calloc((size_t)(max_n > 0 ? max_n : 1) * NC, sizeof(double));
+    double *loc_abs = calloc((size_t)(max_n > 0 ? max_n : 1) * NC, sizeof(double));
+#pragma omp for ordered schedule(dynamic, 1)
+    for (int tile = 0; tile < n_tiles; ++tile) {
+      const int ty = tile / n_tiles_w, tx = tile % n_tiles_w;
       const int s0 = start[tile];
-      if (s0 == -1 || end[tile] - s0 == 0) continue;
-      const int n = end[tile] - s0;
-      for (int ly = 0; ly < tile_size; ++ly) {
-        for (int lx = 0; lx < tile_size; ++lx) {
-          const int gx = tx * tile_size + lx, gy = ty * tile_size + ly;
-          if (gx >= W || gy >= H) continue;
-          const size_t pix = (size_t)gy * W + gx;
-          float pos[2], dir[3], Y[16];
-          orc_pixel(topleft, psx, psy, gx, gy, pos);
-          orc_direction(c2w, pos[0], pos[1], dir);
-          orc_sh_basis(dir, Y, C);
-          const float *g = grad_out + 3 * pix;
-          const float *F = out_saved + 3 * pix;
-          float o[3] = {0.0f, 0.0f, 0.0f};
-          float T = 1.0f;
-          for (int i = 0; i < n; ++i) {
-            if (T < thresh) break;
-            const int gid = ids[s0 + i];
-            const float a = fminf(alpha[gid], 0.99f);
-            const float G = orc_gauss2d_f(mean + 2 * gid, cov + 4 * gid, pos[0], pos[1]);
-            if (a * G < kMinRenderAlpha) continue;
-            float coeff = a * T * G;
-            if (isnan(coeff)) coeff = 0.0f;
-            float y[3];
-            for (int ch = 0; ch < 3; ++ch)
-              y[ch] = orc_sigmoid(orc_dot(sh + (size_t)(3 * gid + ch) * CC, Y, CC));
-            for (int ch = 0; ch < 3; ++ch)
-              if (isnan(y[ch] * coeff)) y[ch] = 0.0f;
-            for (int ch = 0; ch < 3; ++ch) o[ch] += coeff * y[ch];
-            /* backward_C: grad_sh += (coeff * y(1-y) * g) * Y_k */
-            for (int ch = 0; ch < 3; ++ch) {
-              const float gr = coeff * (y[ch] * (1.0f - y[ch])) * g[ch];
-              double *dst = acc_sh + (size_t)(3 * gid + ch) * CC;
-              for (int k = 0; k < CC; ++k) dst[k] += (double)(gr * Y[k]);
+      int used = 0; /* entries of the tile list with a term */
+      if (s0 != -1 && end[tile] - s0 != 0) {
+        const int n = end[tile] - s0;
+        for (int ly = 0; ly < tile_size; ++ly) {
+          for (int lx = 0; lx < tile_size; ++lx) {
+            const int gx = tx * tile_size + lx, gy = ty * tile_size + ly;
+            if (gx >= W || gy >= H) continue;
+            const size_t pix = (size_t)gy * W + gx;
+            float pos[2], dir[3], Y[16];
+            orc_pixel(topleft, psx, psy, gx, gy, pos);
+            orc_direction(c2w, pos[0], pos[1], dir);
+            orc_sh_basis(dir, Y, C);
+            const float *g = grad_out + 3 * pix;
+            const float *F = out_saved + 3 * pix;
+            float o[3] = {0.0f, 0.0f, 0.0f};
+            float T = 1.0f;
+            for (int i = 0; i < n; ++i) {
+              if (T < thresh) break;
+              const int gid = ids[s0 + i];
+              const float a = fminf(alpha[gid], 0.99f);
+              const float G = orc_gauss2d_f(mean + 2 * gid, cov + 4 * gid, pos[0], pos[1]);
+              if (a * G < kMinRenderAlpha) continue;
+              float coeff = a * T * G;
+              if (isnan(coeff)) coeff = 0.0f;
+              float y[3];
+              for (int ch = 0; ch < 3; ++ch)
+                y[ch] = orc_sigmoid(orc_dot(sh + (size_t)(3 * gid + ch) * CC, Y, CC));
+              for (int ch = 0; ch < 3; ++ch)
+                if (isnan(y[ch] * coeff)) y[ch] = 0.0f;
+              for (int ch = 0; ch < 3; ++ch) o[ch] += coeff * y[ch];
+              double *L = loc + (size_t)i * NC, *LA = loc_abs + (size_t)i * NC;
+              if (i + 1 > used) used = i + 1;
+              /* backward_C: grad_sh += (coeff * y(1-y) * g) * Y_k */
+              for (int ch = 0; ch < 3; ++ch) {
+                const float gr = coeff * (y[ch] * (1.0f - y[ch])) * g[ch];
+                for (int k = 0; k < CC; ++k) {
+                  const double v = (double)(gr * Y[k]);
+                  L[6 + ch * CC + k] += v;
+                  LA[6 + ch * CC + k] += fabs(v);
+                }
+              }
+              float paG = 0.0f;
+              for (int ch = 0; ch < 3; ++ch)
+                paG += g[ch] * (y[ch] * T - (F[ch] - o[ch]) / (1 - a * G));
+              /* kernels.h:394-418 in double */
+              {
+                const float *m = mean + 2 * gid, *c = cov + 4 * gid;
+                const double dg = (double)(paG * a * G);
+                const double c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+                const double det = c0 * c3 - c1 * c2;
+                const double x = (double)(pos[0] - m[0]);
+                const double yy = (double)(pos[1] - m[1]);
+                const double tmpx = (x * c3 - yy * c2) / det;
+                const double tmpy = (-x * c1 + yy * c0) / det;
+                double v[6];
+                v[0] = (double)(float)(dg * tmpx);
+                v[1] = (double)(float)(dg * tmpy);
+                v[2] = 0.5 * (double)(float)(dg * tmpx * tmpx);
+                v[3] = 0.5 * (double)(float)(dg * tmpx * tmpy);
+                v[4] = 0.5 * (double)(float)(dg * tmpy * tmpx);
+                v[5] = 0.5 * (double)(float)(dg * tmpy * tmpy);
+                for (int q = 0; q < 6; ++q) {
+                  L[q] += v[q];
+                  LA[q] += fabs(v[q]);
+                }
+              }
+              {
+                const double v = (double)(paG * G);
+                L[6 + 3 * CC] += v;
+                LA[6 + 3 * CC] += fabs(v);
+              }
+              T *= (1 - a * G);
             }
-            float paG = 0.0f;
+            if (mode == 1 && T > thresh)
+              for (int ch = 0; ch < 3; ++ch) o[ch] = o[ch] * T + bg[ch] * (1.0f - T);
             for (int ch = 0; ch < 3; ++ch)
-              paG += g[ch] * (y[ch] * T - (F[ch] - o[ch]) / (1 - a * G));
-            /* kernels.h:394-418 in double */
-            {
-              const float *m = mean + 2 * gid, *c = cov + 4 * gid;
-              const double dg = (double)(paG * a * G);
-              const double c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
-              const double det = c0 * c3 - c1 * c2;
-              const double x = (double)(pos[0] - m[0]);
-              const double yy = (double)(pos[1] - m[1]);
-              const double tmpx = (x * c3 - yy * c2) / det;
-              const double tmpy = (-x * c1 + yy * c0) / det;
-              acc_mean[2 * gid + 0] += (double)(float)(dg * tmpx);
-              acc_mean[2 * gid + 1] += (double)(float)(dg * tmpy);
-              acc_cov[4 * gid + 0] += 0.5 * (double)(float)(dg * tmpx * tmpx);
-              acc_cov[4 * gid + 1] += 0.5 * (double)(float)(dg * tmpx * tmpy);
-              acc_cov[4 * gid + 2] += 0.5 * (double)(float)(dg * tmpy * tmpx);
-              acc_cov[4 * gid + 3] += 0.5 * (double)(float)(dg * tmpy * tmpy);
-            }
-            acc_alpha[gid] += (double)(paG * G);
-            T *= (1 - a * G);
+              if (o[ch] != F[ch]) mismatch = 1;
           }
-          if (mode == 1 && T > thresh)
-            for (int ch = 0; ch < 3; ++ch) o[ch] = o[ch] * T + bg[ch] * (1.0f - T);
-          for (int ch = 0; ch < 3; ++ch)
-            if (o[ch] != F[ch]) mismatch = 1;
+        }
+      }
+#pragma omp ordered
+      {
+        for (int i = 0; i < used; ++i) {
+          const int gid = ids[s0 + i];
+          double *A = acc + (size_t)gid * NC, *AA = acc_abs + (size_t)gid * NC;
+          double *L = loc + (size_t)i * NC, *LA = loc_abs + (size_t)i * NC;
+          for (int q = 0; q < NC; ++q) {
+            A[q] += L[q];
+            AA[q] += LA[q];
+            L[q] = 0.0;
+            LA[q] = 0.0;
+          }
         }
       }
     }
+    free(loc);
+    free(loc_abs);
   }
-  for (size_t i = 0; i < (size_t)V * 2; ++i) grad_mean[i] += (float)acc_mean[i];
-  for (size_t i = 0; i < (size_t)V * 4; ++i) grad_cov[i] += (float)acc_cov[i];
-  for (size_t i = 0; i < (size_t)V * 3 * CC; ++i) grad_sh[i] += (float)acc_sh[i];
-  for (size_t i = 0; i < (size_t)V; ++i) grad_alpha[i] += (float)acc_alpha[i];
-  free(acc_mean);
-  free(acc_cov);
-  free(acc_sh);
-  free(acc_alpha);
+  for (size_t v = 0; v < (size_t)V; ++v) {
+    const double *A = acc + v * NC, *AA = acc_abs + v * NC;
+    grad_mean[2 * v + 0] += (float)A[0];
+    grad_mean[2 * v + 1] += (float)A[1];
+    for (int q = 0; q < 4; ++q) grad_cov[4 * v + q] += (float)A[2 + q];
+    for (int q = 0; q < 3 * CC; ++q) grad_sh[v * 3 * CC + q] += (float)A[6 + q];
+    grad_alpha[v] += (float)A[6 + 3 * CC];
+    if (abs_mean) {
+      abs_mean[2 * v + 0] += AA[0];
+      abs_mean[2 * v + 1] += AA[1];
+    }
+    if (abs_cov)
+      for (int q = 0; q < 4; ++q) abs_cov[4 * v + q] += AA[2 + q];
+    if (abs_sh)
+      for (int q = 0; q < 3 * CC; ++q) abs_sh[v * 3 * CC + q] += AA[6 + q];
+    if (abs_alpha) abs_alpha[v] += AA[6 + 3 * CC];
+  }
+  free(acc);
+  free(acc_abs);
   return mismatch;
+}
+
+/* Threads the parallel oracle loops use (OpenMP); n <= 0 keeps the default. */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 /*
@@ -526,6 +608,7 @@ int orc_render_rgb_bwd(const float *mean, const float *cov, const float *color,
  */
 void orc_cull(int N, const float *mean, const float *svec, const float *normals,
               const float *pts, uint8_t *mask, float thresh) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < N; ++i) {
     const float *m = mean + 3 * i, *s = svec + 3 * i;
     const float r = fmaxf(fmaxf(s[0], s[1]), s[2]) * thresh;
@@ -593,6 +676,7 @@ void orc_project(int V, const float *mean, const float *qvec, const float *svec,
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) Wm[3 * i + j] = c2w[4 * j + i];
   const float t0 = c2w[3], t1 = c2w[7], t2 = c2w[11];
+#pragma omp parallel for schedule(static)
   for (int v = 0; v < V; ++v) {
     const float *m = mean + 3 * v;
     const float d0 = m[0] + (-t0), d1 = m[1] + (-t1), d2 = m[2] + (-t2);
@@ -643,6 +727,7 @@ int64_t orc_aabb(int V, const float *mean2d, const float *cov2d, float fx,
                  float fy, float cx, float cy, int W, int H, float D,
                  int tile_size, int32_t *tl, int32_t *br) {
   int64_t P = 0;
+#pragma omp parallel for schedule(static) reduction(+ : P)
   for (int v = 0; v < V; ++v) {
     const float ex = sqrtf(D * cov2d[4 * v + 0]);
     const float ey = sqrtf(D * cov2d[4 * v + 3]);

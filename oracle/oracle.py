@@ -55,7 +55,10 @@ def lib():
         L.orc_render_sh_fwd.restype = i32
         L.orc_render_sh_fwd.argtypes = [i32] + [P] * 10 + [i32, i32, i32, fl, fl, i32, i32, fl, i32, P, P, P, P]
         L.orc_render_sh_bwd.restype = i32
-        L.orc_render_sh_bwd.argtypes = [i32] + [P] * 15 + [i32, i32, i32, fl, fl, i32, i32, fl, i32, P, i32]
+        L.orc_render_sh_bwd.argtypes = [i32] + [P] * 15 + [i32, i32, i32, fl, fl, i32, i32, fl, i32, P, i32, P, P,
+                                                           P, P]
+        L.orc_set_threads.argtypes = [i32]
+        L.orc_get_threads.restype = i32
         L.orc_render_rgb_fwd.restype = i32
         L.orc_render_rgb_fwd.argtypes = [P] * 9 + [i32, i32, i32, fl, fl, i32, i32, fl, P, P]
         L.orc_render_rgb_bwd.restype = i32
@@ -335,8 +338,11 @@ def render_sh_fwd(mean, cov, sh, alpha, start, end, ids, topleft, c2w, tile_size
 
 
 def render_sh_bwd(mean, cov, sh, alpha, start, end, ids, out, grad_out, topleft, c2w, tile_size,
-                  n_tiles_h, n_tiles_w, psx, psy, H, W, C, thresh, mode=0, bg=None):
-    """Reference backward (forward-order recompute).  Returns (grads dict, mismatch flag)."""
+                  n_tiles_h, n_tiles_w, psx, psy, H, W, C, thresh, mode=0, bg=None, want_abs=False):
+    """Reference backward (forward-order recompute).  Returns (grads dict,
+    mismatch flag); with want_abs, (grads, mismatch, abs) where abs holds,
+    per gradient component, the sum of |per-pixel term| (float64): the scale
+    of the rounding error of any float summation of that component."""
     m = _c(mean, f32)
     c = _c(cov, f32)
     s = _c(sh, f32)
@@ -347,14 +353,31 @@ def render_sh_bwd(mean, cov, sh, alpha, start, end, ids, out, grad_out, topleft,
     gs = np.zeros(s.size, f32)
     ga = np.zeros(V, f32)
     b = _c(bg if bg is not None else [0, 0, 0], f32)
+    am = np.zeros(V * 2) if want_abs else None
+    ac = np.zeros(V * 4) if want_abs else None
+    ash = np.zeros(s.size) if want_abs else None
+    aa = np.zeros(V) if want_abs else None
     rc = lib().orc_render_sh_bwd(C, _p(m), _p(c), _p(s), _p(a), _p(_c(start, np.int32)), _p(_c(end, np.int32)),
                                  _p(_c(ids, np.int32)), _p(_c(out, f32)), _p(gm), _p(gc), _p(gs), _p(ga),
                                  _p(_c(grad_out, f32)), _p(_c(topleft, f32)), _p(_c(c2w, f32)), tile_size,
-                                 n_tiles_h, n_tiles_w, f32(psx), f32(psy), H, W, f32(thresh), mode, _p(b), V)
+                                 n_tiles_h, n_tiles_w, f32(psx), f32(psy), H, W, f32(thresh), mode, _p(b), V,
+                                 _p(am), _p(ac), _p(ash), _p(aa))
     if rc < 0:
         raise RuntimeError("oracle render_sh_bwd: unsupported C")
     grads = dict(mean=gm.reshape(V, 2), cov=gc.reshape(V, 2, 2), sh=gs.reshape(np.shape(sh)), alpha=ga)
-    return grads, rc
+    if not want_abs:
+        return grads, rc
+    absd = dict(mean=am.reshape(V, 2), cov=ac.reshape(V, 2, 2), sh=ash.reshape(np.shape(sh)), alpha=aa)
+    return grads, rc, absd
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel compositing loops (OpenMP)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())
 
 
 def render_rgb_fwd(mean, cov, color, alpha, start, end, ids, topleft, tile_size, n_tiles_h, n_tiles_w,

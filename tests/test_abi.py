@@ -82,12 +82,30 @@ def test_argument_validation_without_gpu(lib):
     assert lib.gsmi_tile_ranges(0, None, 0, None, None, None) == 1
     P = ctypes.c_void_p(16)
     # C = 5 is outside the supported SH orders
-    rc = lib.gsmi_tile_based_vol_rendering_sh(1, 1, P, P, P, 75, 25, None, P, P, P, P, P, P, P, 16, 1, 1, 1.0, 1.0,
-                                              16, 16, 5, 1e-4, None, None, None, None, 0, 0, None)
+    rc = lib.gsmi_render_sh_forward(1, 1, P, P, P, 75, 25, None, P, P, P, P, P, P, P, 16, 1, 1, 1.0, 1.0,
+                                    16, 16, 5, 1e-4, None, None, None, None, 0, 0, None)
     assert rc == 2
-    rc = lib.gsmi_tile_based_vol_rendering_sh(1, 1, P, P, P, 3, 1, None, P, P, P, P, P, P, P, 8, 1, 1, 1.0, 1.0,
-                                              16, 16, 1, 1e-4, None, None, None, None, 0, 0, None)
+    rc = lib.gsmi_render_sh_forward(1, 1, P, P, P, 3, 1, None, P, P, P, P, P, P, P, 8, 1, 1, 1.0, 1.0,
+                                    16, 16, 1, 1e-4, None, None, None, None, 0, 0, None)
     assert rc == 2  # tile size 8
+    # the reference-shaped entries: C = 5 and tile size 8 as above
+    assert lib.gsmi_tile_based_vol_rendering_sh(1, 1, P, P, P, P, P, P, P, P, P, P, 16, 1, 1, 1.0, 1.0, 16, 16, 5,
+                                                1e-4, None) == 2
+    assert lib.gsmi_tile_based_vol_rendering_backward_sh(1, 1, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, 8, 1, 1,
+                                                         1.0, 1.0, 16, 16, 1, 1e-4, None) == 2
+    assert lib.gsmi_tile_based_vol_rendering_backward_start_end(1, 1, P, P, P, P, P, P, P, P, P, P, P, P, P, P, 16,
+                                                                0, 1, 1.0, 1.0, 16, 16, 1e-4, None) == 1
+
+    def bwd(T, nd, pre, bg, flags, ws, nws):
+        return lib.gsmi_render_sh_backward(1, 1, P, P, P, 48, 16, None, P, P, P, P, P, T, nd, pre, P, P, P, P, P, P,
+                                           P, 16, 1, 1, 1.0, 1.0, 16, 16, 4, 1e-4, bg, flags, ws, nws, None)
+
+    assert bwd(P, None, None, None, 0, None, 0) == 1  # T_final without n_done
+    assert bwd(None, P, None, None, 0, None, 0) == 1
+    assert bwd(None, None, None, None, 1, None, 0) == 2  # the bf16 split needs saved state
+    assert bwd(None, None, None, None, 0, P, 64) == 1  # the launch order needs saved n_done
+    assert bwd(P, P, None, None, 4, None, 0) == 1  # undefined flag bit
+    assert bwd(P, P, None, P, 0, None, 0) == 1  # background with saved state needs pre_bg
 
 
 def test_make_view_matches_python_records(lib):

@@ -387,8 +387,8 @@ def test_backward_boundary_gradients(orc, which, bg):
 
 @pytest.mark.parametrize("C", [1, 4])
 def test_backward_launch_order_entry_points_agree(C):
-    """The reference-shaped backward (image-order launch) and the entry that
-    launches the tiles most expensive first give the same gradients (to the
+    """The backward without a workspace (image-order launch) and with one
+    (tiles launched most expensive first) give the same gradients (to the
     float atomics' summation order); a short workspace is refused."""
     sc = small_sh_scene(C)
     c2w, cam = sc.cameras[0]
@@ -401,28 +401,28 @@ def test_backward_launch_order_entry_points_agree(C):
     ntiles = ic["n_tiles_h"] * ic["n_tiles_w"]
     cmax2 = p[4].shape[-1]
 
-    def run(entry, *ws):
+    def run(ws=None, nws=0):
         g = [torch.zeros(b.V, 2, device=DEV), torch.zeros(b.V, 4, device=DEV), torch.zeros_like(p[4]),
              torch.zeros(b.V, device=DEV)]
-        rc = getattr(L, entry)(
+        rc = L.gsmi_render_sh_backward(
             b.V, b.P, _lib.ptr(b.mean2d), _lib.ptr(b.cov2d), _lib.ptr(p[4]), 3 * cmax2, cmax2, _lib.ptr(b.orig),
             _lib.ptr(b.alpha), _lib.ptr(b.start), _lib.ptr(b.end), _lib.ptr(b.ids), _lib.ptr(st.out), _lib.ptr(st.T),
             _lib.ptr(st.n_done), None, _lib.ptr(gout), *[_lib.ptr(x) for x in g], _lib.ptr(st.topleft),
             _lib.ptr(st.c2w_d), 16, ic["n_tiles_h"], ic["n_tiles_w"], ic["pixel_size_x"], ic["pixel_size_y"],
-            ic["H"], ic["W"], sc.C, 1e-4, None, *ws, _lib.stream_of())
+            ic["H"], ic["W"], sc.C, 1e-4, None, 0, _lib.ptr(ws), nws, _lib.stream_of())
         return rc, g
 
-    rc0, g0 = run("gsmi_tile_based_vol_rendering_backward_sh")
+    rc0, g0 = run()
     nws = int(L.gsmi_backward_order_workspace_size(ntiles))
     assert nws >= 8 * ntiles
     ws = torch.empty(nws, dtype=torch.uint8, device=DEV)
-    rc1, g1 = run("gsmi_tile_based_vol_rendering_backward_sh_ws", _lib.ptr(ws), nws)
+    rc1, g1 = run(ws, nws)
     assert rc0 == 0 and rc1 == 0
     for a, w in zip(g1, g0):
         assert torch.isfinite(a).all()
         assert rel_err(a.cpu(), w.cpu()) <= 1e-5
     assert float(g0[0].abs().max()) > 0
-    rc2, _ = run("gsmi_tile_based_vol_rendering_backward_sh_ws", _lib.ptr(ws), nws - 1)
+    rc2, _ = run(ws, nws - 1)
     assert rc2 == 4  # GSMI_ERR_WORKSPACE
 
 

@@ -222,3 +222,44 @@ def test_rgb_variant_closed_form_and_gradient(orc):
     # dL/da1 with G = 1: d/da1 [a1 c1 + (1-a1) a2 c2] . g
     want = col[0] @ [1, 2, 3] - 0.5 * col[1] @ [1, 2, 3]
     assert abs(gr["alpha"][0] - want) < 1e-5
+
+
+def test_oracle_compositing_independent_of_thread_count(orc):
+    """The tiles run in parallel; the per-tile sums are added in tile order,
+    so images and gradients are bit-identical for any number of threads, and
+    the absolute-term sums bound each gradient component."""
+    from gaussian_splatting_3d_amd import scenes
+
+    sc = scenes.plain_forward_10k(n=3000)
+    c2w, cam = sc.cameras[0]
+    ocam = orc.Camera(cam.fx, cam.fy, cam.cx, cam.cy, cam.w, cam.h, cam.near_plane, cam.far_plane)
+    ref = orc.pipeline_forward(sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                               sc.sh_coeffs, c2w, ocam, sc.C)
+    gout = (np.random.default_rng(4).random(ref["out"].size).astype(np.float32) - 0.5)
+    runs = []
+    n0 = orc.get_threads()
+    try:
+        for n in (1, 3, 8):
+            orc.set_threads(n)
+            o, T, nd, _ = orc.render_sh_fwd(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]],
+                                            ref["start"], ref["end"], ref["ids"], ref["topleft"], c2w, 16,
+                                            ref["n_tiles_h"], ref["n_tiles_w"], ref["psx"], ref["psy"], ref["H"],
+                                            ref["W"], sc.C, 1e-4)
+            g, mism, ab = orc.render_sh_bwd(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]],
+                                            ref["start"], ref["end"], ref["ids"], o, gout, ref["topleft"], c2w, 16,
+                                            ref["n_tiles_h"], ref["n_tiles_w"], ref["psx"], ref["psy"], ref["H"],
+                                            ref["W"], sc.C, 1e-4, want_abs=True)
+            assert mism == 0
+            runs.append((o, T, nd, g, ab))
+    finally:
+        orc.set_threads(n0)
+    o0, T0, nd0, g0, ab0 = runs[0]
+    for o, T, nd, g, ab in runs[1:]:
+        assert np.array_equal(o.view(np.uint32), o0.view(np.uint32))
+        assert np.array_equal(T.view(np.uint32), T0.view(np.uint32)) and np.array_equal(nd, nd0)
+        for k in g0:
+            assert np.array_equal(g[k].view(np.uint32), g0[k].view(np.uint32)), k
+            assert np.array_equal(ab[k], ab0[k]), k
+    for k in g0:
+        assert (np.abs(g0[k].astype(np.float64)) <= ab0[k] * (1 + 1e-6) + 1e-30).all(), k
+    assert ab0["sh"].max() > 0
