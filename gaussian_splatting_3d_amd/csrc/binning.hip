@@ -892,7 +892,7 @@ extern "C" int gsmi_tile_culling_aabb_start_end(int64_t V, int64_t P, const int3
   if (((uintptr_t)tl & 7) || ((uintptr_t)br & 7)) return GSMI_ERR_ARG;
   if (P > 0 && !gaussian_ids) return GSMI_ERR_ARG;
   if (P >= (int64_t(1) << 31) || V >= (int64_t(1) << 31)) return GSMI_ERR_UNSUPPORTED;
-  if (((flags >> 8) & 7) > 4 || (flags & ~0x7ff)) return GSMI_ERR_ARG;
+  if (((flags >> 8) & 7) > 4 || (flags & ~0x701)) return GSMI_ERR_ARG;  // bit 0 and bits 8-10 only
   const int64_t n_tiles = (int64_t)n_tiles_h * n_tiles_w;
   if (n_tiles > (1 << 24)) return GSMI_ERR_UNSUPPORTED;
   if (!workspace || workspace_bytes < bin_ws_bytes(V, P)) return GSMI_ERR_WORKSPACE;

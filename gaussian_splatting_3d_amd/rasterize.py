@@ -162,6 +162,10 @@ def preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera: CameraInfo, *, til
 
 
 BLOCK_SHIFT = 2  # two-level binning: blocks of 4 x 4 tiles (64 px)
+# The recorded per-tile lists of the two-level forward are indexed with int32
+# offsets and sized for (tiles per block) x (block pairs) entries; a view
+# whose bound reaches this limit is binned per tile instead.
+TWO_LEVEL_ENTRY_LIMIT = 2 ** 31
 
 
 def _forward_two_level(b: Binning, sh_coeffs, out, topleft, c2w_d, C, thresh, T, nd, exact):
@@ -176,7 +180,7 @@ def _forward_two_level(b: Binning, sh_coeffs, out, topleft, c2w_d, C, thresh, T,
     nth, ntw = ic["n_tiles_h"], ic["n_tiles_w"]
     n_tiles = nth * ntw
     cap = (1 << (2 * b.shift)) * b.P  # >= sum over tiles of their block's list length
-    if cap >= 2 ** 31:
+    if cap >= TWO_LEVEL_ENTRY_LIMIT:  # forward_view bins such views per tile
         raise RuntimeError(f"two-level lists need {cap} entries, beyond the int32 range")
     foff = torch.empty(n_tiles, dtype=torch.int32, device=dev)
     fend = torch.empty(n_tiles, dtype=torch.int32, device=dev)
@@ -223,10 +227,17 @@ def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact
         early["g_sh"] = torch.zeros_like(sh_coeffs)
         early["bnd"] = torch.zeros(V * 7, dtype=torch.float32, device=mean.device)
 
-    b = preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera, tile_size=cfg.tile_size, D=cfg.tile_culling_radius,
-                           frustum_culling_radius=cfg.frustum_culling_radius,
-                           skip_frustum_culling=cfg.skip_frustum_culling, block_shift=BLOCK_SHIFT if two else 0,
-                           bg_rgb=cfg.bg_rgb if cfg.bg else None, on_sync=on_sync if prepare_backward else None)
+    def bin_view(shift):
+        return preprocess_and_bin(mean, qvec, log_s, logit, c2w, camera, tile_size=cfg.tile_size,
+                                  D=cfg.tile_culling_radius, frustum_culling_radius=cfg.frustum_culling_radius,
+                                  skip_frustum_culling=cfg.skip_frustum_culling, block_shift=shift,
+                                  bg_rgb=cfg.bg_rgb if cfg.bg else None,
+                                  on_sync=on_sync if prepare_backward else None)
+
+    b = bin_view(BLOCK_SHIFT if two else 0)
+    if two and (1 << (2 * BLOCK_SHIFT)) * b.P >= TWO_LEVEL_ENTRY_LIMIT:
+        two = False  # the recorded lists would overflow int32 offsets: per-tile lists (same image)
+        b = bin_view(0)
     ic = b.consts
     H, W = ic["H"], ic["W"]
     dev = mean.device

@@ -4,17 +4,24 @@ The reference saves ``SHRenderer`` state with ``torch.save`` of a dict
 (gs/sh_renderer.py:663-676): ``mean [N,3]``, ``qvec [N,4]``,
 ``svec_before_activation [N,3]``, ``sh_coeffs [N,3,C^2]``,
 ``alpha_before_activation [N]``, ``N`` and the Hydra ``cfg``; ``load``
-(:678-708) restores the five tensors as parameters.  Here a checkpoint is
-read with ``torch.load(weights_only=True)`` only: tensors, numbers, strings
-and plain containers load, anything that would need arbitrary unpickling
-(e.g. a Hydra/OmegaConf config object) is refused with an explanation
-instead of being executed.  ``save_checkpoint`` writes the reference's keys
-with the config as a plain dict, which that loader accepts.
+(:678-708) restores the five tensors as parameters.
+
+Here a checkpoint is read with torch's ``weights_only`` unpickler only, which
+builds tensors, numbers, strings and plain containers and nothing else.  A
+file that also names other classes or functions (the reference always stores
+its Hydra/OmegaConf ``cfg``) is read the same way after every such name,
+found by disassembling the pickle without running it, is bound to an inert
+stand-in (``_Inert``): the stand-in takes any arguments and state and keeps
+none, so the config comes back as an empty placeholder and nothing the file
+names is imported or called.  Only the five tensors and ``N`` are used.
+``save_checkpoint`` writes the reference's keys with the config as a plain
+dict.
 """
 from __future__ import annotations
 
 import math
 import os
+import pickle
 from dataclasses import dataclass
 
 import torch
@@ -67,17 +74,47 @@ def _validate(sd):
     return N, C
 
 
-def load_checkpoint(path, device="cuda:0", C=None):
-    """Reads a checkpoint safely and returns a ResidentScene on `device`.
-    C (SH order used for rendering) defaults to the order the stored
-    coefficients hold."""
+class _Inert:
+    """Stand-in for a class or function a checkpoint names but this loader
+    does not run (e.g. the reference's Hydra config classes): constructing,
+    calling or restoring it accepts anything and keeps nothing."""
+
+    def __new__(cls, *args, **kwargs):
+        return object.__new__(cls)
+
+    def __init__(self, *args, **kwargs):
+        pass
+
+    def __setstate__(self, state):
+        pass
+
+
+def read_checkpoint(path):
+    """The checkpoint's top-level object, read with torch's weights_only
+    unpickler; globals outside its allow-list are bound to _Inert (they are
+    listed by disassembly, never imported or called).  A file naming a
+    module torch blocks outright (os, subprocess, ...) raises RuntimeError."""
+    path = os.fspath(path)
     try:
-        sd = torch.load(os.fspath(path), map_location="cpu", weights_only=True)
-    except Exception as e:  # refused: the file needs arbitrary unpickling
-        raise RuntimeError(
-            f"{path}: the checkpoint holds objects that cannot be loaded without executing pickled code "
-            f"({type(e).__name__}); only tensors, numbers and plain containers are accepted. Re-save it with "
-            "scene_io.save_checkpoint (config as a plain dict) or without the config.") from None
+        return torch.load(path, map_location="cpu", weights_only=True)
+    except pickle.UnpicklingError:
+        names = torch.serialization.get_unsafe_globals_in_checkpoint(path)
+        if not names:
+            raise
+    try:
+        with torch.serialization.safe_globals([(_Inert, name) for name in names]):
+            return torch.load(path, map_location="cpu", weights_only=True)
+    except pickle.UnpicklingError:  # torch blocks some modules (os, subprocess, ...) outright
+        raise RuntimeError(f"{path}: the checkpoint names {names}; the loader refuses such a file even with "
+                           "inert placeholders") from None
+
+
+def load_checkpoint(path, device="cuda:0", C=None):
+    """Reads a checkpoint (the reference's SHRenderer.save format, config
+    included) without executing anything from it and returns a
+    ResidentScene on `device`.  C (SH order used for rendering) defaults to
+    the order the stored coefficients hold."""
+    sd = read_checkpoint(path)
     if not isinstance(sd, dict):
         raise ValueError(f"{path}: expected a dict, found {type(sd).__name__}")
     N, C_store = _validate(sd)

@@ -291,8 +291,16 @@ int orc_render_sh_fwd(int C, const float *mean, const float *cov,
  * grad_* arrays (callers pass zeros, renderer.py:760-763).  grad_cov receives
  * both off-diagonals, each 0.5 * P*a*G * s*u.
  * Optional abs_* outputs (double, may be NULL): for every gradient component
- * the sum of the absolute values of the per-pixel terms that make it up, the
- * scale of the rounding error any float summation of those terms can have.
+ * the sum over its per-pixel terms of the term's rounding scale.  For the SH
+ * terms that is |term|.  The mean, cov and alpha terms carry the factor
+ *   P = sum_ch g_ch (y_ch T - (F_ch - o_ch) / (1 - a G)),
+ * whose difference F - o cancels late in a list (o has reached F and T is
+ * small), so their scale is that of P's parts,
+ *   |P| <= sum_ch |g_ch| (|y_ch| T + (|F_ch| + |o_ch|) / (1 - a G)),
+ * times the remaining factors.  Any float evaluation of the reference's
+ * formula (this one included) differs from the exact value by a small
+ * multiple of eps times this scale, and so may differ from the reverse-order
+ * adjoint, which accumulates the colour behind instead of cancelling it.
  * Sums are formed in double, per tile over its pixels, and added to the
  * per-Gaussian totals in tile order, so the result does not depend on the
  * number of threads the tiles run on.
@@ -373,6 +381,10 @@ int orc_render_sh_bwd(int C, const float *mean, const float *cov,
               float paG = 0.0f;
               for (int ch = 0; ch < 3; ++ch)
                 paG += g[ch] * (y[ch] * T - (F[ch] - o[ch]) / (1 - a * G));
+              double pab = 0.0; /* rounding scale of paG */
+              for (int ch = 0; ch < 3; ++ch)
+                pab += fabs((double)g[ch]) * (fabs((double)y[ch]) * (double)T +
+                                              (fabs((double)F[ch]) + fabs((double)o[ch])) / (double)(1 - a * G));
               /* kernels.h:394-418 in double */
               {
                 const float *m = mean + 2 * gid, *c = cov + 4 * gid;
@@ -390,15 +402,19 @@ int orc_render_sh_bwd(int C, const float *mean, const float *cov,
                 v[3] = 0.5 * (double)(float)(dg * tmpx * tmpy);
                 v[4] = 0.5 * (double)(float)(dg * tmpy * tmpx);
                 v[5] = 0.5 * (double)(float)(dg * tmpy * tmpy);
+                const double dab = pab * (double)a * (double)G;
+                const double va[6] = {dab * fabs(tmpx), dab * fabs(tmpy), 0.5 * dab * tmpx * tmpx,
+                                      0.5 * dab * fabs(tmpx * tmpy), 0.5 * dab * fabs(tmpx * tmpy),
+                                      0.5 * dab * tmpy * tmpy};
                 for (int q = 0; q < 6; ++q) {
                   L[q] += v[q];
-                  LA[q] += fabs(v[q]);
+                  LA[q] += va[q];
                 }
               }
               {
                 const double v = (double)(paG * G);
                 L[6 + 3 * CC] += v;
-                LA[6 + 3 * CC] += fabs(v);
+                LA[6 + 3 * CC] += pab * (double)G;
               }
               T *= (1 - a * G);
             }
@@ -447,6 +463,144 @@ int orc_render_sh_bwd(int C, const float *mean, const float *cov,
   free(acc);
   free(acc_abs);
   return mismatch;
+}
+
+/*
+ * The same adjoint evaluated in double (test infrastructure: the exact value
+ * that the float evaluations above and on the GPU approximate).  The skip and
+ * break decisions are the float forward's (orc_gauss2d_f, float T), as in
+ * orc_render_sh_bwd; every value along the composited entries (Gaussian
+ * value, sigmoid colour, transmittance, F, the colour behind) is formed in
+ * double, with F - o taken in double after a first pass.  Per-tile sums are
+ * added in tile order.  Gradients are ADDED into the double grad_* arrays.
+ * mode 1 blends bg as vol_render_bg.h:183-188 (where the float T ends above
+ * thresh).  Returns 0, or -1 for unsupported C.
+ */
+int orc_render_sh_bwd_f64(int C, const float *mean, const float *cov, const float *sh, const float *alpha,
+                          const int *start, const int *end, const int *ids, double *grad_mean,
+                          double *grad_cov, double *grad_sh, double *grad_alpha, const float *grad_out,
+                          const float *topleft, const float *c2w, int tile_size, int n_tiles_h, int n_tiles_w,
+                          float psx, float psy, int H, int W, float thresh, int mode, const float *bg, int V) {
+  if (C < 1 || C > ORC_MAX_C) return -1;
+  const int CC = C * C;
+  const int NC = 7 + 3 * CC;
+  const int n_tiles = n_tiles_h * n_tiles_w;
+  int max_n = 0;
+  for (int t = 0; t < n_tiles; ++t)
+    if (start[t] != -1 && end[t] - start[t] > max_n) max_n = end[t] - start[t];
+  const size_t cap = (size_t)(max_n > 0 ? max_n : 1);
+  double *acc = calloc((size_t)V * NC, sizeof(double));
+#pragma omp parallel
+  {
+    double *loc = calloc(cap * NC, sizeof(double));
+    int *sel = malloc(cap * sizeof(int));         /* composited list positions */
+    double *val = malloc(cap * 6 * sizeof(double)); /* G, T before, y0..y2, a */
+#pragma omp for ordered schedule(dynamic, 1)
+    for (int tile = 0; tile < n_tiles; ++tile) {
+      const int ty = tile / n_tiles_w, tx = tile % n_tiles_w;
+      const int s0 = start[tile];
+      int used = 0;
+      if (s0 != -1 && end[tile] - s0 != 0) {
+        const int n = end[tile] - s0;
+        for (int ly = 0; ly < tile_size; ++ly) {
+          for (int lx = 0; lx < tile_size; ++lx) {
+            const int gx = tx * tile_size + lx, gy = ty * tile_size + ly;
+            if (gx >= W || gy >= H) continue;
+            const size_t pix = (size_t)gy * W + gx;
+            float pos[2], dir[3], Y[16];
+            orc_pixel(topleft, psx, psy, gx, gy, pos);
+            orc_direction(c2w, pos[0], pos[1], dir);
+            orc_sh_basis(dir, Y, C);
+            const float *g = grad_out + 3 * pix;
+            float Tf = 1.0f;
+            double Td = 1.0, o[3] = {0.0, 0.0, 0.0};
+            int m = 0;
+            for (int i = 0; i < n; ++i) {
+              if (Tf < thresh) break;
+              const int gid = ids[s0 + i];
+              const float af = fminf(alpha[gid], 0.99f);
+              const float Gf = orc_gauss2d_f(mean + 2 * gid, cov + 4 * gid, pos[0], pos[1]);
+              if (af * Gf < kMinRenderAlpha) continue;
+              const float *mm = mean + 2 * gid, *c = cov + 4 * gid;
+              const double c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], det = c0 * c3 - c1 * c2;
+              const double x = (double)pos[0] - mm[0], yy = (double)pos[1] - mm[1];
+              double r = ((x * c3 - yy * c2) * x + (-x * c1 + yy * c0) * yy) / det;
+              if (r < 0.0) r = 1000.0;
+              const double G = exp(-0.5 * r), a = (double)af;
+              double *vv = val + (size_t)m * 6;
+              vv[0] = G;
+              vv[1] = Td;
+              vv[5] = a;
+              for (int ch = 0; ch < 3; ++ch) {
+                double d = 0.0;
+                for (int k = 0; k < CC; ++k) d += (double)sh[(size_t)(3 * gid + ch) * CC + k] * (double)Y[k];
+                vv[2 + ch] = 1.0 / (1.0 + exp(-d));
+                o[ch] += a * Td * G * vv[2 + ch];
+              }
+              sel[m++] = i;
+              Tf *= (1 - af * Gf);
+              Td *= 1.0 - a * G;
+            }
+            double F[3] = {o[0], o[1], o[2]};
+            if (mode == 1 && Tf > thresh)
+              for (int ch = 0; ch < 3; ++ch) F[ch] = o[ch] * Td + (double)bg[ch] * (1.0 - Td);
+            double oa[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < m; ++q) {
+              const int i = sel[q];
+              const int gid = ids[s0 + i];
+              const double *vv = val + (size_t)q * 6;
+              const double G = vv[0], T = vv[1], a = vv[5], w = a * T * G;
+              double *L = loc + (size_t)i * NC;
+              if (i + 1 > used) used = i + 1;
+              double P = 0.0;
+              for (int ch = 0; ch < 3; ++ch) {
+                const double y = vv[2 + ch];
+                oa[ch] += w * y;
+                const double gr = w * y * (1.0 - y) * (double)g[ch];
+                for (int k = 0; k < CC; ++k) L[6 + ch * CC + k] += gr * (double)Y[k];
+                P += (double)g[ch] * (y * T - (F[ch] - oa[ch]) / (1.0 - a * G));
+              }
+              const float *mm = mean + 2 * gid, *c = cov + 4 * gid;
+              const double c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], det = c0 * c3 - c1 * c2;
+              const double x = (double)pos[0] - mm[0], yy = (double)pos[1] - mm[1];
+              const double su = (x * c3 - yy * c2) / det, uu = (-x * c1 + yy * c0) / det;
+              const double D = P * a * G;
+              L[0] += D * su;
+              L[1] += D * uu;
+              L[2] += 0.5 * D * su * su;
+              L[3] += 0.5 * D * su * uu;
+              L[4] += 0.5 * D * uu * su;
+              L[5] += 0.5 * D * uu * uu;
+              L[6 + 3 * CC] += P * G;
+            }
+          }
+        }
+      }
+#pragma omp ordered
+      {
+        for (int i = 0; i < used; ++i) {
+          double *A = acc + (size_t)ids[s0 + i] * NC, *L = loc + (size_t)i * NC;
+          for (int q = 0; q < NC; ++q) {
+            A[q] += L[q];
+            L[q] = 0.0;
+          }
+        }
+      }
+    }
+    free(loc);
+    free(sel);
+    free(val);
+  }
+  for (size_t v = 0; v < (size_t)V; ++v) {
+    const double *A = acc + v * NC;
+    grad_mean[2 * v + 0] += A[0];
+    grad_mean[2 * v + 1] += A[1];
+    for (int q = 0; q < 4; ++q) grad_cov[4 * v + q] += A[2 + q];
+    for (int q = 0; q < 3 * CC; ++q) grad_sh[v * 3 * CC + q] += A[6 + q];
+    grad_alpha[v] += A[6 + 3 * CC];
+  }
+  free(acc);
+  return 0;
 }
 
 /* Threads the parallel oracle loops use (OpenMP); n <= 0 keeps the default. */

@@ -57,6 +57,8 @@ def lib():
         L.orc_render_sh_bwd.restype = i32
         L.orc_render_sh_bwd.argtypes = [i32] + [P] * 15 + [i32, i32, i32, fl, fl, i32, i32, fl, i32, P, i32, P, P,
                                                            P, P]
+        L.orc_render_sh_bwd_f64.restype = i32
+        L.orc_render_sh_bwd_f64.argtypes = [i32] + [P] * 14 + [i32, i32, i32, fl, fl, i32, i32, fl, i32, P, i32]
         L.orc_set_threads.argtypes = [i32]
         L.orc_get_threads.restype = i32
         L.orc_render_rgb_fwd.restype = i32
@@ -341,8 +343,10 @@ def render_sh_bwd(mean, cov, sh, alpha, start, end, ids, out, grad_out, topleft,
                   n_tiles_h, n_tiles_w, psx, psy, H, W, C, thresh, mode=0, bg=None, want_abs=False):
     """Reference backward (forward-order recompute).  Returns (grads dict,
     mismatch flag); with want_abs, (grads, mismatch, abs) where abs holds,
-    per gradient component, the sum of |per-pixel term| (float64): the scale
-    of the rounding error of any float summation of that component."""
+    per gradient component, the sum of its per-pixel terms' rounding scales
+    (float64): |term| for SH, and for mean, cov and alpha the term with the
+    factor P = sum g (y T - (F - o)/(1 - aG)) replaced by the sum of the
+    magnitudes of its parts, since F - o cancels (gs_oracle.c)."""
     m = _c(mean, f32)
     c = _c(cov, f32)
     s = _c(sh, f32)
@@ -369,6 +373,30 @@ def render_sh_bwd(mean, cov, sh, alpha, start, end, ids, out, grad_out, topleft,
         return grads, rc
     absd = dict(mean=am.reshape(V, 2), cov=ac.reshape(V, 2, 2), sh=ash.reshape(np.shape(sh)), alpha=aa)
     return grads, rc, absd
+
+
+def render_sh_bwd_f64(mean, cov, sh, alpha, start, end, ids, grad_out, topleft, c2w, tile_size, n_tiles_h,
+                      n_tiles_w, psx, psy, H, W, C, thresh, mode=0, bg=None):
+    """The adjoint of render_sh_bwd evaluated in float64 along the float
+    forward's decisions (gs_oracle.c orc_render_sh_bwd_f64): the exact value
+    the float evaluations approximate.  Returns a dict of float64 arrays."""
+    m = _c(mean, f32)
+    c = _c(cov, f32)
+    s = _c(sh, f32)
+    a = _c(alpha, f32)
+    V = a.size
+    gm = np.zeros(V * 2)
+    gc = np.zeros(V * 4)
+    gs = np.zeros(s.size)
+    ga = np.zeros(V)
+    b = _c(bg if bg is not None else [0, 0, 0], f32)
+    rc = lib().orc_render_sh_bwd_f64(C, _p(m), _p(c), _p(s), _p(a), _p(_c(start, np.int32)), _p(_c(end, np.int32)),
+                                     _p(_c(ids, np.int32)), _p(gm), _p(gc), _p(gs), _p(ga), _p(_c(grad_out, f32)),
+                                     _p(_c(topleft, f32)), _p(_c(c2w, f32)), tile_size, n_tiles_h, n_tiles_w,
+                                     f32(psx), f32(psy), H, W, f32(thresh), mode, _p(b), V)
+    if rc < 0:
+        raise RuntimeError("oracle render_sh_bwd_f64: unsupported C")
+    return dict(mean=gm.reshape(V, 2), cov=gc.reshape(V, 2, 2), sh=gs.reshape(np.shape(sh)), alpha=ga)
 
 
 def set_threads(n):

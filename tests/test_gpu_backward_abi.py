@@ -4,9 +4,10 @@
    start_end pair) take exactly the reference's arguments (gs/src/render.h:
    70-133) through ctypes; the backward recomputes the forward state inside
    its launch.  Checked against the oracle (image within 1e-4; gradients
-   within 1e-3 max-norm and per element, see elementwise_ok) and against the
-   saved-state backward.  The in-place `_backend` methods of the same names
-   are checked the same way.
+   within 1e-3 max-norm and per element, see gradmetrics) and against the
+   saved-state backward, and per element against the float64 evaluation of
+   the adjoint (exact_excess).  The in-place `_backend` methods of the same
+   names are checked the same way.
 2. The default SH-gradient reduction forms fp32 products (vol_render_sh.h:
    28-36); on a scene of mostly non-overlapping Gaussians with a mixed-sign
    gradient image (the per-Gaussian sums cancel), every SH-gradient element
@@ -21,30 +22,13 @@ import torch
 
 from gaussian_splatting_3d_amd import _lib, scenes
 from gaussian_splatting_3d_amd.backend import _backend
+from gradmetrics import elementwise_excess, exact_excess, rel_err
 
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
 IMG_TOL = 1e-4
 GRAD_RTOL = 1e-3
-
-
-def rel_err(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
-
-
-def elementwise_excess(got, want, absum, rtol, atol_abs):
-    """max over elements of |g - r| / (rtol |r| + atol_abs * absum + tiny);
-    <= 1 means every element is within the bound.  absum is the sum of the
-    element's absolute per-pixel terms, the scale of the rounding error of
-    any float summation of them."""
-    g = np.asarray(got, np.float64).reshape(-1)
-    r = np.asarray(want, np.float64).reshape(-1)
-    a = np.asarray(absum, np.float64).reshape(-1)
-    tiny = 1e-12 * max(np.abs(r).max(), 1e-30)
-    return float((np.abs(g - r) / (rtol * np.abs(r) + atol_abs * a + tiny)).max())
 
 
 def sh_scene(C, n=4000, W=96, H=80, seed=11, log_s=-3.0, spread=0.8):
@@ -119,11 +103,16 @@ def test_reference_signature_sh_entries(orc, C, with_bg):
                                      ref["end"], ref["ids"], out.cpu().numpy(), gout_np, ref["topleft"], c2w, 16,
                                      nth, ntw, ref["psx"], ref["psy"], H, W, C, 1e-4,
                                      mode=1 if with_bg else 0, bg=bg, want_abs=True)
+    exact = orc.render_sh_bwd_f64(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]], ref["start"],
+                                  ref["end"], ref["ids"], gout_np, ref["topleft"], c2w, 16, nth, ntw, ref["psx"],
+                                  ref["psy"], H, W, C, 1e-4, mode=1 if with_bg else 0, bg=bg)
     for name, got in zip(("mean", "cov", "sh", "alpha"), g):
         want = np.asarray(rg[name]).reshape(got.shape)
         assert rel_err(got.cpu(), want) <= GRAD_RTOL, name
-        ex = elementwise_excess(got.cpu(), want, np.asarray(ab[name]).reshape(got.shape), GRAD_RTOL, 1e-5)
+        ex = elementwise_excess(got, want, ab[name], GRAD_RTOL, 1e-5)
         assert ex <= 1.0, (name, ex)
+        ex = exact_excess(got, exact[name], GRAD_RTOL, 1e-6)
+        assert ex <= 1.0, (name, "vs float64", ex)
     assert float(g[2].abs().max()) > 0
 
     # the saved-state backward (forward state from the extended forward) agrees

@@ -3,29 +3,60 @@ Gaussians at 1920x1080, SH degree 3.
 
 The oracle runs the whole preprocessing and binning of each workload (every
 mask bit, record, tile bound, sorted id and tile range is compared bit for
-bit) and composites a seeded sample of tiles, forward and backward; the GPU
-renders the full image.  On the sampled tiles the exact colour mode must be
-bit-identical (image, final transmittance, entries examined), the fast mode
-within 1e-4, and the adjoint of a gradient image that is non-zero only there
-within 1e-3 (max-norm relative) for every visible Gaussian.
+bit) and composites a sample of tiles, forward and backward; the GPU renders
+the full image.  The sample is stratified (every 10th non-empty tile in
+order of list length) and always holds the 16 tiles with the longest lists
+and the 16 with the largest entry count examined by a pixel (n_done); for
+more than 80% of the sampled tiles the 64-px block list is longer than the
+two-level filter's 512-entry LDS ring, so the ring wraps there.  On the
+sampled tiles the exact colour mode must be bit-identical (image, final
+transmittance, entries examined), the fast mode within 1e-4, and the adjoint
+of a gradient image that is non-zero only there within 1e-3 (max-norm
+relative) for every visible Gaussian, and per element within the oracle's
+rounding scale (elementwise_excess).  Against the float64 evaluation of the
+adjoint, per element by 1e-3 |r| + 1e-6 max |r| (exact_excess), the GPU must
+be within that bound or no further off than the reference's own float
+formula (the oracle) is: at 3M Gaussians no float evaluation meets it (the
+oracle's excess is 2.5 for mean2d, 6.7 for cov, 1.1 for alpha, 1.2 for SH;
+the errors of long float products of transmittance and Gaussian values are
+larger than 1e-6 of the largest gradient for some small elements).
 """
 import numpy as np
 import pytest
 import torch
 
 from gaussian_splatting_3d_amd import rasterize, scenes
+from gradmetrics import elementwise_excess, exact_excess, rel_err as _rel
 
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
 IMG_TOL = 1e-4
 GRAD_RTOL = 1e-3
+RING = 512  # entries of the two-level filter's LDS ring (composite.hip)
 
 
-def _rel(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+def sample_tiles(counts, n_done_tile, every=10, top=16):
+    """Every `every`-th non-empty tile in order of list length, plus the
+    `top` longest lists and the `top` largest per-tile n_done."""
+    nonempty = np.nonzero(counts)[0]
+    by_len = nonempty[np.argsort(counts[nonempty], kind="stable")]
+    by_nd = nonempty[np.argsort(n_done_tile[nonempty], kind="stable")]
+    return np.unique(np.concatenate([by_len[every // 2::every], by_len[-top:], by_nd[-top:]]))
+
+
+def block_list_lengths(tl, br, nth, ntw, shift=2):
+    """Entries of each 64-px block's list: Gaussians whose tile rectangle,
+    shifted to the block grid, covers the block."""
+    nbh, nbw = -(-nth // (1 << shift)), -(-ntw // (1 << shift))
+    cnt = np.zeros((nbh + 1, nbw + 1), np.int64)
+    x0, y0 = tl[:, 0] >> shift, tl[:, 1] >> shift
+    x1, y1 = br[:, 0] >> shift, br[:, 1] >> shift
+    np.add.at(cnt, (y0, x0), 1)
+    np.add.at(cnt, (y0, x1 + 1), -1)
+    np.add.at(cnt, (y1 + 1, x0), -1)
+    np.add.at(cnt, (y1 + 1, x1 + 1), 1)
+    return cnt.cumsum(0).cumsum(1)[:nbh, :nbw]
 
 
 def _oracle_binning(O, sc):
@@ -79,10 +110,18 @@ def test_full_size_binning_and_sampled_tiles(orc, which):
     assert np.array_equal(b.ids.cpu().numpy(), bn["ids"])
     assert np.array_equal(b.start.cpu().numpy(), bn["start"]) and np.array_equal(b.end.cpu().numpy(), bn["end"])
 
-    # the oracle composites a seeded sample of non-empty tiles
+    # the oracle composites a stratified sample of non-empty tiles plus the
+    # heaviest ones
+    counts = np.where(bn["start"] >= 0, bn["end"] - bn["start"], 0).astype(np.int64)
+    nd_img = np.zeros((nth * 16, ntw * 16), np.int64)
+    nd_img[:H, :W] = st_x.n_done.cpu().numpy().reshape(H, W)
+    nd_tile = nd_img.reshape(nth, 16, ntw, 16).max(axis=(1, 3)).reshape(-1)
+    tiles = sample_tiles(counts, nd_tile)
+    assert tiles.size >= 0.1 * np.count_nonzero(counts)
+    blen = block_list_lengths(ref["tl"], ref["br"], nth, ntw)
+    ty, tx = np.divmod(tiles, ntw)
+    assert (blen[ty >> 2, tx >> 2] > RING).mean() > 0.8  # the LDS ring wraps in most sampled tiles
     rng = np.random.default_rng(17)
-    nonempty = np.nonzero(bn["start"] >= 0)[0]
-    tiles = rng.choice(nonempty, size=min(12, nonempty.size), replace=False)
     st = -np.ones(nth * ntw, np.int32)
     en = -np.ones(nth * ntw, np.int32)
     st[tiles] = bn["start"][tiles]
@@ -110,10 +149,20 @@ def test_full_size_binning_and_sampled_tiles(orc, which):
     gout = np.zeros(H * W * 3, np.float32)
     gout[pix3] = (rng.random(pix3.size).astype(np.float32) - 0.5) * 1e-3
     _, _, _, _, g_sh, bd = rasterize.backward_view(st_f, *p, torch.from_numpy(gout).to(DEV))
-    rg, mism = orc.render_sh_bwd(ref["m2d"], ref["cov"], sh_v, al, st, en, bn["ids"], o_out, gout, ic["topleft"], c2w,
-                                 16, nth, ntw, ic["psx"], ic["psy"], H, W, C, 1e-4)
+    rg, mism, scale = orc.render_sh_bwd(ref["m2d"], ref["cov"], sh_v, al, st, en, bn["ids"], o_out, gout,
+                                        ic["topleft"], c2w, 16, nth, ntw, ic["psx"], ic["psy"], H, W, C, 1e-4,
+                                        want_abs=True)
     assert mism == 0
-    assert _rel(bd["mean2d"].cpu(), rg["mean"]) <= GRAD_RTOL
-    assert _rel(bd["cov"].cpu(), rg["cov"]) <= GRAD_RTOL
-    assert _rel(bd["alpha"].cpu(), rg["alpha"]) <= GRAD_RTOL
-    assert _rel(g_sh.cpu().numpy()[ref["idx"]][..., : C * C], rg["sh"]) <= GRAD_RTOL
+    exact = orc.render_sh_bwd_f64(ref["m2d"], ref["cov"], sh_v, al, st, en, bn["ids"], gout, ic["topleft"], c2w, 16,
+                                  nth, ntw, ic["psx"], ic["psy"], H, W, C, 1e-4)
+    got = dict(mean=bd["mean2d"].cpu().numpy(), cov=bd["cov"].cpu().numpy(), alpha=bd["alpha"].cpu().numpy(),
+               sh=g_sh.cpu().numpy()[ref["idx"]][..., : C * C])
+    for name in ("mean", "cov", "alpha", "sh"):
+        want = np.asarray(rg[name]).reshape(got[name].shape)
+        assert _rel(got[name], want) <= GRAD_RTOL, name
+        ex = elementwise_excess(got[name], want, scale[name], GRAD_RTOL, 1e-5)
+        assert ex <= 1.0, (name, ex)
+        ex = exact_excess(got[name], exact[name], GRAD_RTOL, 1e-6)
+        ex_ref = exact_excess(want, exact[name], GRAD_RTOL, 1e-6)
+        print(f"{which} {name}: excess over the float64 bound {ex:.3g} (reference formula {ex_ref:.3g})")
+        assert ex <= max(1.0, 1.5 * ex_ref), (name, "vs float64", ex, ex_ref)

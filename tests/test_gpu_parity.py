@@ -503,6 +503,23 @@ def test_two_level_matches_single_level_full_size():
         assert rel_err(a, b) <= 1e-4
 
 
+def test_two_level_falls_back_to_tile_lists_beyond_the_entry_limit(monkeypatch):
+    """A view whose recorded two-level lists would overflow their int32
+    offsets is binned per tile instead (forced here by a small limit): same
+    image and n_done as the per-tile path, and per-tile lists in the state."""
+    sc = small_sh_scene(4, n=3000)
+    c2w, cam = sc.cameras[0]
+    p = dev_params(sc)
+    out1, st1 = rasterize.forward_view(*p, c2w, cam, sc.C, scenes.RasterConfig(two_level=False))
+    out2, st2 = rasterize.forward_view(*p, c2w, cam, sc.C, scenes.RasterConfig())
+    assert st2.b.shift == 2  # two-level by default
+    monkeypatch.setattr(rasterize, "TWO_LEVEL_ENTRY_LIMIT", 16 * st2.b.P)
+    out3, st3 = rasterize.forward_view(*p, c2w, cam, sc.C, scenes.RasterConfig())
+    assert st3.b.shift == 0 and st3.b.P == st1.b.P
+    assert torch.equal(out3, out1) and torch.equal(out2, out1)
+    assert torch.equal(st3.n_done, st1.n_done)
+
+
 def test_full_chain_gradients(orc):
     sc = small_sh_scene(4, n=3000)
     c2w, cam = sc.cameras[0]

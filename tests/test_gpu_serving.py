@@ -171,3 +171,42 @@ def test_continuous_submission_while_batches_run():
         assert torch.equal(out, reference(p, C, c2w, cam)), rid
     lat = np.array([c.latency_ms for c in done.values()])
     assert np.all(lat >= 0)
+
+
+def test_reference_checkpoint_served_resident(tmp_path):
+    """A checkpoint in the reference's SHRenderer.save layout, with a config
+    object of a library that is not importable here, is loaded with
+    scene_io.load_checkpoint, registered with add_resident and served: each
+    image equals the per-call forward of the loaded tensors."""
+    import sys
+    import types
+
+    from gaussian_splatting_3d_amd import scene_io
+
+    pkg = types.ModuleType("fake_hydra_cfg")
+
+    class DictConfig:
+        def __init__(self):
+            self._content = {"sh_order": 4, "white_bg": False}
+
+    DictConfig.__module__, DictConfig.__qualname__ = pkg.__name__, "DictConfig"
+    pkg.DictConfig = DictConfig
+    sys.modules[pkg.__name__] = pkg
+    (mean, q, ls, logit, sh), C = scene(4, 5000, 9)
+    path = tmp_path / "ckpt" / "scene.pt"
+    path.parent.mkdir()
+    try:
+        torch.save({"mean": mean.cpu(), "qvec": q.cpu(), "svec_before_activation": ls.cpu(), "sh_coeffs": sh.cpu(),
+                    "alpha_before_activation": logit.cpu(), "N": 5000, "cfg": DictConfig()}, path)
+    finally:
+        del sys.modules[pkg.__name__]
+    res = scene_io.load_checkpoint(path, device=DEV)
+    assert res.C == 4 and res.N == 5000
+    with serving.RenderServer(arena_bytes=256 << 20, max_views=8, n_slots=2) as srv:
+        sid = srv.add_resident(res)
+        reqs = [(srv.submit(sid, *camera(k)), camera(k)) for k in range(5)]
+        srv.drain()
+        done = {c.request_id: c for c in srv.poll()}
+    for (rid, out), (c2w, cam) in reqs:
+        assert done[rid].status == 0
+        assert torch.equal(out, reference(res.params(), res.C, c2w, cam)), rid

@@ -1,6 +1,10 @@
 """Checkpoint -> resident scene (CPU): reference key layout round trip,
-validation, and refusal of files that would need arbitrary unpickling."""
+validation, and checkpoints whose config object names classes and
+functions the loader must not import or call."""
+import os
 import pickle
+import sys
+import types
 
 import pytest
 import torch
@@ -53,11 +57,6 @@ def test_validation(tmp_path, bad):
         scene_io.load_checkpoint(p, device="cpu")
 
 
-class _Config:  # stands in for a config object that needs unpickling
-    def __reduce__(self):
-        return (_mark_executed, ())
-
-
 _EXECUTED = []
 
 
@@ -66,11 +65,95 @@ def _mark_executed():
     return {}
 
 
-def test_refuses_objects_needing_unpickling(tmp_path):
+class _Config:  # a config object whose restoration would call a function
+    def __reduce__(self):
+        return (_mark_executed, ())
+
+
+def test_config_needing_code_is_not_executed(tmp_path):
     mean, q, s, sh, a = _params()
     p = tmp_path / "unsafe.pt"
     torch.save({"mean": mean, "qvec": q, "svec_before_activation": s, "sh_coeffs": sh,
                 "alpha_before_activation": a, "N": 50, "cfg": _Config()}, p, pickle_module=pickle)
-    with pytest.raises(RuntimeError, match="cannot be loaded without executing"):
-        scene_io.load_checkpoint(p, device="cpu")
+    sc = scene_io.load_checkpoint(p, device="cpu")
     assert not _EXECUTED
+    assert torch.equal(sc.mean, mean) and sc.C == 4
+    assert isinstance(scene_io.read_checkpoint(p)["cfg"], scene_io._Inert)
+
+
+def _fake_config_library():
+    """A config library registered under a module name only for pickling:
+    DictConfig with nested metadata objects, and Hostile, whose restoration
+    would run a shell command.  Returns (classes, unregister)."""
+    pkg = types.ModuleType("fake_cfg_lib")
+    mod = types.ModuleType("fake_cfg_lib.dictconfig")
+    pkg.dictconfig = mod
+
+    class Metadata:
+        def __init__(self):
+            self.ref_type = dict
+            self.flags = {"readonly": False}
+
+    class DictConfig:
+        def __init__(self):
+            self._metadata = Metadata()
+            self._content = {"sh_order": 4, "lr": {"mean": 1e-3}, "tile_size": 16}
+
+    class Hostile:
+        flag = None
+
+        def __reduce__(self):
+            return (os.system, (f"touch {Hostile.flag}",))
+
+    for c in (Metadata, DictConfig, Hostile):
+        c.__module__ = mod.__name__
+        c.__qualname__ = c.__name__
+        setattr(mod, c.__name__, c)
+    sys.modules[pkg.__name__] = pkg
+    sys.modules[mod.__name__] = mod
+
+    def unregister():
+        del sys.modules[mod.__name__]
+        del sys.modules[pkg.__name__]
+
+    return (DictConfig, Hostile), unregister
+
+
+def test_reference_checkpoint_with_hydra_like_config(tmp_path):
+    """A checkpoint laid out as SHRenderer.save writes it: the five tensors,
+    N and a config object of a class from a config library, with nested
+    metadata objects.  The library is not importable when the file is read:
+    the loader binds its names to inert stand-ins and never imports them."""
+    (DictConfig, _), unregister = _fake_config_library()
+    mean, q, s, sh, a = _params(N=64, C=3)
+    p = tmp_path / "ckpt.pt"
+    try:
+        torch.save({"mean": mean, "qvec": q, "svec_before_activation": s, "sh_coeffs": sh,
+                    "alpha_before_activation": a, "N": 64, "cfg": DictConfig()}, p)
+    finally:
+        unregister()
+    names = torch.serialization.get_unsafe_globals_in_checkpoint(str(p))
+    assert "fake_cfg_lib.dictconfig.DictConfig" in names and "fake_cfg_lib.dictconfig.Metadata" in names
+    sc = scene_io.load_checkpoint(p, device="cpu")
+    assert "fake_cfg_lib" not in sys.modules
+    assert sc.N == 64 and sc.C == 3
+    for got, want in zip(sc.params(), (mean, q, s, a, sh)):
+        assert torch.equal(got, want)
+    assert isinstance(scene_io.read_checkpoint(p)["cfg"], scene_io._Inert)
+
+
+def test_checkpoint_naming_a_shell_call_is_refused_unexecuted(tmp_path):
+    (DictConfig, Hostile), unregister = _fake_config_library()
+    Hostile.flag = tmp_path / "ran"
+    cfg = DictConfig()
+    cfg._content["hook"] = Hostile()
+    mean, q, s, sh, a = _params()
+    p = tmp_path / "hostile.pt"
+    try:
+        torch.save({"mean": mean, "qvec": q, "svec_before_activation": s, "sh_coeffs": sh,
+                    "alpha_before_activation": a, "N": 50, "cfg": cfg}, p)
+    finally:
+        unregister()
+    with pytest.raises(RuntimeError, match="refuses"):
+        scene_io.load_checkpoint(p, device="cpu")
+    assert not (tmp_path / "ran").exists()
