@@ -68,6 +68,9 @@ SIGNATURES = {
     "gsmi_render_sh_backward": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                        _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _I32, _I32, _I32, _F, _P,
                                        _I32, _P, _SZ, _P]),
+    "gsmi_render_sh_backward_noalloc": (_I32, [_I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P,
+                                               _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _I32, _I32,
+                                               _I32, _F, _P, _I32, _P]),
     "gsmi_render_rgb_forward": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _F, _F,
                                        _I32, _I32, _F, _P, _P, _I32, _P]),
     "gsmi_render_rgb_backward": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,

@@ -15,6 +15,8 @@
 // pixels be rejected from q * (1/det) alone: for r > rlim the product a*G is
 // below 1/255 by a relative margin of 5e-4, far above the few-ulp gap between
 // q*(1/det) and q/det.  Everything else takes the exact path.
+#include <mutex>
+
 #include "gsmi_device.h"
 #include "scan.h"
 
@@ -1770,6 +1772,53 @@ extern "C" int gsmi_render_sh_backward(int64_t V, int64_t P, const float* mean, 
     if (rc) return rc;
   }
   return dispatch_bwd_sh(a, C, (flags & GSMI_BWD_SPLIT_BF16) != 0, recomp, st);
+}
+
+// ---------------------------------------------------------------------------
+// Backward without allocation: the tile order (costs, then the ordered tile
+// indices) goes to a file-scope device array instead of a caller workspace.
+// Each GPU holds its own copy of `scratch` at its own address; the address is
+// looked up once per device (with that device current) and only that address
+// is handed to kernels.  Every call on a GPU shares the array and the order is
+// rebuilt in full by each call (every tile's cost and position is written),
+// so nothing needs clearing; calls that overlap on one GPU (other streams or
+// threads) race on it.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxTiles = 1 << 18;  // 262,144 tiles: 8 views of 3840x2160
+static __device__ __align__(256) uint8_t scratch[2 * kNoAllocMaxTiles * 4];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of this device's `scratch`; null on failure.
+static void* scratch_on_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+extern "C" int gsmi_render_sh_backward_noalloc(
+    int64_t V, int64_t P, const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
+    int64_t sh_stride_c, const int32_t* sh_index, const float* alpha, const int32_t* start, const int32_t* end,
+    const int32_t* gaussian_ids, const float* out, const float* T_final, const int32_t* n_done, const float* pre_bg,
+    const float* grad_out, float* grad_mean, float* grad_cov, float* grad_sh, float* grad_alpha,
+    const float* topleft, const float* c2w, int tile_size, int n_tiles_h, int n_tiles_w, float pixel_size_x,
+    float pixel_size_y, int H, int W, int C, float thresh, const float* bg, int flags, void* stream) {
+  if (!T_final || !n_done) return GSMI_ERR_ARG;  // the order comes from the saved n_done
+  if (n_tiles_h <= 0 || n_tiles_w <= 0 || (int64_t)n_tiles_h * n_tiles_w > kNoAllocMaxTiles)
+    return n_tiles_h <= 0 || n_tiles_w <= 0 ? GSMI_ERR_ARG : GSMI_ERR_WORKSPACE;
+  void* ws = scratch_on_current_device();
+  if (!ws) return GSMI_ERR_LAUNCH;
+  return gsmi_render_sh_backward(V, P, mean, cov, sh, sh_stride_g, sh_stride_c, sh_index, alpha, start, end,
+                                 gaussian_ids, out, T_final, n_done, pre_bg, grad_out, grad_mean, grad_cov, grad_sh,
+                                 grad_alpha, topleft, c2w, tile_size, n_tiles_h, n_tiles_w, pixel_size_x,
+                                 pixel_size_y, H, W, C, thresh, bg, flags, ws, sizeof(scratch), stream);
 }
 
 // The reference's own signatures (render.h:86-101, 119-133): sh_coeffs is the

@@ -306,6 +306,27 @@ int gsmi_render_sh_backward(int64_t V, int64_t P, const float *mean, const float
                             size_t workspace_bytes, void *stream);
 
 /*
+ * gsmi_render_sh_backward with the launch-order workspace taken from a
+ * file-scope device array instead of an argument: allocates nothing (no
+ * hipMalloc, no allocating library call).  No counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid allocation).  Requires the saved state
+ * (T_final and n_done; GSMI_ERR_ARG otherwise) and at most 262,144 tiles
+ * (n_tiles_h * n_tiles_w; GSMI_ERR_WORKSPACE beyond).  The array (2 MiB per
+ * GPU, held until the process ends) is shared by every call on a GPU: calls
+ * that overlap on one GPU race.
+ */
+int gsmi_render_sh_backward_noalloc(int64_t V, int64_t P, const float *mean, const float *cov, const float *sh,
+                                    int64_t sh_stride_g, int64_t sh_stride_c, const int32_t *sh_index,
+                                    const float *alpha, const int32_t *start, const int32_t *end,
+                                    const int32_t *gaussian_ids, const float *out, const float *T_final,
+                                    const int32_t *n_done, const float *pre_bg, const float *grad_out,
+                                    float *grad_mean, float *grad_cov, float *grad_sh, float *grad_alpha,
+                                    const float *topleft, const float *c2w, int tile_size, int n_tiles_h,
+                                    int n_tiles_w, float pixel_size_x, float pixel_size_y, int H, int W, int C,
+                                    float thresh, const float *bg, int flags, void *stream);
+
+/*
  * Extended RGB compositing (render_start_end), called by the reference-shaped
  * entries above: the forward with optional T_final / n_done outputs and flags
  * bit 0 (write empty tiles' pixels); the backward from saved T_final / n_done
