@@ -180,9 +180,11 @@ def forward_views(mean, qvec, log_s, logit, sh_coeffs, cams, C, cfg, exact=False
     return out, ViewsState(b, int(C), cfg, bg, out, T, nd, pre)
 
 
-def backward_views(s: ViewsState, mean, qvec, log_s, logit, sh_coeffs, grad_out):
+def backward_views(s: ViewsState, mean, qvec, log_s, logit, sh_coeffs, grad_out, boundary=False):
     """Steps 6-7: gradients w.r.t. the raw parameters summed over the views.
-    Returns (g_mean, g_qvec, g_log_s, g_logit, g_sh)."""
+    Returns (g_mean, g_qvec, g_log_s, g_logit, g_sh); with boundary=True also
+    a dict of the compositing gradients of the records (mean2d [V,2], cov
+    [V,2,2], alpha [V], view-major as the records)."""
     L = _lib.load()
     b = s.b
     dev = mean.device
@@ -195,8 +197,10 @@ def backward_views(s: ViewsState, mean, qvec, log_s, logit, sh_coeffs, grad_out)
     go = grad_out.contiguous().view(-1)
     if go.numel() != b.n_pix * 3:
         raise RuntimeError(f"grad_out has {go.numel()} values, expected {b.n_pix * 3}")
+    bd = dict(mean2d=g_m2d, cov=g_cov, alpha=g_al)
     if b.V == 0:  # no view sees a Gaussian: every gradient is zero (no launch takes empty buffers)
-        return torch.zeros_like(mean), torch.zeros_like(qvec), torch.zeros_like(log_s), torch.zeros_like(logit), g_sh
+        g = (torch.zeros_like(mean), torch.zeros_like(qvec), torch.zeros_like(log_s), torch.zeros_like(logit), g_sh)
+        return (*g, bd) if boundary else g
     nws = int(L.gsmi_backward_order_workspace_size(b.n_tiles))  # launch order, most expensive tiles first
     ows = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
     check(L.gsmi_render_views_backward_sh(b.V, b.P, b.B, ptr(b.views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d),
@@ -212,7 +216,8 @@ def backward_views(s: ViewsState, mean, qvec, log_s, logit, sh_coeffs, grad_out)
     check(L.gsmi_project_backward_views(b.N, b.B, ptr(b.views_d), ptr(mean), ptr(qvec), ptr(log_s), ptr(logit),
                                         ptr(b.vis), ptr(b.vis_off), ptr(g_m2d), ptr(g_cov), ptr(g_al), ptr(g_mean),
                                         ptr(g_q), ptr(g_ls), ptr(g_lg), st), "project_backward_views")
-    return g_mean, g_q, g_ls, g_lg, g_sh
+    g = (g_mean, g_q, g_ls, g_lg, g_sh)
+    return (*g, bd) if boundary else g
 
 
 class _RasterizeSHViews(torch.autograd.Function):

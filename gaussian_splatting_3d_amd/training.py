@@ -1,10 +1,11 @@
 """Training-step stages around the rasteriser: loss seeding and the optimizer.
 
-* ``photometric_loss`` — the base term of the reference's loss
-  (utils/loss.py:5-22: ``l1_loss`` / ``mse_loss``, reduction "mean") and its
-  gradient w.r.t. the image in one HBM pass (gsmi_photometric_loss).  The SSIM
-  term of that loss comes from kornia, which is not available here; it is
-  not provided.
+* ``photometric_loss`` — the reference's loss (utils/loss.py:5-22): the base
+  term (``l1_loss`` / ``mse_loss``, reduction "mean") and its gradient w.r.t.
+  the image in one HBM pass (gsmi_photometric_loss), and with ``ssim_mult``
+  the SSIM term that the reference takes from kornia (``ssim_loss``, window
+  ``ssim_win``), fused with the base term and its gradient
+  (gsmi_photometric_ssim_loss).
 * ``FusedAdam`` — ``torch.optim.Adam`` over the five SHRenderer parameter
   groups (gs/sh_renderer.py:710-729, betas (0.9, 0.99)) as one launch
   (gsmi_adam_step).  ``fresh=True`` reproduces what the reference actually

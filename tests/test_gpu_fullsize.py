@@ -27,6 +27,7 @@ import torch
 
 from gaussian_splatting_3d_amd import rasterize, scenes
 from gradmetrics import elementwise_excess, exact_excess, rel_err as _rel
+from tilesample import sample_tiles, tile_max, tile_pixels as _tile_pixels
 
 pytestmark = pytest.mark.gpu
 
@@ -34,15 +35,6 @@ DEV = "cuda:0"
 IMG_TOL = 1e-4
 GRAD_RTOL = 1e-3
 RING = 512  # entries of the two-level filter's LDS ring (composite.hip)
-
-
-def sample_tiles(counts, n_done_tile, every=10, top=16):
-    """Every `every`-th non-empty tile in order of list length, plus the
-    `top` longest lists and the `top` largest per-tile n_done."""
-    nonempty = np.nonzero(counts)[0]
-    by_len = nonempty[np.argsort(counts[nonempty], kind="stable")]
-    by_nd = nonempty[np.argsort(n_done_tile[nonempty], kind="stable")]
-    return np.unique(np.concatenate([by_len[every // 2::every], by_len[-top:], by_nd[-top:]]))
 
 
 def block_list_lengths(tl, br, nth, ntw, shift=2):
@@ -71,16 +63,6 @@ def _oracle_binning(O, sc):
     ic = O.image_constants(oc)
     bn = O.bin_gaussians(tl, br, depth, ic["n_tiles_h"], ic["n_tiles_w"])
     return dict(mask=mask, idx=idx, m2d=m2d, cov=cov, depth=depth, a=a, P=int(P), tl=tl, br=br, ic=ic, bn=bn)
-
-
-def _tile_pixels(tiles, ntw, H, W):
-    pix = []
-    for t in tiles:
-        ty, tx = divmod(int(t), ntw)
-        ys = np.arange(ty * 16, min(ty * 16 + 16, H))
-        xs = np.arange(tx * 16, min(tx * 16 + 16, W))
-        pix.append((ys[:, None] * W + xs[None, :]).reshape(-1))
-    return np.concatenate(pix)
 
 
 @pytest.mark.parametrize("which", ["fwd_bwd_500k", "dense_3m"])
@@ -113,9 +95,7 @@ def test_full_size_binning_and_sampled_tiles(orc, which):
     # the oracle composites a stratified sample of non-empty tiles plus the
     # heaviest ones
     counts = np.where(bn["start"] >= 0, bn["end"] - bn["start"], 0).astype(np.int64)
-    nd_img = np.zeros((nth * 16, ntw * 16), np.int64)
-    nd_img[:H, :W] = st_x.n_done.cpu().numpy().reshape(H, W)
-    nd_tile = nd_img.reshape(nth, 16, ntw, 16).max(axis=(1, 3)).reshape(-1)
+    nd_tile = tile_max(st_x.n_done.cpu().numpy(), H, W)
     tiles = sample_tiles(counts, nd_tile)
     assert tiles.size >= 0.1 * np.count_nonzero(counts)
     blen = block_list_lengths(ref["tl"], ref["br"], nth, ntw)

@@ -10,6 +10,8 @@ import pytest
 import torch
 
 from gaussian_splatting_3d_amd import multiview, rasterize, scenes
+from gradmetrics import elementwise_excess, exact_excess, rel_err
+from tilesample import oracle_records, sample_tiles, tile_lists, tile_max, tile_pixels
 
 pytestmark = pytest.mark.gpu
 
@@ -138,3 +140,115 @@ def test_full_size_views_match_single_view():
         assert torch.equal(st.n_done[sl], s1.n_done)
         del out1, s1
         torch.cuda.empty_cache()
+
+
+def test_full_size_4k_batched_adjoint(orc):
+    """Two 3840x2160 views of the 6M-Gaussian workload through the batched
+    forward and adjoint.  View 0 against the oracle: compacted records bit
+    for bit, and on a sample of tiles (every 100th non-empty tile by list
+    length, plus the 16 longest lists and the 16 largest n_done) the tile
+    lists (brute-force membership, stable depth order), the image (fast
+    colour mode, within 1e-4), the entries examined per pixel, and the
+    compositing gradients of a gradient image supported on those tiles
+    (1e-3 max-norm; per element within the oracle's rounding scale, and no
+    further from the float64 adjoint than the oracle's own float formula).
+    Then, for a dense gradient image over both views, each view's
+    compositing gradients equal a single-view adjoint of that view (1e-4),
+    and the raw-parameter gradients of 20000 sampled Gaussians equal the
+    float64 projection adjoint of the per-view gradients summed (1e-3)."""
+    sc = scenes.multiview_6m()
+    cams = sc.cameras[:2]
+    C = sc.C
+    p = [t.to(DEV).contiguous() for t in (sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                                          sc.sh_coeffs)]
+    cfg = scenes.RasterConfig()
+    flat, st = multiview.forward_views(*p, cams, C, cfg)
+    b = st.b
+    c2w, cam = cams[0]
+    ref = oracle_records(orc, sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation, c2w, cam)
+    ic = ref["ic"]
+    H, W, nth, ntw = ic["H"], ic["W"], ic["n_tiles_h"], ic["n_tiles_w"]
+    V0, P0 = int(b.bases[1, 0]), int(b.bases[1, 1])
+    assert (V0, P0) == (ref["idx"].size, ref["P"])
+    assert np.array_equal(b.orig[:V0].cpu().numpy(), ref["idx"])
+    for name, got, want in (("mean2d", b.mean2d[:V0], ref["m2d"]), ("cov2d", b.cov2d[:V0], ref["cov"]),
+                            ("depth", b.depth[:V0], ref["depth"]), ("tl", b.tl[:V0], ref["tl"]),
+                            ("br", b.br[:V0], ref["br"])):
+        g = got.cpu().numpy().reshape(-1)
+        w = np.asarray(want).astype(g.dtype).reshape(-1)
+        assert np.array_equal(g.view(np.uint32), w.view(np.uint32)), name
+    n0 = nth * ntw
+    start0, end0 = b.start[:n0].cpu().numpy(), b.end[:n0].cpu().numpy()
+    counts = np.where(start0 >= 0, end0 - start0, 0).astype(np.int64)
+    assert counts.sum() == P0
+    nd0 = st.n_done[:H * W].cpu().numpy()
+    tiles = sample_tiles(counts, tile_max(nd0, H, W), every=100, top=16)
+    st_o, en_o, ids_o = tile_lists(ref["tl"], ref["br"], ref["depth"], tiles, ntw, n0)
+    ids_g = b.ids.cpu().numpy()
+    for t in tiles:
+        assert np.array_equal(ids_g[start0[t]:end0[t]], ids_o[st_o[t]:en_o[t]]), f"tile {t}"
+    sh_v = np.ascontiguousarray(sc.sh_coeffs.numpy()[ref["idx"]][..., : C * C])
+    al = ref["a"][ref["idx"]]
+    o_out, o_T, o_nd, _ = orc.render_sh_fwd(ref["m2d"], ref["cov"], sh_v, al, st_o, en_o, ids_o, ic["topleft"], c2w,
+                                            16, nth, ntw, ic["psx"], ic["psy"], H, W, C, 1e-4)
+    pix = tile_pixels(tiles, ntw, H, W)
+    pix3 = (pix[:, None] * 3 + np.arange(3)[None, :]).reshape(-1)
+    img0 = flat[:H * W * 3].cpu().numpy()
+    assert np.abs(img0[pix3] - o_out[pix3]).max() <= 1e-4
+    assert np.array_equal(nd0[pix], o_nd[pix])
+
+    # adjoint of a gradient image supported on the sampled tiles of view 0
+    rng = np.random.default_rng(23)
+    gout = torch.zeros(flat.numel(), dtype=torch.float32)
+    gvals = (rng.random(pix3.size).astype(np.float32) - 0.5) * 1e-3
+    gout[torch.from_numpy(pix3)] = torch.from_numpy(gvals)
+    g_np = gout[:H * W * 3].numpy()
+    *_, g_sh, bd = multiview.backward_views(st, *p, gout.to(DEV), boundary=True)
+    assert float(bd["mean2d"][V0:].abs().sum()) == 0.0  # view 1 received no gradient
+    rg, mism, scale = orc.render_sh_bwd(ref["m2d"], ref["cov"], sh_v, al, st_o, en_o, ids_o, o_out, g_np,
+                                        ic["topleft"], c2w, 16, nth, ntw, ic["psx"], ic["psy"], H, W, C, 1e-4,
+                                        want_abs=True)
+    assert mism == 0
+    exact = orc.render_sh_bwd_f64(ref["m2d"], ref["cov"], sh_v, al, st_o, en_o, ids_o, g_np, ic["topleft"], c2w, 16,
+                                  nth, ntw, ic["psx"], ic["psy"], H, W, C, 1e-4)
+    got = dict(mean=bd["mean2d"][:V0].cpu().numpy(), cov=bd["cov"][:V0].cpu().numpy(),
+               alpha=bd["alpha"][:V0].cpu().numpy(), sh=g_sh.cpu().numpy()[ref["idx"]][..., : C * C])
+    for name in ("mean", "cov", "alpha", "sh"):
+        want = np.asarray(rg[name]).reshape(got[name].shape)
+        assert rel_err(got[name], want) <= GRAD_RTOL, name
+        assert elementwise_excess(got[name], want, scale[name], GRAD_RTOL, 1e-5) <= 1.0, name
+        ex = exact_excess(got[name], exact[name], GRAD_RTOL, 1e-6)
+        ex_ref = exact_excess(want, exact[name], GRAD_RTOL, 1e-6)
+        print(f"4K view 0 {name}: excess over the float64 bound {ex:.3g} (reference formula {ex_ref:.3g})")
+        assert ex <= max(1.0, 1.5 * ex_ref), (name, ex, ex_ref)
+
+    # dense gradient image over both views: each view's compositing gradients
+    # equal a single-view adjoint of that view, and the raw-parameter
+    # gradients equal the float64 projection adjoint of their sum over views
+    gd = (torch.rand(flat.numel(), generator=torch.Generator().manual_seed(4)) - 0.5).to(DEV) * 1e-3
+    got2 = multiview.backward_views(st, *p, gd, boundary=True)
+    bd2 = got2[5]
+    orig = b.orig.cpu().numpy()
+    rng = np.random.default_rng(5)
+    subset = np.sort(rng.choice(np.unique(orig), size=20000, replace=False))
+    want2 = [np.zeros((subset.size,) + tuple(t.shape[1:])) for t in p[:4]]
+    sub = [t.numpy()[subset] for t in (sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
+                                       sc.sh_coeffs)]
+    for v, (c2w_v, cam_v) in enumerate(cams):
+        m = b.meta[v]
+        r0, r1 = int(b.bases[v, 0]), int(b.bases[v + 1, 0])
+        out1, s1 = rasterize.forward_view(*p, c2w_v, cam_v, C, scenes.RasterConfig(two_level=False))
+        gs = rasterize.backward_view(s1, *p, gd[3 * m["pix_base"]:3 * (m["pix_base"] + cam_v.h * cam_v.w)])
+        for name in ("mean2d", "cov", "alpha"):
+            assert rel_err(bd2[name][r0:r1].cpu(), gs[5][name].cpu()) <= 1e-4, (v, name)
+        del out1, s1, gs
+        sel = np.nonzero(np.isin(orig[r0:r1], subset))[0]
+        pos = np.searchsorted(subset, orig[r0:r1][sel])
+        w = orc.projection_adjoint(*sub, np.asarray(c2w_v, np.float32), pos, bd2["mean2d"][r0:r1][sel].cpu(),
+                                   bd2["cov"][r0:r1][sel].cpu(), np.zeros((sel.size, 3, C * C), np.float32),
+                                   bd2["alpha"][r0:r1][sel].cpu(), C)
+        for acc, g in zip(want2, w[:4]):
+            acc += g
+    for name, a, w in zip(("mean", "qvec", "log_s", "logit"), got2[:4], want2):
+        assert torch.isfinite(a).all()
+        assert rel_err(a.cpu().numpy()[subset], w) <= GRAD_RTOL, name
