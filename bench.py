@@ -312,20 +312,23 @@ def cpu_baseline(scene, reps=3, tile_frac=0.05):
 KERNEL_PREFIX = {"composite_fwd": "gsmi::k_fwd_sh", "composite_bwd": "gsmi::k_bwd_sh"}
 
 
-def _pmc_pass(counter, child_args, timeout_s=90):
-    """One rocprofv3 counter pass over a child run of this script; returns
-    {kernel name: mean counter value per dispatch} or raises."""
+def _pmc_pass(counters, child_args, timeout_s=90):
+    """One rocprofv3 counter pass (counters: a name or a list of names that
+    fit one pass) over a child run of this script; returns {kernel name:
+    {counter: mean value per dispatch}} or raises."""
     import shutil
     import signal
     import sqlite3
     import subprocess
     import tempfile
 
+    counters = [counters] if isinstance(counters, str) else list(counters)
     exe = shutil.which("rocprofv3")
     if exe is None:
         raise RuntimeError("rocprofv3 not found")
     out = tempfile.mkdtemp(prefix="gsmi_pmc_", dir="/tmp")
-    cmd = [exe, "--pmc", counter, "-d", out, "-o", "run", "--", sys.executable, os.path.abspath(__file__)] + child_args
+    cmd = [exe, "--pmc", *counters, "-d", out, "-o", "run", "--", sys.executable, os.path.abspath(__file__)] + \
+        child_args
     env = dict(os.environ, TMPDIR="/tmp")
     proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
                             start_new_session=True)
@@ -334,16 +337,28 @@ def _pmc_pass(counter, child_args, timeout_s=90):
     except subprocess.TimeoutExpired:
         os.killpg(proc.pid, signal.SIGKILL)
         proc.wait()
-        raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
+        raise RuntimeError(f"rocprofv3 --pmc {counters} timed out")
     db = os.path.join(out, "run_results.db")
     if not os.path.exists(db):
-        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {rc} without results")
+        raise RuntimeError(f"rocprofv3 --pmc {counters} exited {rc} without results")
     con = sqlite3.connect(db)
-    rows = con.execute("select kernel_name, avg(value), count(*) from counters_collection "
-                       "where counter_name = ? group by kernel_name", (counter,)).fetchall()
+    rows = con.execute("select kernel_name, counter_name, avg(value) from counters_collection "
+                       "group by kernel_name, counter_name").fetchall()
     con.close()
     shutil.rmtree(out, ignore_errors=True)
-    return {name: (float(v), int(n)) for name, v, n in rows}
+    res = {}
+    for name, counter, v in rows:
+        res.setdefault(name, {})[counter] = float(v)
+    return res
+
+
+def _kernel_values(run, kernel):
+    """The counter values of the first kernel whose name starts with the
+    dominant kernel's prefix."""
+    for k, vals in run.items():
+        if k.split("void ", 1)[-1].startswith(KERNEL_PREFIX[kernel]):
+            return vals
+    raise RuntimeError(f"{kernel} not seen in the counter run")
 
 
 # Nominal non-packed FP32 VALU issue rate of MI355X: 256 CUs x 4 SIMDs x one
@@ -361,14 +376,34 @@ def measure_valu(workload, exact, kernel, kernel_ms, split=False):
     """Wave-level VALU instructions per launch of `kernel` (rocprofv3
     SQ_INSTS_VALU, summed over the chip) and their issue rate over the
     kernel's HIP-event time, against the non-packed issue peak."""
-    child = _child_args(workload, exact, split)
-    run = _pmc_pass("SQ_INSTS_VALU", child)
-    vals = [v for k, (v, n) in run.items() if k.split("void ", 1)[-1].startswith(KERNEL_PREFIX[kernel])]
-    if not vals:
-        raise RuntimeError(f"{kernel} not seen in the SQ_INSTS_VALU run")
-    rate = vals[0] / (kernel_ms / 1e3)
-    return dict(instructions_per_launch=int(vals[0]), achieved_g_instr_s=round(rate / 1e9, 2),
+    vals = _kernel_values(_pmc_pass("SQ_INSTS_VALU", _child_args(workload, exact, split)), kernel)
+    n = vals["SQ_INSTS_VALU"]
+    rate = n / (kernel_ms / 1e3)
+    return dict(instructions_per_launch=int(n), achieved_g_instr_s=round(rate / 1e9, 2),
                 peak_g_instr_s=round(VALU_PEAK_INSTR_S / 1e9, 1), frac=round(rate / VALU_PEAK_INSTR_S, 4))
+
+
+FP32_PEAK_FLOP_S = 157.3e12  # MI355X FP32 vector peak (published)
+
+
+def measure_flops(workload, exact, kernel, kernel_ms, split=False):
+    """FP32 floating-point operations per launch of `kernel` from one
+    counter pass, as rocprofv3's TOTAL_32_OPS metric counts them without the
+    integer term: 64 lanes x (2 SQ_INSTS_VALU_FMA_F32 + SQ_INSTS_VALU_ADD_F32 +
+    SQ_INSTS_VALU_MUL_F32 + SQ_INSTS_VALU_TRANS_F32) + 512 x
+    SQ_INSTS_VALU_MFMA_MOPS_F32 (every lane counted, active or not), over the
+    kernel's HIP-event time, against the 157.3 TFLOP/s FP32 peak."""
+    names = ["SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_TRANS_F32",
+             "SQ_INSTS_VALU_MFMA_MOPS_F32"]
+    vals = _kernel_values(_pmc_pass(names, _child_args(workload, exact, split)), kernel)
+    fma, add, mul, trans, mops = (vals.get(k, 0.0) for k in names)
+    vec = 64.0 * (2.0 * fma + add + mul + trans)
+    mfma = 512.0 * mops
+    rate = (vec + mfma) / (kernel_ms / 1e3)
+    return dict(flops_per_launch=int(vec + mfma), vector_flops=int(vec), mfma_flops=int(mfma),
+                wave_instr={"fma": int(fma), "add": int(add), "mul": int(mul), "trans": int(trans)},
+                achieved_tflop_s=round(rate / 1e12, 3), peak_tflop_s=FP32_PEAK_FLOP_S / 1e12,
+                frac=round(rate / FP32_PEAK_FLOP_S, 4))
 
 
 def measure_traffic(workload, exact, kernel, split=False):
@@ -380,15 +415,11 @@ def measure_traffic(workload, exact, kernel, split=False):
     child = _child_args(workload, exact, split)
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         cal = _pmc_pass(counter, ["--copy-calibration"])
-        copy_kib = [v for k, (v, n) in cal.items() if "copyBuffer" in k]
+        copy_kib = [v[counter] for k, v in cal.items() if "copyBuffer" in k and counter in v]
         if not copy_kib or copy_kib[0] <= 0:
             raise RuntimeError(f"no copy kernel seen in the {counter} calibration run")
         scale = copy_bytes / copy_kib[0]  # bytes per counter KiB
-        run = _pmc_pass(counter, child)
-        vals = [v for k, (v, n) in run.items() if k.split("void ", 1)[-1].startswith(KERNEL_PREFIX[kernel])]
-        if not vals:
-            raise RuntimeError(f"{kernel} not seen in the {counter} run")
-        res[counter] = (vals[0], scale)
+        res[counter] = (_kernel_values(_pmc_pass(counter, child), kernel)[counter], scale)
     fetch = res["FETCH_SIZE"][0] * res["FETCH_SIZE"][1]
     write = res["WRITE_SIZE"][0] * res["WRITE_SIZE"][1]
     return dict(bytes=fetch + write, fetch_bytes=fetch, write_bytes=write,
@@ -411,7 +442,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the dominant kernel's HBM bytes")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement of the default run")
-    ap.add_argument("--serve-levels", default="0.7,0.85,1,1.15,1.3,1.6,2",
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 8-view 4K step and the training-iteration measurements of the default run")
+    ap.add_argument("--serve-levels", default="0.7,0.85,1,1.15,1.3,1.45,1.6,1.8,2,2.3,2.6",
                     help="server arrival rates as multiples of the plain per-call capacity")
     ap.add_argument("--serve-duration", type=float, default=2.0, help="seconds of arrivals per serving level")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -499,13 +532,35 @@ def main():
         except Exception as e:  # counters unavailable: traffic stays null, say why
             line["roofline"]["traffic_error"] = str(e)
         print(f"[bench] traffic: {line['roofline'].get('traffic')}", file=sys.stderr, flush=True)
-        # the compositing kernels are VALU-issue bound, not HBM bound: report that rate too
+        # the compositing kernels are VALU-issue bound, not HBM bound: report that rate and the FP32 FLOP roof
         try:
             line["roofline"]["valu_issue"] = measure_valu(args.workload, args.exact, dom, line["kernels_ms"][dom],
                                                           args.bwd_split)
         except Exception as e:
             line["roofline"]["valu_issue"] = {"error": str(e)}
         print(f"[bench] valu: {line['roofline']['valu_issue']}", file=sys.stderr, flush=True)
+        try:
+            line["roofline"]["fp32_flops"] = measure_flops(args.workload, args.exact, dom, line["kernels_ms"][dom],
+                                                           args.bwd_split)
+        except Exception as e:
+            line["roofline"]["fp32_flops"] = {"error": str(e)}
+        print(f"[bench] flops: {line['roofline']['fp32_flops']}", file=sys.stderr, flush=True)
+    if args.workload == "dense_3m" and not args.no_extra:
+        # the batched 8-view 4K training step and one whole training iteration,
+        # measured in the same run (a few steps each)
+        mv = make_scene("multiview_6m")
+        print(f"[bench] workload {mv.name}: N={mv.N}", file=sys.stderr, flush=True)
+        rv = gpu_bench_views(mv, 3, 1)
+        line["multiview_6m"] = {"workload": mv.name, "ms_per_step": round(rv["ms_step"], 3),
+                                "mpix_s": round(rv["n_pix"] / (rv["ms_step"] / 1e3) / 1e6, 3), "views": rv["views"],
+                                "pairs": rv["P"], "kernels_ms": {"composite_fwd": round(rv["ms_composite_fwd"], 3),
+                                                                 "composite_bwd": round(rv["ms_composite_bwd"], 3)},
+                                "steps": 3, "warmup": 1}
+        print(f"[bench] multiview: {line['multiview_6m']}", file=sys.stderr, flush=True)
+        del mv
+        ts = train_step(5, 2)
+        line["train_step_3m"] = ts
+        print(f"[bench] training iteration: {ts}", file=sys.stderr, flush=True)
     if args.workload == "dense_3m" and not args.no_serving:
         from gaussian_splatting_3d_amd import loadgen
 
@@ -522,13 +577,14 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def train_main(args):
+def train_step(steps, warmup):
     """One whole training iteration of main_sh.py:141-240 at 3M Gaussians,
     1920x1080: render, loss 0.8 L1 + 0.2 SSIM (window 11, the conf/*.yaml
     values), adjoint, the visibility count and gradient statistics of
     adaptive control (sh_renderer.py:215-216, 602-623), and the Adam step of
     the optimizer the reference rebuilds every iteration (fresh moments,
-    per-group learning rates of conf/garden_sh.yaml:72-76)."""
+    per-group learning rates of conf/garden_sh.yaml:72-76).  Returns the
+    timing record."""
     from gaussian_splatting_3d_amd import densify, rasterize, scenes, training
 
     scene = scenes.dense_3m()
@@ -551,34 +607,40 @@ def train_main(args):
         adam.step(list(g[:5]))
         return loss, st
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         loss, st = step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
+    ms = (t1 - t0) * 1e3 / steps
     assert math.isfinite(float(loss))
+    return {"workload": "train_step_3000k_1920x1080_sh3", "ms_per_step": round(ms, 4), "it_s": round(1e3 / ms, 3),
+            "mpix_s": round(cam.h * cam.w / (ms / 1e3) / 1e6, 3), "gaussians": scene.N, "visible": st.b.V,
+            ("block_pairs" if st.b.shift else "pairs"): st.b.P, "width": cam.w, "height": cam.h,
+            "sh_degree": scene.C - 1, "ssim_mult": 0.2, "ssim_win": 11, "steps": steps, "warmup": warmup}
+
+
+def train_main(args):
+    """The training-iteration workload as its own line."""
+    r = train_step(args.steps, args.warmup)
     line = {
         "metric": "training iterations/s @3M Gaussians 1080p (render, L1+SSIM loss, adjoint, density stats, Adam)",
-        "value": round(1e3 / ms, 3),
+        "value": r["it_s"],
         "unit": "it/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scene and target, random-init Gaussians)",
-        "config": {"workload": "train_step_3000k_1920x1080_sh3", "gaussians": scene.N, "visible": st.b.V,
-                   ("block_pairs" if st.b.shift else "pairs"): st.b.P, "width": cam.w, "height": cam.h,
-                   "sh_degree": scene.C - 1,
-                   "ssim_mult": 0.2, "ssim_win": 11},
-        "mpix_s": round(cam.h * cam.w / (ms / 1e3) / 1e6, 3),
+        "config": {k: r[k] for k in r if k not in ("ms_per_step", "it_s", "mpix_s", "steps", "warmup")},
+        "mpix_s": r["mpix_s"],
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -95,6 +95,13 @@ SIGNATURES = {
     "gsmi_render_views_backward_sh": (_I32, [_I64, _I64, _I32, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P, _P,
                                              _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _P, _I32, _P, _SZ,
                                              _P]),
+    "gsmi_two_level_offsets_views": (_I32, [_I32, _P, _P, _P, _I32, _P, _P, _P, _P, _SZ, _P]),
+    "gsmi_render_views_sh_two_level_record": (_I32, [_I64, _I32, _P, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P,
+                                                     _P, _P, _P, _P, _I32, _P, _P, _P, _P, _I32, _F, _P, _P, _I32,
+                                                     _P]),
+    "gsmi_render_views_backward_sh_lists": (_I32, [_I64, _I64, _I32, _P, _I32, _P, _P, _P, _I64, _I64, _P, _P, _P,
+                                                   _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _F, _I32, _P, _SZ,
+                                                   _P]),
     "gsmi_project_backward_views": (_I32, [_I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                            _P]),
     "gsmi_loss_workspace_size": (_SZ, [_I64]),
@@ -111,11 +118,13 @@ SIGNATURES = {
     "gsmi_prune_alpha_write": (_I32, [_I64, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P, _P, _SZ, _P]),
     "gsmi_make_view": (_I32, [_P, _D, _D, _D, _D, _I32, _I32, _D, _D, _I32, _P]),
     "gsmi_server_create": (_I32, [_I32, _I64, _I32, _I32, _F, _F, _F, _I32, _P]),
+    "gsmi_server_configure": (_I32, [_P, _I64, _I32]),
     "gsmi_server_add_scene": (_I32, [_P, _I64, _I32, _I64, _P, _P, _P, _P, _P, _P]),
     "gsmi_server_submit": (_I32, [_P, _I32, _P, _D, _D, _D, _D, _I32, _I32, _D, _D, _P, _P]),
     "gsmi_server_poll": (_I32, [_P, _P, _P, _P, _P, _P, _I32, _P]),
     "gsmi_server_drain": (_I32, [_P, _D]),
     "gsmi_server_stats": (_I32, [_P, _P]),
+    "gsmi_server_stats_ex": (_I32, [_P, _P, _I32]),
     "gsmi_server_destroy": (_I32, [_P]),
     "gsmi_exp_spec": (_I32, [_I64, _P, _P, _P]),
 }

@@ -15,10 +15,12 @@
 // pixels be rejected from q * (1/det) alone: for r > rlim the product a*G is
 // below 1/255 by a relative margin of 5e-4, far above the few-ulp gap between
 // q*(1/det) and q/det.  Everything else takes the exact path.
+#include <algorithm>
 #include <mutex>
 
 #include "gsmi_device.h"
 #include "scan.h"
+#include "serve_internal.h"
 
 namespace gsmi {
 
@@ -83,6 +85,22 @@ struct RasterArgs {
   int32_t* fids;
   int32_t* fend;
   int super_shift, n_super_w;
+  // Multi-view forward of mixed scenes (render server): view b reads its SH
+  // rows from view_sh[b] (null: sh) and writes its [H, W, 3] image to
+  // view_out[b] (null: out at the view's pix_base).
+  const float* const* view_sh;
+  float* const* view_out;
+  // Multi-view lists kept per view (the batched training path's recorded
+  // two-level lists): view b's list entries live at view_ids[b] (read) and the
+  // forward records them to view_fids[b], with start/end / foff/fend
+  // relative to that view's array; null: the shared ids / fids.
+  const int32_t* const* view_ids;
+  int32_t* const* view_fids;
+  // Work queue (forward): null, one tile per workgroup; else a persistent
+  // launch whose workgroups take tiles from 8 counters work[x], one per band
+  // of ceil(n_tiles_total / 8) consecutive tiles, starting with band
+  // blockIdx.x % 8 and then the others; the counters start at 0.
+  int* work;
 };
 
 // Pixel of this thread in the block's tile.
@@ -92,9 +110,12 @@ struct TileCtx {
   int gx, gy;
   bool inside;
   int64_t pix;  // flat pixel index into out / T_final / n_done / grad_out
+  int64_t lpix;  // pixel index within the view (gy * W + gx)
   float px, py;
   float cw[9];  // first 9 floats of the view's c2w buffer (they feed the ray)
 };
+
+__device__ __forceinline__ bool tile_setup_at(const RasterArgs& a, TileCtx& c, int tile);
 
 __device__ __forceinline__ bool tile_setup(const RasterArgs& a, TileCtx& c) {
   int tile;
@@ -104,6 +125,11 @@ __device__ __forceinline__ bool tile_setup(const RasterArgs& a, TileCtx& c) {
   } else {
     tile = xcd_tile(blockIdx.x, a.n_tiles_total);
   }
+  return tile_setup_at(a, c, tile);
+}
+
+// Setup for a given tile id (the work-queue launch takes tiles directly).
+__device__ __forceinline__ bool tile_setup_at(const RasterArgs& a, TileCtx& c, int tile) {
   if (tile < 0 || tile >= a.n_tiles_total) return false;
   int local = tile, ntw = a.n_tiles_w, W = a.W, H = a.H;
   float tlx, tly, psx = a.psx, psy = a.psy;
@@ -137,7 +163,8 @@ __device__ __forceinline__ bool tile_setup(const RasterArgs& a, TileCtx& c) {
   c.gx = tx * kTile + (t & 15);
   c.gy = ty * kTile + (t >> 4);
   c.inside = c.gx < W && c.gy < H;
-  c.pix = pix_base + (int64_t)c.gy * W + c.gx;
+  c.lpix = (int64_t)c.gy * W + c.gx;
+  c.pix = pix_base + c.lpix;
   c.px = tlx + (float)c.gx * psx;
   c.py = tly + (float)c.gy * psy;
   return true;
@@ -266,11 +293,12 @@ __device__ __forceinline__ float dot_fast(const float* a, const float* b) {
 // Stage entries [s0+base, s0+base+cnt) of the tile list into LDS (without
 // WITH_SH only the geometry and opacity).
 template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true>
-__device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
-                                            GeoB* s_gb, float2* s_gc, float* s_sh) {
+__device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const float* __restrict__ shb,
+                                                 const int32_t* __restrict__ idsb, int s0, int base, int cnt,
+                                                 int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc, float* s_sh) {
   const int t = threadIdx.x;
   if (t < cnt) {
-    const int g = IDS_IN_LDS ? s_id[t] : a.ids[s0 + base + t];
+    const int g = IDS_IN_LDS ? s_id[t] : idsb[s0 + base + t];
     s_id[t] = g;
     const float2 m = reinterpret_cast<const float2*>(a.mean)[g];
     const float4 c = reinterpret_cast<const float4*>(a.cov)[g];
@@ -289,8 +317,14 @@ __device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int bas
     const int k = r - ch * CC;
     const int g = s_id[j];
     const int64_t row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
-    s_sh[e] = a.sh[row * a.sh_sg + ch * a.sh_sc + k];
+    s_sh[e] = shb[row * a.sh_sg + ch * a.sh_sc + k];
   }
+}
+
+template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true>
+__device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
+                                            GeoB* s_gb, float2* s_gc, float* s_sh) {
+  stage_batch_from<SHN, CC, IDS_IN_LDS, WITH_SH>(a, a.sh, a.ids, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
 }
 
 // Two-level lists: the next (up to kBatch) entries of the block-of-tiles list
@@ -344,8 +378,8 @@ __device__ __forceinline__ int filter_batch(const RasterArgs& a, int e0, int tx,
 // ---------------------------------------------------------------------------
 // TWO: the tile's list is filtered on the fly from its block's list (see
 // filter_batch) and recorded for the backward; single view, no background.
-template <int C, bool BG, bool EXACT, bool TWO = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_fwd_sh(RasterArgs a) {
+template <int C, bool BG, bool EXACT, bool TWO>
+__device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
   static_assert(!(TWO && BG), "two-level lists without background");
@@ -357,11 +391,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   __shared__ int s_fq[TWO ? kFilterQueue : 1];
   __shared__ int s_fwh[8];
 
-  TileCtx tc;
-  if (!tile_setup(a, tc)) return;
   const int tile = tc.tile;
   const bool inside = tc.inside;
   const int64_t pix = tc.pix;
+  // the view's SH rows and output image (mixed-scene batches), else the shared ones
+  const float* shb = a.view_sh ? a.view_sh[tc.view] : a.sh;
+  float* outb = a.view_out ? a.view_out[tc.view] : a.out;
+  const int64_t opix = a.view_out ? tc.lpix : pix;
+  const int32_t* idsb = a.view_ids ? a.view_ids[tc.view] : a.ids;
+  int32_t* fidsb = a.view_fids ? a.view_fids[tc.view] : a.fids;  // TWO: where the tile's list is recorded
   const int tx = tc.gx / kTile, ty = tc.gy / kTile;  // the view's own tile coordinates (TWO only)
   int s0, e0;
   if constexpr (TWO) {
@@ -384,13 +422,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
     if (inside) {
       if (BG && s0 == -1) {
-        a.out[3 * pix + 0] = a.bg[0];
-        a.out[3 * pix + 1] = a.bg[1];
-        a.out[3 * pix + 2] = a.bg[2];
+        outb[3 * opix + 0] = a.bg[0];
+        outb[3 * opix + 1] = a.bg[1];
+        outb[3 * opix + 2] = a.bg[2];
       } else if (a.write_empty) {
-        a.out[3 * pix + 0] = 0.0f;
-        a.out[3 * pix + 1] = 0.0f;
-        a.out[3 * pix + 2] = 0.0f;
+        outb[3 * opix + 0] = 0.0f;
+        outb[3 * opix + 1] = 0.0f;
+        outb[3 * opix + 2] = 0.0f;
       }
       if (a.T_final) a.T_final[pix] = 1.0f;
       if (a.n_done) a.n_done[pix] = 0;
@@ -428,12 +466,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     if constexpr (TWO) {
       cnt = filter_batch(a, e0, tx, ty, s_id, s_fq, s_fwh, fcur, fqh, fqn, fpar);
       if (cnt == 0) break;
-      if (a.fids && (int)threadIdx.x < cnt) a.fids[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
-      stage_batch<SHN, CC, true>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+      if (fidsb && (int)threadIdx.x < cnt) fidsb[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
+      stage_batch_from<SHN, CC, true>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
     } else {
       if (base >= n) break;
       cnt = min(kBatch, n - base);
-      stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+      stage_batch_from<SHN, CC>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
     }
     __syncthreads();
     // entries of the batch that can reach a pixel of this wave, in order
@@ -510,11 +548,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       o2 = o2 * T + a.bg[2] * omt;
     }
   }
-  a.out[3 * pix + 0] = o0;
-  a.out[3 * pix + 1] = o1;
-  a.out[3 * pix + 2] = o2;
+  outb[3 * opix + 0] = o0;
+  outb[3 * opix + 1] = o1;
+  outb[3 * opix + 2] = o2;
   if (a.T_final) a.T_final[pix] = T;
   if (a.n_done) a.n_done[pix] = ndone;
+}
+
+// Next tile of a work-queue launch: the band of this workgroup's XCD first
+// (consecutive workgroups land on consecutive XCDs, and a band is a run of
+// neighbouring tiles, which share Gaussians in that XCD's L2), then the other
+// bands in turn; -1 once every band is taken.  Block-uniform.
+__device__ __forceinline__ int next_queued_tile(const RasterArgs& a, int& band, int* s_item) {
+  const int per = (a.n_tiles_total + 7) >> 3;
+  for (;;) {
+    __syncthreads();  // everyone has read the previous item
+    if (threadIdx.x == 0) {
+      const int b = (blockIdx.x + band) & 7;
+      const int k = atomicAdd(a.work + b, 1);
+      *s_item = (k < per && b * per + k < a.n_tiles_total) ? b * per + k : -1;
+    }
+    __syncthreads();
+    const int item = *s_item;
+    if (item >= 0) return item;
+    if (++band == 8) return -1;
+  }
+}
+
+template <int C, bool BG, bool EXACT, bool TWO = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_fwd_sh(RasterArgs a) {
+  TileCtx tc;
+  if (!a.work) {
+    if (tile_setup(a, tc)) fwd_sh_tile<C, BG, EXACT, TWO>(a, tc);
+    return;
+  }
+  __shared__ int s_item;
+  int band = 0;
+  for (;;) {
+    const int tile = next_queued_tile(a, band, &s_item);
+    if (tile < 0) return;
+    if (tile_setup_at(a, tc, tile)) fwd_sh_tile<C, BG, EXACT, TWO>(a, tc);
+  }
 }
 
 __device__ __forceinline__ void stage_rgb(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
@@ -779,6 +853,7 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
   const int s0 = a.start[tile];
   const int e0 = a.end[tile];
   if (s0 == -1 || e0 - s0 == 0) return;
+  const int32_t* idsb = a.view_ids ? a.view_ids[tc.view] : a.ids;  // the tile's list entries
 
   const float px = tc.px;
   const float py = tc.py;
@@ -825,7 +900,7 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
     const int cnt = min(kBatch, nmax - base);
     __syncthreads();
     for (int e = t; e < cnt * NCOMP; e += kTilePixels) s_acc[e] = 0.0f;
-    stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    stage_batch_from<SHN, CC>(a, a.sh, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
     __syncthreads();
     for (int j = cnt - 1; j >= 0; --j) {
       const int idx = base + j;
@@ -1008,6 +1083,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   const int s0 = a.start[tile];
   const int e0 = a.end[tile];
   if (s0 == -1 || e0 - s0 == 0) return;
+  const int32_t* idsb = a.view_ids ? a.view_ids[tc.view] : a.ids;  // the tile's list entries
 
   const float px = tc.px;
   const float py = tc.py;
@@ -1112,7 +1188,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int cnt = min(kMfmaBatch, nmax - base);
     __syncthreads();
     if (t == 0) s_touch = 0u;
-    stage_batch<SHN, CC>(a, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    stage_batch_from<SHN, CC>(a, a.sh, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
     __syncthreads();
     int n_slots = 0;      // filled slots of the current chunk (wave-uniform)
     uint32_t slots = 0;   // batch position of slot m in bits [kSlotBits m, kSlotBits (m + 1))
@@ -2051,22 +2127,22 @@ extern "C" int gsmi_render_views_sh(int64_t V, int64_t P, int B, const gsmi_view
   }
 }
 
-extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view* views,
-                                             int n_tiles_total, const float* mean, const float* cov,
-                                             const float* sh, int64_t sh_stride_g, int64_t sh_stride_c,
-                                             const int32_t* sh_index, const float* alpha, const int32_t* start,
-                                             const int32_t* end, const int32_t* gaussian_ids, const float* out,
-                                             const float* T_final, const int32_t* n_done, const float* pre_bg,
-                                             const float* grad_out, float* grad_mean, float* grad_cov,
-                                             float* grad_sh, float* grad_alpha, int C, float thresh,
-                                             const float* bg, int flags, void* workspace, size_t workspace_bytes,
-                                             void* stream) {
+// The multi-view adjoint over shared lists (gaussian_ids) or per-view lists
+// (view_ids: device array of B pointers; start/end relative to each).
+static int views_backward_impl(int64_t V, int64_t P, int B, const gsmi_view* views, int n_tiles_total,
+                               const float* mean, const float* cov, const float* sh, int64_t sh_stride_g,
+                               int64_t sh_stride_c, const int32_t* sh_index, const float* alpha, const int32_t* start,
+                               const int32_t* end, const int32_t* gaussian_ids, const int32_t* const* view_ids,
+                               const float* out, const float* T_final, const int32_t* n_done, const float* pre_bg,
+                               const float* grad_out, float* grad_mean, float* grad_cov, float* grad_sh,
+                               float* grad_alpha, int C, float thresh, const float* bg, int flags, void* workspace,
+                               size_t workspace_bytes, hipStream_t st) {
   int rc = check_views(V, P, B, views, n_tiles_total);
   if (rc) return rc;
   if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
   if (!start || !end || !grad_out) return GSMI_ERR_ARG;
-  if (P > 0 && (!mean || !cov || !sh || !alpha || !gaussian_ids || !grad_mean || !grad_cov || !grad_sh ||
-                !grad_alpha))
+  if (P > 0 && (!mean || !cov || !sh || !alpha || !(gaussian_ids || view_ids) || !grad_mean || !grad_cov ||
+                !grad_sh || !grad_alpha))
     return GSMI_ERR_ARG;
   bool recomp;
   rc = check_bwd_state(out, T_final, n_done, pre_bg, bg, flags, workspace, &recomp);
@@ -2082,6 +2158,7 @@ extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const 
   a.start = start;
   a.end = end;
   a.ids = gaussian_ids;
+  a.view_ids = view_ids;
   a.bg = bg;
   a.out_saved = out;
   a.T_saved = T_final;
@@ -2098,13 +2175,44 @@ extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const 
   a.views = views;
   a.n_views = B;
   a.n_tiles_total = n_tiles_total;
-  hipStream_t st = (hipStream_t)stream;
   if (workspace) {
     if (workspace_bytes < order_ws_bytes(n_tiles_total)) return GSMI_ERR_WORKSPACE;
     rc = make_tile_order(a, workspace, st);
     if (rc) return rc;
   }
   return dispatch_bwd_sh(a, C, (flags & GSMI_BWD_SPLIT_BF16) != 0, recomp, st);
+}
+
+extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view* views,
+                                             int n_tiles_total, const float* mean, const float* cov,
+                                             const float* sh, int64_t sh_stride_g, int64_t sh_stride_c,
+                                             const int32_t* sh_index, const float* alpha, const int32_t* start,
+                                             const int32_t* end, const int32_t* gaussian_ids, const float* out,
+                                             const float* T_final, const int32_t* n_done, const float* pre_bg,
+                                             const float* grad_out, float* grad_mean, float* grad_cov,
+                                             float* grad_sh, float* grad_alpha, int C, float thresh,
+                                             const float* bg, int flags, void* workspace, size_t workspace_bytes,
+                                             void* stream) {
+  return views_backward_impl(V, P, B, views, n_tiles_total, mean, cov, sh, sh_stride_g, sh_stride_c, sh_index, alpha,
+                             start, end, gaussian_ids, nullptr, out, T_final, n_done, pre_bg, grad_out, grad_mean,
+                             grad_cov, grad_sh, grad_alpha, C, thresh, bg, flags, workspace, workspace_bytes,
+                             (hipStream_t)stream);
+}
+
+extern "C" int gsmi_render_views_backward_sh_lists(int64_t V, int64_t P, int B, const gsmi_view* views,
+                                                   int n_tiles_total, const float* mean, const float* cov,
+                                                   const float* sh, int64_t sh_stride_g, int64_t sh_stride_c,
+                                                   const int32_t* sh_index, const float* alpha, const int32_t* start,
+                                                   const int32_t* end, const int32_t* const* view_ids,
+                                                   const float* T_final, const int32_t* n_done,
+                                                   const float* grad_out, float* grad_mean, float* grad_cov,
+                                                   float* grad_sh, float* grad_alpha, int C, float thresh, int flags,
+                                                   void* workspace, size_t workspace_bytes, void* stream) {
+  if (!view_ids || !T_final || !n_done) return GSMI_ERR_ARG;
+  return views_backward_impl(V, P, B, views, n_tiles_total, mean, cov, sh, sh_stride_g, sh_stride_c, sh_index, alpha,
+                             start, end, nullptr, view_ids, nullptr, T_final, n_done, nullptr, grad_out, grad_mean,
+                             grad_cov, grad_sh, grad_alpha, C, thresh, nullptr, flags, workspace, workspace_bytes,
+                             (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -2122,10 +2230,34 @@ __global__ void k_tile_caps(int n_tiles_h, int n_tiles_w, int shift, int n_super
   const int s0 = sstart[sb], e0 = send[sb];
   cap[t] = (s0 == -1 || e0 <= s0) ? 0 : e0 - s0;
 }
+
+// Per tile of a multi-view batch: the length of its block's list.
+__global__ void k_tile_caps_views(const gsmi_view* __restrict__ views, const gsmi_view* __restrict__ bviews, int B,
+                                  int n_tiles_total, int shift, const int32_t* __restrict__ bstart,
+                                  const int32_t* __restrict__ bend, int32_t* __restrict__ cap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles_total) return;
+  const int v = view_of_tile(views, B, t);
+  const int local = t - views[v].tile_base;
+  const int ntw = views[v].n_tiles_w;
+  const int tx = local % ntw, ty = local / ntw;
+  const int sb = bviews[v].tile_base + (ty >> shift) * bviews[v].n_tiles_w + (tx >> shift);
+  const int s0 = bstart[sb], e0 = bend[sb];
+  cap[t] = (s0 == -1 || e0 <= s0) ? 0 : e0 - s0;
+}
 }  // namespace gsmi
 
-static int launch_fwd_two(const RasterArgs& a, int C, int exact, hipStream_t st) {
-  const int grid = xcd_grid(a.n_tiles_total);
+__global__ void k_zero_work(int* w) {
+  if (threadIdx.x < 8) w[threadIdx.x] = 0;
+}
+
+// grid_cap > 0 with a.work: a work-queue launch of at most grid_cap workgroups.
+static int launch_fwd_two(const RasterArgs& a, int C, int exact, hipStream_t st, int grid_cap = 0) {
+  int grid = xcd_grid(a.n_tiles_total);
+  if (a.work) {
+    hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, st, a.work);  // a fill kernel, not a memset node
+    grid = std::min(grid, std::max(grid_cap, 8));
+  }
 #define GSMI_FWD2(CC_)                                                                      \
   do {                                                                                       \
     if (exact)                                                                               \
@@ -2142,6 +2274,122 @@ static int launch_fwd_two(const RasterArgs& a, int C, int exact, hipStream_t st)
 #undef GSMI_FWD2
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
+}
+
+extern "C" int gsmi_two_level_offsets_views(int B, const gsmi_view* views, const gsmi_view* block_views,
+                                            const int32_t* view_tiles, int shift, const int32_t* bstart,
+                                            const int32_t* bend, int32_t* foff, void* workspace,
+                                            size_t workspace_bytes, void* stream) {
+  if (B <= 0 || !views || !block_views || !view_tiles || shift < 0 || shift > 4 || !bstart || !bend || !foff)
+    return GSMI_ERR_ARG;
+  int64_t n = 0;
+  int n_max = 0;
+  for (int b = 0; b < B; ++b) {
+    if (view_tiles[b] <= 0) return GSMI_ERR_ARG;
+    n += view_tiles[b];
+    n_max = std::max(n_max, (int)view_tiles[b]);
+  }
+  if (n > (1 << 30)) return GSMI_ERR_UNSUPPORTED;
+  if (!workspace || workspace_bytes < gsmi_two_level_offsets_workspace_size(n_max)) return GSMI_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_tile_caps_views, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, views, block_views, B,
+                     (int)n, shift, bstart, bend, foff);
+  GSMI_CHECK_LAUNCH();
+  int64_t base = 0;  // each view's offsets start at 0 in its own list array
+  for (int b = 0; b < B; ++b) {
+    const int rc = exclusive_scan<int32_t>(foff + base, foff + base, view_tiles[b], nullptr, workspace, st);
+    if (rc) return rc;
+    base += view_tiles[b];
+  }
+  return GSMI_OK;
+}
+
+extern "C" int gsmi_render_views_sh_two_level_record(
+    int64_t V, int B, const gsmi_view* views, const gsmi_view* block_views, int n_tiles_total, const float* mean,
+    const float* cov, const float* sh, int64_t sh_stride_g, int64_t sh_stride_c, const int32_t* sh_index,
+    const float* alpha, const int32_t* tl, const int32_t* br, const int32_t* bstart, const int32_t* bend,
+    const int32_t* bids, int shift, const int32_t* foff, int32_t* const* view_fids, int32_t* fend, float* out, int C,
+    float thresh, float* T_final, int32_t* n_done, int exact, void* stream) {
+  int rc = check_views(V, 0, B, views, n_tiles_total);
+  if (rc) return rc;
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (shift < 0 || shift > 4) return GSMI_ERR_ARG;
+  if (!block_views || ((uintptr_t)block_views & 7) || !bstart || !bend || !out || !foff || !view_fids || !fend)
+    return GSMI_ERR_ARG;
+  if (V > 0 && (!mean || !cov || !sh || !alpha || !tl || !br || !bids)) return GSMI_ERR_ARG;
+  if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
+  if (((uintptr_t)mean & 7) || ((uintptr_t)cov & 15) || ((uintptr_t)tl & 7) || ((uintptr_t)br & 7))
+    return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh = sh;
+  a.sh_index = sh_index;
+  a.alpha = alpha;
+  a.out = out;
+  a.T_final = T_final;
+  a.n_done = n_done;
+  a.sh_sg = sh_stride_g;
+  a.sh_sc = sh_stride_c;
+  a.thresh = thresh;
+  a.write_empty = 1;
+  a.views = views;
+  a.n_views = B;
+  a.n_tiles_total = n_tiles_total;
+  a.bviews = block_views;
+  a.sstart = bstart;
+  a.send = bend;
+  a.sids = bids;
+  a.rtl = tl;
+  a.rbr = br;
+  a.super_shift = shift;
+  a.foff = foff;
+  a.view_fids = view_fids;
+  a.fend = fend;
+  return launch_fwd_two(a, C, exact, (hipStream_t)stream);
+}
+
+// Forward of a render-server batch (declared in serve_internal.h): as
+// gsmi_render_views_sh_two_level with each view's SH rows at view_sh[b] and
+// image written to view_out[b] ([H, W, 3]), no T_final / n_done; with work
+// (device int[8]) and grid_cap > 0 the tiles are taken from per-XCD work
+// queues by at most grid_cap workgroups.
+int gsmi_internal_render_served(int64_t V, int B, const gsmi_view* views, const gsmi_view* block_views,
+                                  int n_tiles_total, const float* mean, const float* cov,
+                                  const float* const* view_sh, int64_t sh_stride_g, int64_t sh_stride_c,
+                                  const int32_t* sh_index, const float* alpha, const int32_t* tl, const int32_t* br,
+                                  const int32_t* bstart, const int32_t* bend, const int32_t* bids, int shift,
+                                  float* const* view_out, int C, float thresh, int exact, int* work, int grid_cap,
+                                  hipStream_t st) {
+  int rc = check_views(V, 0, B, views, n_tiles_total);
+  if (rc) return rc;
+  if (C < 1 || C > kMaxC) return GSMI_ERR_UNSUPPORTED;
+  if (shift < 0 || shift > 4 || !block_views || !bstart || !bend || !view_sh || !view_out) return GSMI_ERR_ARG;
+  if (V > 0 && (!mean || !cov || !alpha || !tl || !br || !bids || !sh_index)) return GSMI_ERR_ARG;
+  if (sh_stride_c < C * C || sh_stride_g < 3 * sh_stride_c) return GSMI_ERR_ARG;
+  RasterArgs a{};
+  a.mean = mean;
+  a.cov = cov;
+  a.sh_index = sh_index;
+  a.alpha = alpha;
+  a.sh_sg = sh_stride_g;
+  a.sh_sc = sh_stride_c;
+  a.thresh = thresh;
+  a.write_empty = 1;
+  a.views = views;
+  a.n_views = B;
+  a.n_tiles_total = n_tiles_total;
+  a.bviews = block_views;
+  a.sstart = bstart;
+  a.send = bend;
+  a.sids = bids;
+  a.rtl = tl;
+  a.rbr = br;
+  a.super_shift = shift;
+  a.view_sh = view_sh;
+  a.view_out = view_out;
+  a.work = grid_cap > 0 ? work : nullptr;
+  return launch_fwd_two(a, C, exact, st, grid_cap);
 }
 
 extern "C" size_t gsmi_two_level_offsets_workspace_size(int n_tiles) {

@@ -3,22 +3,33 @@
 // The reference serves viewers by calling SHRenderer.forward once per client
 // in turn (utils/viewer/viser_viewer.py:118-158, vis.py:18-32).  Here scenes
 // stay resident in HBM (caller-owned device tensors) and camera requests are
-// queued; a dispatcher thread packs up to max_views pending requests of one
-// scene into a batch and runs the multi-view pipeline on one of n_slots HIP
-// streams, so a new batch is counted, binned and composited while earlier
-// batches are still on the GPU.  Every batch draws its buffers from one
-// device arena allocated at start-up (no allocation per request):
-//   phase A (before the pair count is known): view records, [B,N] visibility
-//     and offsets, count workspace;
-//   phase B (after one readback of the per-view bases): compacted records,
-//     tile lists, binning workspace, the flat image.
-// Admission: after the count, the batch keeps the longest prefix of its views
-// whose phase-B bytes fit the arena (and whose pair count fits int32); the
-// other views go back to the front of the queue.  A request that cannot fit
-// even alone while nothing else holds memory completes with
-// GSMI_ERR_WORKSPACE.  Each view's image is exactly what the single-view
-// pipeline returns for that camera: the batch runs the same per-view
-// arithmetic, only the launches are shared.
+// queued.  n_slots worker threads each own a HIP stream; a free worker takes
+// the oldest pending request and, in arrival order, the other pending
+// requests whose scenes share its SH layout (scenes may mix), up to
+// max_views views and max_batch_pixels pixels (the cap bounds how long one
+// batch holds the GPU, which bounds the latency of the requests behind it).
+// The batch runs the multi-view pipeline: the views are grouped by scene,
+// each group is counted (cull, project, pairs per 64-px block) against its
+// scene, one readback gives the pair counts, then each group's records are
+// emitted into one record array, all views are binned together and one
+// compositing launch over every tile of every view writes each image
+// straight into its request's buffer (per-view SH rows and output pointers);
+// that launch is a persistent kernel whose workgroups take tiles from
+// per-XCD work queues.  Several workers keep several batches in flight on
+// the GPU; no thread waits on a batch but the worker that owns it.
+//
+// Memory: one device arena allocated at start-up; a batch takes
+//   phase A (before the pair counts are known): view records, per-group
+//     [B_g, N_g] visibility and offsets, counts and count workspace;
+//   phase B (after the readback): combined bases, per-view pointers, work
+//     queues, compacted records, block lists, binning workspace.
+// Admission: after the count the batch keeps the longest prefix of its
+// views whose phase-B bytes fit the arena's largest free block (and whose
+// pair count fits int32); the others go back to the front of the queue.  A
+// request that cannot fit even alone while nothing else holds memory
+// completes with GSMI_ERR_WORKSPACE.  Each image is exactly what the
+// single-view pipeline returns for that camera: the batch runs the same
+// per-view arithmetic, only the launches are shared.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +44,7 @@
 #include <vector>
 
 #include "../../include/gsmi.h"
+#include "serve_internal.h"
 
 namespace gsmi_serve {
 
@@ -106,6 +118,7 @@ struct Request {
   gsmi_view view;  // tile_base / pix_base set when batched
   float* out;      // caller's device buffer [H*W*3]
   double t_submit;
+  int64_t pixels;
 };
 
 struct Done {
@@ -115,38 +128,28 @@ struct Done {
   int batch_views;
 };
 
-struct Slot {
-  hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
-  gsmi_view* h_views = nullptr;  // pinned staging
-  int64_t* h_bases = nullptr;    // pinned readback
-  std::vector<Request> reqs;
-  size_t offA = 0, bytesA = 0, offB = 0, bytesB = 0;
-  bool busy = false;
-  bool has_work = false;
-  std::thread waiter;
-};
-
 // Pairs are binned per block of 2^kBlockShift x 2^kBlockShift tiles; each
 // tile filters its block's list while compositing (the same list, so the
 // same image as per-tile binning, with a sixth of the pairs to sort at 1080p).
 constexpr int kBlockShift = 2;
 
-// Bytes of phase A / phase B for a batch (layout carved in run_batch); P
-// counts (record, block) pairs, n_blocks the blocks of all views.
-static size_t phase_a_bytes(int64_t N, int B) {
-  return align_up((size_t)(2 * B) * sizeof(gsmi_view)) + align_up((size_t)(B * N)) +
-         align_up((size_t)(B * N) * 4) + align_up((size_t)(2 * (B + 1)) * 8) +
-         align_up(gsmi_preprocess_views_workspace_size(N, B));
-}
-
-static size_t phase_b_bytes(int64_t V, int64_t P, int64_t n_blocks, int64_t n_pix) {
-  return align_up(V * 8) + align_up(V * 16) + align_up(V * 4) * 2 + align_up(V * 8) * 4 + align_up(V * 4) +
-         align_up(P * 4) + align_up(n_blocks * 4) * 2 + align_up(gsmi_binning_workspace_size(V, P, (int)n_blocks)) +
-         align_up(n_pix * 12);
-}
-
 static int blocks_of(int n_tiles) { return (n_tiles + (1 << kBlockShift) - 1) >> kBlockShift; }
+
+// A scene group of a batch: views [v0, v0 + nv) render scene `scene`.
+struct Group {
+  int scene, v0, nv;
+  size_t vis_off, voff_off, bases_off, ws_off, ws_bytes;  // phase-A offsets
+};
+
+struct Worker {
+  hipStream_t stream = nullptr;
+  hipEvent_t counted = nullptr, done = nullptr;
+  gsmi_view* h_views = nullptr;  // pinned: [B] views, then [B] block views
+  int64_t* h_lbases = nullptr;   // pinned: per group 2 (nv + 1) local bases, groups back to back
+  int64_t* h_bases = nullptr;    // pinned: 2 (B + 1) combined bases
+  void** h_ptrs = nullptr;       // pinned: [B] SH pointers, then [B] output pointers
+  std::thread th;
+};
 
 }  // namespace gsmi_serve
 
@@ -155,34 +158,32 @@ using namespace gsmi_serve;
 struct gsmi_server {
   int device = 0;
   int max_views = 32;
+  int64_t max_batch_pixels = int64_t(1) << 40;
+  int queue_grid = 0;  // work-queue compositing: workgroups (0: one per tile)
   float D = 6.0f, frustum_radius = 1.0f, thresh = 1e-4f;
   int exact = 0;
   void* arena_mem = nullptr;
   Arena arena;
   std::vector<Scene> scenes;
-  std::vector<Slot> slots;
+  std::vector<Worker> workers;
   std::mutex mu;
   std::condition_variable cv;
+  // Held by the one batch that is acquiring memory, from its phase-A
+  // allocation to its phase-B allocation: every other batch in flight then
+  // holds all its memory and will release it, so waiting cannot deadlock.
+  std::mutex acquire_mu;
   std::deque<Request> pending;
   std::deque<Done> done;
-  std::thread dispatcher;
   bool stop = false;
   int64_t next_id = 0;
-  int in_flight = 0;
+  int in_flight = 0;  // batches taken by workers and not finished
   Clock::time_point t0;
-  int64_t stat_batches = 0, stat_views = 0, stat_max_batch = 0, stat_splits = 0, stat_failed = 0;
+  int64_t stat_batches = 0, stat_views = 0, stat_max_batch = 0, stat_splits = 0, stat_failed = 0, stat_mixed = 0;
 
   double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
 };
 
 namespace gsmi_serve {
-
-// Batches launched and not yet completed (their memory will be released).
-static bool any_launched(gsmi_server* s) {
-  for (auto& sl : s->slots)
-    if (sl.has_work) return true;
-  return false;
-}
 
 static void fail_requests(gsmi_server* s, const std::vector<Request>& reqs, int status) {
   const double t = s->now_ms();
@@ -192,13 +193,14 @@ static void fail_requests(gsmi_server* s, const std::vector<Request>& reqs, int 
   s->cv.notify_all();
 }
 
-// Allocates n bytes from the arena, waiting for in-flight batches to release
-// memory; false when it cannot fit even with nothing else in flight.
+// Allocates n bytes from the arena, waiting for other batches to release
+// memory; false when it cannot fit even with no other batch in flight.
+// Called with acquire_mu held.
 static bool arena_take(gsmi_server* s, size_t n, size_t* off) {
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     if (s->arena.alloc(n, off)) return true;
-    if (!any_launched(s)) return false;
+    if (s->in_flight <= 1) return false;  // only this batch: waiting frees nothing
     s->cv.wait(lk);
   }
 }
@@ -209,64 +211,128 @@ static void arena_give(gsmi_server* s, size_t off, size_t n) {
   s->cv.notify_all();
 }
 
-// Runs one batch on slot `sl` up to the launch of its last kernel; returns
-// false when the requests were failed or re-queued instead.
-static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
-  const Scene sc = s->scenes[reqs[0].scene];
-  const int64_t N = sc.N;
-  int B = (int)reqs.size();
-  // tile / block / pixel numbering of the batch; h_views[B + b] is view b
-  // with its block grid
-  const int B0 = B;
-  int64_t tile_base = 0, block_base = 0, pix_base = 0;
-  for (int b = 0; b < B; ++b) {
-    gsmi_view& v = reqs[b].view;
-    v.tile_base = (int32_t)tile_base;
-    v.pix_base = pix_base;
-    tile_base += (int64_t)v.n_tiles_w * v.n_tiles_h;
-    pix_base += (int64_t)v.W * v.H;
-    sl.h_views[b] = v;
-    gsmi_view bv = v;
-    bv.n_tiles_w = blocks_of(v.n_tiles_w);
-    bv.n_tiles_h = blocks_of(v.n_tiles_h);
-    bv.tile_base = (int32_t)block_base;
-    block_base += (int64_t)bv.n_tiles_w * bv.n_tiles_h;
-    sl.h_views[B + b] = bv;
+static bool same_layout(const Scene& a, const Scene& b) { return a.C == b.C && a.sh_stride_c == b.sh_stride_c; }
+
+// Takes a batch from the queue (caller holds the lock; pending not empty):
+// the oldest request, then in arrival order the requests of scenes with the
+// same SH layout while the view and pixel budgets allow.  Sorted by scene
+// (stable), so each scene's views are contiguous.
+static std::vector<Request> take_batch(gsmi_server* s) {
+  std::vector<Request> batch;
+  const Scene& lead = s->scenes[s->pending.front().scene];
+  int64_t pixels = 0;
+  for (auto it = s->pending.begin(); it != s->pending.end() && (int)batch.size() < s->max_views;) {
+    const bool fits = batch.empty() || pixels + it->pixels <= s->max_batch_pixels;
+    if (fits && same_layout(s->scenes[it->scene], lead)) {
+      pixels += it->pixels;
+      batch.push_back(*it);
+      it = s->pending.erase(it);
+    } else {
+      ++it;
+    }
   }
-  // ---- phase A: count
-  const size_t bytesA = phase_a_bytes(N, B);
+  std::stable_sort(batch.begin(), batch.end(), [](const Request& a, const Request& b) { return a.scene < b.scene; });
+  return batch;
+}
+
+// Phase-B bytes for the views of the batch prefix described by V, P,
+// n_blocks and B.
+static size_t phase_b_bytes(int64_t V, int64_t P, int64_t n_blocks, int B) {
+  return align_up((size_t)(2 * (B + 1)) * 8) + align_up((size_t)(2 * B) * sizeof(void*)) + align_up(8 * 4) +
+         align_up(V * 8) + align_up(V * 16) + align_up(V * 4) * 2 + align_up(V * 8) * 4 + align_up(V * 4) +
+         align_up(P * 4) + align_up(n_blocks * 4) * 2 + align_up(gsmi_binning_workspace_size(V, P, (int)n_blocks));
+}
+
+// Runs one batch to completion on worker w (returns when its images are
+// written or its requests failed / re-queued).
+static void run_batch(gsmi_server* s, Worker& w, std::vector<Request> reqs) {
+  hipStream_t st = w.stream;
+  const int B0 = (int)reqs.size();
+  // groups of consecutive views of one scene
+  std::vector<Group> groups;
+  for (int b = 0; b < B0; ++b)
+    if (groups.empty() || groups.back().scene != reqs[b].scene)
+      groups.push_back(Group{reqs[b].scene, b, 1, 0, 0, 0, 0, 0});
+    else
+      ++groups.back().nv;
+  // tile / block numbering of the batch; h_views[B0 + b] is view b's block grid
+  {
+    int64_t tile_base = 0, block_base = 0, pix_base = 0;
+    for (int b = 0; b < B0; ++b) {
+      gsmi_view& v = reqs[b].view;
+      v.tile_base = (int32_t)tile_base;
+      v.pix_base = pix_base;
+      tile_base += (int64_t)v.n_tiles_w * v.n_tiles_h;
+      pix_base += (int64_t)v.W * v.H;
+      w.h_views[b] = v;
+      gsmi_view bv = v;
+      bv.n_tiles_w = blocks_of(v.n_tiles_w);
+      bv.n_tiles_h = blocks_of(v.n_tiles_h);
+      bv.tile_base = (int32_t)block_base;
+      block_base += (int64_t)bv.n_tiles_w * bv.n_tiles_h;
+      w.h_views[B0 + b] = bv;
+    }
+  }
+  // ---- phase A: count every group
+  size_t bytesA = align_up((size_t)(2 * B0) * sizeof(gsmi_view));
+  for (auto& g : groups) {
+    const int64_t N = s->scenes[g.scene].N;
+    g.vis_off = bytesA;
+    bytesA += align_up((size_t)(g.nv * N));
+    g.voff_off = bytesA;
+    bytesA += align_up((size_t)(g.nv * N) * 4);
+    g.bases_off = bytesA;
+    bytesA += align_up((size_t)(2 * (g.nv + 1)) * 8);
+    g.ws_off = bytesA;
+    g.ws_bytes = gsmi_preprocess_views_workspace_size(N, g.nv);
+    bytesA += align_up(g.ws_bytes);
+  }
+  std::unique_lock<std::mutex> acquiring(s->acquire_mu);
   size_t offA;
   if (!arena_take(s, bytesA, &offA)) {
     fail_requests(s, reqs, GSMI_ERR_WORKSPACE);
-    return false;
+    return;
   }
-  char* p = s->arena.ptr(offA);
-  gsmi_view* views_d = reinterpret_cast<gsmi_view*>(p);
+  char* pa = s->arena.ptr(offA);
+  gsmi_view* views_d = reinterpret_cast<gsmi_view*>(pa);
   gsmi_view* bviews_d = views_d + B0;
-  p += align_up((size_t)(2 * B) * sizeof(gsmi_view));
-  uint8_t* vis = reinterpret_cast<uint8_t*>(p);
-  p += align_up((size_t)(B * N));
-  int32_t* vis_off = reinterpret_cast<int32_t*>(p);
-  p += align_up((size_t)(B * N) * 4);
-  int64_t* bases_d = reinterpret_cast<int64_t*>(p);
-  p += align_up((size_t)(2 * (B + 1)) * 8);
-  void* cws = p;
-  const size_t cws_bytes = gsmi_preprocess_views_workspace_size(N, B);
-  hipStream_t st = sl.stream;
-  int rc = hipMemcpyAsync(views_d, sl.h_views, (size_t)(2 * B) * sizeof(gsmi_view), hipMemcpyHostToDevice, st) ==
+  int rc = hipMemcpyAsync(views_d, w.h_views, (size_t)(2 * B0) * sizeof(gsmi_view), hipMemcpyHostToDevice, st) ==
                    hipSuccess
                ? GSMI_OK
                : GSMI_ERR_LAUNCH;
-  if (!rc)
-    rc = gsmi_preprocess_views_count_blocks(N, B, views_d, sc.mean, sc.qvec, sc.log_s, s->D, s->frustum_radius, 0,
-                                            kBlockShift, vis, vis_off, bases_d, cws, cws_bytes, st);
-  if (!rc && hipMemcpyAsync(sl.h_bases, bases_d, (size_t)(2 * (B + 1)) * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
-    rc = GSMI_ERR_LAUNCH;
-  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = GSMI_ERR_LAUNCH;
+  size_t lb = 0;  // offset of a group's local bases in h_lbases
+  for (auto& g : groups) {
+    const Scene& sc = s->scenes[g.scene];
+    if (!rc)
+      rc = gsmi_preprocess_views_count_blocks(sc.N, g.nv, views_d + g.v0, sc.mean, sc.qvec, sc.log_s, s->D,
+                                              s->frustum_radius, 0, kBlockShift,
+                                              reinterpret_cast<uint8_t*>(pa + g.vis_off),
+                                              reinterpret_cast<int32_t*>(pa + g.voff_off),
+                                              reinterpret_cast<int64_t*>(pa + g.bases_off), pa + g.ws_off,
+                                              g.ws_bytes, st);
+    if (!rc && hipMemcpyAsync(w.h_lbases + lb, pa + g.bases_off, (size_t)(2 * (g.nv + 1)) * 8,
+                              hipMemcpyDeviceToHost, st) != hipSuccess)
+      rc = GSMI_ERR_LAUNCH;
+    lb += 2 * (g.nv + 1);
+  }
+  if (!rc && hipEventRecord(w.counted, st) != hipSuccess) rc = GSMI_ERR_LAUNCH;
+  if (!rc && hipEventSynchronize(w.counted) != hipSuccess) rc = GSMI_ERR_LAUNCH;  // this worker only
   if (rc) {
+    (void)hipStreamSynchronize(st);
     arena_give(s, offA, bytesA);
     fail_requests(s, reqs, rc);
-    return false;
+    return;
+  }
+  // per-view (records, pairs) and the batch prefix sums
+  std::vector<int64_t> vV(B0), vP(B0);
+  lb = 0;
+  for (auto& g : groups) {
+    const int64_t* l = w.h_lbases + lb;
+    for (int k = 0; k < g.nv; ++k) {
+      vV[g.v0 + k] = l[2 * (k + 1)] - l[2 * k];
+      vP[g.v0 + k] = l[2 * (k + 1) + 1] - l[2 * k + 1];
+    }
+    lb += 2 * (g.nv + 1);
   }
   // ---- admission: longest prefix whose phase-B bytes fit
   size_t largest;
@@ -274,56 +340,62 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
     std::lock_guard<std::mutex> g(s->mu);
     largest = s->arena.largest();
   }
-  int k = B;
-  auto need = [&](int kk) {
-    const int64_t V = sl.h_bases[2 * kk], P = sl.h_bases[2 * kk + 1];
-    int64_t nt = 0, np = 0;
-    for (int b = 0; b < kk; ++b) {
-      nt += (int64_t)sl.h_views[B0 + b].n_tiles_w * sl.h_views[B0 + b].n_tiles_h;
-      np += (int64_t)reqs[b].view.W * reqs[b].view.H;
+  auto need = [&](int k, bool* fits_int) {
+    int64_t V = 0, P = 0, nb = 0;
+    for (int b = 0; b < k; ++b) {
+      V += vV[b];
+      P += vP[b];
+      nb += (int64_t)w.h_views[B0 + b].n_tiles_w * w.h_views[B0 + b].n_tiles_h;
     }
-    return std::make_pair(P < (int64_t(1) << 31), phase_b_bytes(V, P, nt, np));
+    *fits_int = P < (int64_t(1) << 31) && V < (int64_t(1) << 31);
+    return phase_b_bytes(V, P, nb, k);
   };
-  while (k > 1) {
-    auto nb = need(k);
-    if (nb.first && nb.second <= largest) break;
-    --k;
-  }
-  if (k < B) {
+  int B = B0;
+  bool fits_int = true;
+  size_t bytesB = need(B, &fits_int);
+  while (B > 1 && !(fits_int && bytesB <= largest)) bytesB = need(--B, &fits_int);
+  if (B < B0) {
     std::lock_guard<std::mutex> g(s->mu);
-    for (int b = B - 1; b >= k; --b) s->pending.push_front(reqs[b]);
-    reqs.resize(k);
+    for (int b = B0 - 1; b >= B; --b) s->pending.push_front(reqs[b]);
     ++s->stat_splits;
     s->cv.notify_all();
   }
-  B = k;
-  auto nb = need(B);
-  if (!nb.first) {
+  reqs.resize(B);
+  if (!fits_int) {
     arena_give(s, offA, bytesA);
     fail_requests(s, reqs, GSMI_ERR_UNSUPPORTED);
-    return false;
+    return;
   }
-  const int64_t V = sl.h_bases[2 * B], P = sl.h_bases[2 * B + 1];
-  int64_t n_tiles = 0, n_blocks = 0, n_pix = 0;
-  for (int b = 0; b < B; ++b) {
-    n_tiles += (int64_t)reqs[b].view.n_tiles_w * reqs[b].view.n_tiles_h;
-    n_blocks += (int64_t)sl.h_views[B0 + b].n_tiles_w * sl.h_views[B0 + b].n_tiles_h;
-    n_pix += (int64_t)reqs[b].view.W * reqs[b].view.H;
-  }
-  const size_t bytesB = nb.second;
   size_t offB;
   if (!arena_take(s, bytesB, &offB)) {
     arena_give(s, offA, bytesA);
     fail_requests(s, reqs, GSMI_ERR_WORKSPACE);
-    return false;
+    return;
   }
-  // ---- phase B: emit, bin, composite, copy out
-  p = s->arena.ptr(offB);
+  acquiring.unlock();
+  // ---- phase B: emit per group, bin all views, composite into the outputs
+  int64_t V = 0, P = 0, n_tiles = 0, n_blocks = 0;
+  for (int b = 0; b < B; ++b) {
+    w.h_bases[2 * b] = V;
+    w.h_bases[2 * b + 1] = P;
+    V += vV[b];
+    P += vP[b];
+    n_tiles += (int64_t)reqs[b].view.n_tiles_w * reqs[b].view.n_tiles_h;
+    n_blocks += (int64_t)w.h_views[B0 + b].n_tiles_w * w.h_views[B0 + b].n_tiles_h;
+    w.h_ptrs[b] = const_cast<float*>(s->scenes[reqs[b].scene].sh);
+    w.h_ptrs[B + b] = reqs[b].out;
+  }
+  w.h_bases[2 * B] = V;
+  w.h_bases[2 * B + 1] = P;
+  char* p = s->arena.ptr(offB);
   auto take = [&p](size_t n) {
     char* r = p;
     p += align_up(n);
     return r;
   };
+  int64_t* bases_d = reinterpret_cast<int64_t*>(take((size_t)(2 * (B + 1)) * 8));
+  void** ptrs_d = reinterpret_cast<void**>(take((size_t)(2 * B) * sizeof(void*)));
+  int* work_d = reinterpret_cast<int*>(take(8 * 4));
   float* mean2d = reinterpret_cast<float*>(take(V * 8));
   float* cov2d = reinterpret_cast<float*>(take(V * 16));
   float* depth = reinterpret_cast<float*>(take(V * 4));
@@ -338,108 +410,68 @@ static bool run_batch(gsmi_server* s, Slot& sl, std::vector<Request> reqs) {
   int32_t* end = reinterpret_cast<int32_t*>(take(n_blocks * 4));
   const size_t bws_bytes = gsmi_binning_workspace_size(V, P, (int)n_blocks);
   void* bws = take(bws_bytes);
-  float* out = reinterpret_cast<float*>(take(n_pix * 12));
-  rc = gsmi_preprocess_views_emit(N, B, views_d, sc.mean, sc.qvec, sc.log_s, sc.logit, s->D, vis, vis_off, mean2d,
-                                  cov2d, depth, alpha, tl, br, orig, st);
+  rc = hipMemcpyAsync(bases_d, w.h_bases, (size_t)(2 * (B + 1)) * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
+               hipMemcpyAsync(ptrs_d, w.h_ptrs, (size_t)(2 * B) * sizeof(void*), hipMemcpyHostToDevice, st) ==
+                   hipSuccess
+           ? GSMI_OK
+           : GSMI_ERR_LAUNCH;
+  for (auto& g : groups) {
+    if (rc || g.v0 >= B) break;
+    const Scene& sc = s->scenes[g.scene];
+    const int nv = std::min(g.nv, B - g.v0);
+    const int64_t v0 = w.h_bases[2 * g.v0];
+    rc = gsmi_preprocess_views_emit(sc.N, nv, views_d + g.v0, sc.mean, sc.qvec, sc.log_s, sc.logit, s->D,
+                                    reinterpret_cast<const uint8_t*>(pa + g.vis_off),
+                                    reinterpret_cast<const int32_t*>(pa + g.voff_off), mean2d + 2 * v0,
+                                    cov2d + 4 * v0, depth + v0, alpha + v0, tl + 2 * v0, br + 2 * v0, orig + v0, st);
+  }
   if (!rc) rc = gsmi_shift_rects(V, tl, br, kBlockShift, tl_b, br_b, st);
   if (!rc)
     rc = gsmi_bin_views(V, P, B, bviews_d, bases_d, (int)n_blocks, tl_b, br_b, depth, ids, start, end, bws,
                         bws_bytes, st);
-  if (!rc)
-    rc = gsmi_render_views_sh_two_level(V, B, views_d, bviews_d, (int)n_tiles, mean2d, cov2d, sc.sh,
-                                        3 * sc.sh_stride_c, sc.sh_stride_c, orig, alpha, tl, br, start, end, ids,
-                                        kBlockShift, out, sc.C, s->thresh, nullptr, nullptr, s->exact, st);
-  for (int b = 0; !rc && b < B; ++b) {
-    const gsmi_view& v = reqs[b].view;
-    if (hipMemcpyAsync(reqs[b].out, out + 3 * v.pix_base, (size_t)v.W * v.H * 12, hipMemcpyDeviceToDevice, st) !=
-        hipSuccess)
-      rc = GSMI_ERR_LAUNCH;
+  if (!rc) {
+    const Scene& sc = s->scenes[reqs[0].scene];
+    rc = gsmi_internal_render_served(V, B, views_d, bviews_d, (int)n_tiles, mean2d, cov2d,
+                                     reinterpret_cast<const float* const*>(ptrs_d), 3 * sc.sh_stride_c,
+                                     sc.sh_stride_c, orig, alpha, tl, br, start, end, ids, kBlockShift,
+                                     reinterpret_cast<float* const*>(ptrs_d + B), sc.C, s->thresh, s->exact, work_d,
+                                     s->queue_grid, st);
   }
-  if (!rc && hipEventRecord(sl.done, st) != hipSuccess) rc = GSMI_ERR_LAUNCH;
-  if (rc) {
-    (void)hipStreamSynchronize(st);
-    arena_give(s, offB, bytesB);
-    arena_give(s, offA, bytesA);
-    fail_requests(s, reqs, rc);
-    return false;
-  }
+  if (!rc && hipEventRecord(w.done, st) != hipSuccess) rc = GSMI_ERR_LAUNCH;
+  const int ok = !rc && hipEventSynchronize(w.done) == hipSuccess ? GSMI_OK : (rc ? rc : GSMI_ERR_LAUNCH);
+  if (ok != GSMI_OK) (void)hipStreamSynchronize(st);
+  const double t = s->now_ms();
   std::lock_guard<std::mutex> g(s->mu);
-  sl.reqs = std::move(reqs);
-  sl.offA = offA;
-  sl.bytesA = bytesA;
-  sl.offB = offB;
-  sl.bytesB = bytesB;
-  sl.has_work = true;
-  ++s->stat_batches;
-  s->stat_views += B;
-  s->stat_max_batch = std::max<int64_t>(s->stat_max_batch, B);
-  s->cv.notify_all();
-  return true;
-}
-
-static void waiter_loop(gsmi_server* s, int idx) {
-  (void)hipSetDevice(s->device);
-  Slot& sl = s->slots[idx];
-  for (;;) {
-    {
-      std::unique_lock<std::mutex> lk(s->mu);
-      s->cv.wait(lk, [&] { return sl.has_work || (s->stop && !sl.busy); });
-      if (!sl.has_work) return;
-    }
-    const int ok = hipEventSynchronize(sl.done) == hipSuccess ? GSMI_OK : GSMI_ERR_LAUNCH;
-    const double t = s->now_ms();
-    std::lock_guard<std::mutex> g(s->mu);
-    for (auto& r : sl.reqs) s->done.push_back(Done{r.id, r.t_submit, t, ok, (int)sl.reqs.size()});
-    s->arena.release(sl.offB, sl.bytesB);
-    s->arena.release(sl.offA, sl.bytesA);
-    sl.reqs.clear();
-    sl.has_work = false;
-    sl.busy = false;
-    --s->in_flight;
-    s->cv.notify_all();
+  for (auto& r : reqs) s->done.push_back(Done{r.id, r.t_submit, t, ok, B});
+  s->arena.release(offB, bytesB);
+  s->arena.release(offA, bytesA);
+  if (ok == GSMI_OK) {
+    s->stat_mixed += groups.size() > 1 && groups[1].v0 < B;
+    ++s->stat_batches;
+    s->stat_views += B;
+    s->stat_max_batch = std::max<int64_t>(s->stat_max_batch, B);
+  } else {
+    s->stat_failed += B;
   }
+  s->cv.notify_all();
 }
 
-static void dispatcher_loop(gsmi_server* s) {
+static void worker_loop(gsmi_server* s, int idx) {
   (void)hipSetDevice(s->device);
+  Worker& w = s->workers[idx];
   for (;;) {
-    int idx = -1;
     std::vector<Request> batch;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv.wait(lk, [&] {
-        if (s->stop) return true;
-        if (s->pending.empty()) return false;
-        for (auto& sl : s->slots)
-          if (!sl.busy) return true;
-        return false;
-      });
-      if (s->stop) return;
-      for (size_t i = 0; i < s->slots.size(); ++i)
-        if (!s->slots[i].busy) {
-          idx = (int)i;
-          break;
-        }
-      // oldest request's scene; then the other pending requests of that
-      // scene in arrival order
-      const int scene = s->pending.front().scene;
-      for (auto it = s->pending.begin(); it != s->pending.end() && (int)batch.size() < s->max_views;) {
-        if (it->scene == scene) {
-          batch.push_back(*it);
-          it = s->pending.erase(it);
-        } else {
-          ++it;
-        }
-      }
-      s->slots[idx].busy = true;
+      s->cv.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+      if (s->pending.empty()) return;  // stopping with nothing queued
+      batch = take_batch(s);
       ++s->in_flight;
     }
-    if (!run_batch(s, s->slots[idx], std::move(batch))) {
-      std::lock_guard<std::mutex> g(s->mu);
-      s->slots[idx].busy = false;
-      --s->in_flight;
-      s->cv.notify_all();
-    }
+    run_batch(s, w, std::move(batch));
+    std::lock_guard<std::mutex> g(s->mu);
+    --s->in_flight;
+    s->cv.notify_all();
   }
 }
 
@@ -540,20 +572,32 @@ extern "C" int gsmi_server_create(int device, int64_t arena_bytes, int max_views
     return GSMI_ERR_WORKSPACE;
   }
   s->arena.init(reinterpret_cast<char*>(s->arena_mem), (size_t)arena_bytes);
-  s->slots.resize(n_slots);
-  for (auto& sl : s->slots) {
-    bool ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
-              hipHostMalloc((void**)&sl.h_views, (size_t)(2 * max_views) * sizeof(gsmi_view), hipHostMallocDefault) ==
+  s->workers.resize(n_slots);
+  for (auto& w : s->workers) {
+    bool ok = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&w.counted, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&w.done, hipEventDisableTiming) == hipSuccess &&
+              hipHostMalloc((void**)&w.h_views, (size_t)(2 * max_views) * sizeof(gsmi_view), hipHostMallocDefault) ==
                   hipSuccess &&
-              hipHostMalloc((void**)&sl.h_bases, (size_t)(2 * (max_views + 1)) * 8, hipHostMallocDefault) ==
+              hipHostMalloc((void**)&w.h_lbases, (size_t)(4 * max_views + 2) * 8, hipHostMallocDefault) ==
+                  hipSuccess &&
+              hipHostMalloc((void**)&w.h_bases, (size_t)(2 * (max_views + 1)) * 8, hipHostMallocDefault) ==
+                  hipSuccess &&
+              hipHostMalloc((void**)&w.h_ptrs, (size_t)(2 * max_views) * sizeof(void*), hipHostMallocDefault) ==
                   hipSuccess;
     if (!ok) return GSMI_ERR_LAUNCH;  // resources of a failed create are not reclaimed
   }
   s->t0 = Clock::now();
-  for (int i = 0; i < n_slots; ++i) s->slots[i].waiter = std::thread(waiter_loop, s, i);
-  s->dispatcher = std::thread(dispatcher_loop, s);
+  for (int i = 0; i < n_slots; ++i) s->workers[i].th = std::thread(worker_loop, s, i);
   *out = s;
+  return GSMI_OK;
+}
+
+extern "C" int gsmi_server_configure(gsmi_server* s, int64_t max_batch_pixels, int queue_grid) {
+  if (!s || max_batch_pixels <= 0 || queue_grid < 0 || queue_grid > (1 << 20)) return GSMI_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  s->max_batch_pixels = max_batch_pixels;
+  s->queue_grid = queue_grid;
   return GSMI_OK;
 }
 
@@ -577,6 +621,7 @@ extern "C" int gsmi_server_submit(gsmi_server* s, int scene_id, const float* c2w
   int rc = gsmi_make_view(c2w, fx, fy, cx, cy, W, H, near_plane, far_plane, 16, &r.view);
   if (rc) return rc;
   r.out = out;
+  r.pixels = (int64_t)W * H;
   std::lock_guard<std::mutex> g(s->mu);
   if (scene_id < 0 || scene_id >= (int)s->scenes.size()) return GSMI_ERR_ARG;
   r.scene = scene_id;
@@ -633,6 +678,19 @@ extern "C" int gsmi_server_stats(gsmi_server* s, int64_t* stats) {
   return GSMI_OK;
 }
 
+extern "C" int gsmi_server_stats_ex(gsmi_server* s, int64_t* stats, int n) {
+  if (!s || !stats || n < 0 || n > 7) return GSMI_ERR_ARG;
+  int64_t v[7];
+  int rc = gsmi_server_stats(s, v);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    v[6] = s->stat_mixed;
+  }
+  for (int k = 0; k < n; ++k) stats[k] = v[k];
+  return GSMI_OK;
+}
+
 extern "C" int gsmi_server_destroy(gsmi_server* s) {
   if (!s) return GSMI_ERR_ARG;
   gsmi_server_drain(s, 600000.0);
@@ -641,15 +699,17 @@ extern "C" int gsmi_server_destroy(gsmi_server* s) {
     s->stop = true;
     s->cv.notify_all();
   }
-  if (s->dispatcher.joinable()) s->dispatcher.join();
-  for (auto& sl : s->slots)
-    if (sl.waiter.joinable()) sl.waiter.join();
-  for (auto& sl : s->slots) {
-    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-    if (sl.done) (void)hipEventDestroy(sl.done);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    if (sl.h_views) (void)hipHostFree(sl.h_views);
-    if (sl.h_bases) (void)hipHostFree(sl.h_bases);
+  for (auto& w : s->workers)
+    if (w.th.joinable()) w.th.join();
+  for (auto& w : s->workers) {
+    if (w.stream) (void)hipStreamSynchronize(w.stream);
+    if (w.counted) (void)hipEventDestroy(w.counted);
+    if (w.done) (void)hipEventDestroy(w.done);
+    if (w.stream) (void)hipStreamDestroy(w.stream);
+    if (w.h_views) (void)hipHostFree(w.h_views);
+    if (w.h_lbases) (void)hipHostFree(w.h_lbases);
+    if (w.h_bases) (void)hipHostFree(w.h_bases);
+    if (w.h_ptrs) (void)hipHostFree(w.h_ptrs);
   }
   if (s->arena_mem) (void)hipFree(s->arena_mem);
   delete s;

@@ -4,10 +4,11 @@ Python face of the native server in csrc/serve.hip (C ABI gsmi_server_*).
 The reference's viewer renders each client's camera with one
 ``SHRenderer.forward`` call under ``no_grad``, one client after another
 (utils/viewer/viser_viewer.py:118-158).  ``RenderServer`` keeps the scenes in
-HBM, queues requests from any thread, and a native dispatcher thread batches
-pending requests of a scene into one multi-view launch sequence per batch,
-several batches in flight on separate streams; per-batch memory comes from a
-fixed device arena, and a batch admits as many views as their key/sort
+HBM, queues requests from any thread, and native worker threads (one HIP
+stream each) batch pending requests, of several scenes when their SH layout
+matches, into one multi-view launch sequence per batch under a view and
+pixel budget, several batches in flight at once; per-batch memory comes from
+a fixed device arena, and a batch admits as many views as their key/sort
 buffers fit.  Every image equals ``rasterize.forward_view`` for that camera.
 """
 from __future__ import annotations
@@ -53,9 +54,13 @@ class RenderServer:
     """Continuous-batching renderer on one GPU.
 
     arena_bytes: device memory for all in-flight batches (allocated once);
-    max_views: views per batch; n_slots: batches in flight at once."""
+    max_views: views per batch; n_slots: worker threads, i.e. batches in
+    flight at once; max_batch_pixels: pixel budget of a batch (None:
+    unlimited); queue_grid: workgroups of the work-queue compositing launch
+    (0: one workgroup per tile)."""
 
-    def __init__(self, device="cuda:0", arena_bytes=16 << 30, max_views=32, n_slots=2, cfg=None, exact=False):
+    def __init__(self, device="cuda:0", arena_bytes=16 << 30, max_views=32, n_slots=2, cfg=None, exact=False,
+                 max_batch_pixels=None, queue_grid=0):
         from .scenes import RasterConfig
 
         self._L = _lib.load()
@@ -69,6 +74,8 @@ class RenderServer:
                                           float(cfg.tile_culling_radius), float(cfg.frustum_culling_radius),
                                           float(cfg.T_thresh), 1 if exact else 0, ctypes.byref(h)), "server_create")
         self._h = h
+        check(self._L.gsmi_server_configure(h, int(max_batch_pixels) if max_batch_pixels else 1 << 40,
+                                            int(queue_grid)), "server_configure")
         self._scenes = []  # keep the registered tensors alive
         self._outs = {}    # request id -> output tensor (alive until completed)
 
@@ -122,10 +129,10 @@ class RenderServer:
         check(self._L.gsmi_server_drain(self._h, float(timeout_ms)), "server_drain")
 
     def stats(self):
-        s = np.zeros(6, np.int64)
-        check(self._L.gsmi_server_stats(self._h, s.ctypes.data_as(ctypes.c_void_p)), "server_stats")
+        s = np.zeros(7, np.int64)
+        check(self._L.gsmi_server_stats_ex(self._h, s.ctypes.data_as(ctypes.c_void_p), 7), "server_stats")
         return dict(batches=int(s[0]), views=int(s[1]), max_batch=int(s[2]), splits=int(s[3]), failed=int(s[4]),
-                    arena_bytes=int(s[5]))
+                    arena_bytes=int(s[5]), mixed_batches=int(s[6]))
 
     def close(self):
         if self._h:

@@ -463,6 +463,44 @@ int gsmi_render_views_sh_two_level(int64_t V, int B, const gsmi_view *views, con
                                    const int32_t *bend, const int32_t *bids, int shift, float *out, int C,
                                    float thresh, float *T_final, int32_t *n_done, int exact, void *stream);
 
+/* Two-level lists of a batch kept per view (no counterpart in the
+ * reference; the batched training path): a 16-px tile's filtered list is
+ * recorded in its view's own array, so that no offset exceeds the int32
+ * range however many views the batch holds.
+ * gsmi_two_level_offsets_views: foff[t] for every tile t of every view (the
+ * views' tiles in order, view_tiles [B] host array of tile counts) = the
+ * exclusive prefix, within the tile's view, of the lengths of the block
+ * lists the tiles fall in (block_views / bstart / bend as in
+ * gsmi_render_views_sh_two_level); so view b's array needs at most
+ * 4^shift x (block pairs of view b) entries.  Workspace:
+ * gsmi_two_level_offsets_workspace_size(max view_tiles).
+ * gsmi_render_views_sh_two_level_record: gsmi_render_views_sh_two_level that
+ * also writes each tile's filtered list (up to where its pixels saturate) to
+ * view_fids[b] (device array of B pointers) from foff[t] on and its end to
+ * fend[t]; T_final / n_done as gsmi_render_views_sh.
+ * gsmi_render_views_backward_sh_lists: gsmi_render_views_backward_sh over
+ * those lists (start = foff, end = fend, entries at view_ids[b]), saved
+ * state required, no background. */
+int gsmi_two_level_offsets_views(int B, const gsmi_view *views, const gsmi_view *block_views,
+                                 const int32_t *view_tiles, int shift, const int32_t *bstart, const int32_t *bend,
+                                 int32_t *foff, void *workspace, size_t workspace_bytes, void *stream);
+int gsmi_render_views_sh_two_level_record(int64_t V, int B, const gsmi_view *views, const gsmi_view *block_views,
+                                          int n_tiles_total, const float *mean, const float *cov, const float *sh,
+                                          int64_t sh_stride_g, int64_t sh_stride_c, const int32_t *sh_index,
+                                          const float *alpha, const int32_t *tl, const int32_t *br,
+                                          const int32_t *bstart, const int32_t *bend, const int32_t *bids, int shift,
+                                          const int32_t *foff, int32_t *const *view_fids, int32_t *fend, float *out,
+                                          int C, float thresh, float *T_final, int32_t *n_done, int exact,
+                                          void *stream);
+int gsmi_render_views_backward_sh_lists(int64_t V, int64_t P, int B, const gsmi_view *views, int n_tiles_total,
+                                        const float *mean, const float *cov, const float *sh, int64_t sh_stride_g,
+                                        int64_t sh_stride_c, const int32_t *sh_index, const float *alpha,
+                                        const int32_t *start, const int32_t *end, const int32_t *const *view_ids,
+                                        const float *T_final, const int32_t *n_done, const float *grad_out,
+                                        float *grad_mean, float *grad_cov, float *grad_sh, float *grad_alpha, int C,
+                                        float thresh, int flags, void *workspace, size_t workspace_bytes,
+                                        void *stream);
+
 /* Reverse-order adjoint of gsmi_render_views_sh; gradients of records seen
  * in several views accumulate (SH rows through sh_index).  T_final, n_done,
  * pre_bg, flags and workspace as for gsmi_render_sh_backward (workspace of
@@ -567,13 +605,18 @@ int gsmi_prune_alpha_write(int64_t N, const float *mean, const float *qvec, cons
  * (utils/viewer/viser_viewer.py:118-158, vis.py:18-32: one
  * SHRenderer.forward under no_grad per client, one at a time).  Scenes stay
  * resident (caller-owned device tensors, registered once); camera requests
- * are queued and a dispatcher thread batches up to max_views pending
- * requests of one scene into one multi-view pass on one of n_slots streams,
- * while earlier batches are still running.  All per-batch buffers come from
- * one device arena of arena_bytes allocated by gsmi_server_create; a batch
- * keeps the longest prefix of its views whose key/sort buffers fit (the rest
- * wait in the queue).  Each result equals the single-view forward of that
- * camera bit for bit.  The only entry points that allocate device memory.
+ * are queued, and each of n_slots worker threads (one HIP stream each)
+ * takes the oldest pending request plus, in arrival order, pending requests
+ * of any scene with the same SH layout, up to max_views views and the pixel
+ * budget of gsmi_server_configure, and runs them as one multi-view pass
+ * (scenes counted and emitted group by group, binned and composited
+ * together; each image written straight into its request's buffer) while
+ * other workers' batches are still running.  All per-batch buffers come
+ * from one device arena of arena_bytes allocated by gsmi_server_create; a
+ * batch keeps the longest prefix of its views whose key/sort buffers fit
+ * (the rest wait in the queue).  Each result equals the single-view forward
+ * of that camera bit for bit.  The only entry points that allocate device
+ * memory.
  */
 typedef struct gsmi_server gsmi_server;
 
@@ -586,6 +629,13 @@ int gsmi_make_view(const float *c2w, double fx, double fy, double cx, double cy,
 
 int gsmi_server_create(int device, int64_t arena_bytes, int max_views, int n_slots, float D, float frustum_radius,
                        float thresh, int exact, gsmi_server **out);
+
+/* Batch shaping: at most max_batch_pixels pixels per batch (a single larger
+ * request still runs alone; default unlimited), and queue_grid > 0 composites
+ * each batch with that many workgroups taking tiles from per-XCD work queues
+ * (0, the default: one workgroup per tile).  Applies to batches formed
+ * afterwards. */
+int gsmi_server_configure(gsmi_server *s, int64_t max_batch_pixels, int queue_grid);
 
 /* Registers a resident scene (raw SHRenderer parameters on the device; the
  * caller keeps them alive and unchanged while requests use them). */
@@ -611,6 +661,10 @@ int gsmi_server_drain(gsmi_server *s, double timeout_ms);
 /* stats[0..5]: batches, views served, largest batch, admission splits,
  * failed requests, arena bytes. */
 int gsmi_server_stats(gsmi_server *s, int64_t *stats);
+
+/* The first n (<= 7) of: the six values above, then the batches that mixed
+ * scenes. */
+int gsmi_server_stats_ex(gsmi_server *s, int64_t *stats, int n);
 
 /* Drains, stops the threads, frees the arena. */
 int gsmi_server_destroy(gsmi_server *s);

@@ -1,8 +1,9 @@
 """Render server on gfx950: every served image equals the per-call forward of
 that camera bit for bit, whether the request was served alone or batched
-with others, split by admission, or submitted while other batches were on
-the GPU; a request whose buffers can never fit completes with
-GSMI_ERR_WORKSPACE."""
+with others (of the same or of other scenes), split by admission or by the
+pixel budget, composited by the work-queue launch, or submitted while other
+batches were on the GPU; a request whose buffers can never fit completes
+with GSMI_ERR_WORKSPACE."""
 import threading
 import time
 
@@ -62,6 +63,49 @@ def test_batched_requests_match_per_call():
         assert torch.equal(out, reference(p, C, c2w, cam)), rid
 
 
+@pytest.mark.parametrize("queue_grid", [0, 64])
+def test_mixed_scene_batches(queue_grid):
+    """Requests of two scenes with the same SH layout, interleaved and
+    submitted back to back, are served in batches that mix the scenes (each
+    view reads its own scene's SH rows and writes its own buffer); with
+    queue_grid the batch is composited by 64 workgroups taking tiles from
+    the per-XCD work queues."""
+    (pa, Ca), (pb, Cb) = scene(4, 6000, 11), scene(4, 3000, 12)
+    with serving.RenderServer(arena_bytes=512 << 20, max_views=8, n_slots=1, queue_grid=queue_grid) as srv:
+        sa = srv.add_scene(*pa, Ca)
+        sb = srv.add_scene(*pb, Cb)
+        reqs = []
+        for k in range(12):
+            sid, p, C = (sa, pa, Ca) if k % 2 else (sb, pb, Cb)
+            c2w, cam = camera(k)
+            rid, out = srv.submit(sid, c2w, cam)
+            reqs.append((rid, out, p, C, c2w, cam))
+        srv.drain()
+        done = {c.request_id: c for c in srv.poll()}
+        st = srv.stats()
+    assert st["failed"] == 0 and st["mixed_batches"] >= 1, st
+    for rid, out, p, C, c2w, cam in reqs:
+        assert done[rid].status == 0
+        assert torch.equal(out, reference(p, C, c2w, cam)), rid
+
+
+def test_pixel_budget_limits_batches():
+    """With a budget of two 160x128 views per batch no batch holds more
+    pixels than that (a single larger request would still run alone)."""
+    p, C = scene(4, 4000, 13)
+    cams = [camera(2) for _ in range(6)]
+    with serving.RenderServer(arena_bytes=512 << 20, max_views=8, n_slots=1, max_batch_pixels=2 * 160 * 128) as srv:
+        sid = srv.add_scene(*p, C)
+        reqs = [srv.submit(sid, c2w, cam) + (c2w, cam) for c2w, cam in cams]
+        srv.drain()
+        done = {c.request_id: c for c in srv.poll()}
+        st = srv.stats()
+    assert st["failed"] == 0 and st["max_batch"] <= 2 and st["views"] == 6
+    for rid, out, c2w, cam in reqs:
+        assert done[rid].status == 0 and done[rid].batch_views <= 2
+        assert torch.equal(out, reference(p, C, c2w, cam))
+
+
 def _phase_bytes(N, p, cams, L):
     """Mirror of the server's per-batch arena accounting (pairs counted per
     block of 4 x 4 tiles, as the server bins them)."""
@@ -78,8 +122,8 @@ def _phase_bytes(N, p, cams, L):
 
     def b(k):
         V, P, nb, npx = (sum(x[i] for x in per[:k]) for i in range(4))
-        return au(V * 8) + au(V * 16) + au(V * 4) * 2 + au(V * 8) * 4 + au(V * 4) + au(P * 4) + au(nb * 4) * 2 + \
-            au(L.gsmi_binning_workspace_size(V, P, nb)) + au(npx * 12)
+        return au(2 * (k + 1) * 8) + au(2 * k * 8) + au(32) + au(V * 8) + au(V * 16) + au(V * 4) * 2 + \
+            au(V * 8) * 4 + au(V * 4) + au(P * 4) + au(nb * 4) * 2 + au(L.gsmi_binning_workspace_size(V, P, nb))
 
     return a, b
 
@@ -221,7 +265,7 @@ def test_serving_workload_1m_4m_mixed_resolutions(orc):
     within 1e-4 on a sample of tiles (every 20th non-empty tile by list
     length plus the 16 longest lists)."""
     from gaussian_splatting_3d_amd import loadgen
-    from tilesample import oracle_records, sample_tiles, tile_lists, tile_pixels
+    from tilesample import oracle_records, sample_tiles, tile_counts, tile_lists, tile_pixels
 
     scs = [scenes.serving_scene(n, seed=4 + k) for k, n in enumerate((1_000_000, 4_000_000))]
     params = [[t.to(DEV).contiguous() for t in (s.mean, s.qvec, s.svec_before_activation, s.alpha_before_activation,
@@ -248,12 +292,7 @@ def test_serving_workload_1m_4m_mixed_resolutions(orc):
     ic = ref["ic"]
     H, W, nth, ntw = ic["H"], ic["W"], ic["n_tiles_h"], ic["n_tiles_w"]
     tl, br = ref["tl"], ref["br"]
-    cnt = np.zeros((nth + 1, ntw + 1), np.int64)  # pairs per tile from the rectangles (2-D difference array)
-    np.add.at(cnt, (tl[:, 1], tl[:, 0]), 1)
-    np.add.at(cnt, (tl[:, 1], br[:, 0] + 1), -1)
-    np.add.at(cnt, (br[:, 1] + 1, tl[:, 0]), -1)
-    np.add.at(cnt, (br[:, 1] + 1, br[:, 0] + 1), 1)
-    counts = cnt.cumsum(0).cumsum(1)[:nth, :ntw].reshape(-1)
+    counts = tile_counts(tl, br, nth, ntw)
     assert counts.sum() == ref["P"]
     tiles = sample_tiles(counts, counts, every=20, top=16)
     st_o, en_o, ids_o = tile_lists(tl, br, ref["depth"], tiles, ntw, nth * ntw)

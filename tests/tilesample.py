@@ -3,6 +3,7 @@
 sample_tiles: a stratified sample of non-empty tiles plus the heaviest ones.
 oracle_records: the oracle's preprocessing of one view (cull, projection,
   tile bounds) without binning.
+tile_counts: pairs per tile from the tile rectangles.
 tile_lists: the sorted list of each sampled tile by brute force: the
   Gaussians whose tile rectangle covers the tile, in (depth bits, index)
   order, i.e. the order of a stable sort of the reference's keys
@@ -42,6 +43,17 @@ def oracle_records(O, mean, qvec, log_s, logit, c2w, cam, D=6.0, frustum_radius=
     P, tl, br = O.aabb(m2d, cov, oc, D)
     return dict(mask=mask, idx=idx, m2d=m2d, cov=cov, depth=depth, a=a, P=int(P), tl=tl, br=br,
                 ic=O.image_constants(oc))
+
+
+def tile_counts(tl, br, n_tiles_h, n_tiles_w):
+    """Pairs per tile (row-major) from the tile rectangles [tl, br] (a 2-D
+    difference array)."""
+    cnt = np.zeros((n_tiles_h + 1, n_tiles_w + 1), np.int64)
+    np.add.at(cnt, (tl[:, 1], tl[:, 0]), 1)
+    np.add.at(cnt, (tl[:, 1], br[:, 0] + 1), -1)
+    np.add.at(cnt, (br[:, 1] + 1, tl[:, 0]), -1)
+    np.add.at(cnt, (br[:, 1] + 1, br[:, 0] + 1), 1)
+    return cnt.cumsum(0).cumsum(1)[:n_tiles_h, :n_tiles_w].reshape(-1)
 
 
 def tile_lists(tl, br, depth, tiles, n_tiles_w, n_tiles_total):
