@@ -173,7 +173,9 @@ def gpu_bench_views(scene, steps, warmup):
     L = _lib.load()
     t_fwd, t_bwd = Timed(), Timed()
     state = {"on": False}
-    orig = {k: getattr(L, k) for k in ("gsmi_render_views_sh", "gsmi_render_views_backward_sh")}
+    names = ("gsmi_render_views_sh", "gsmi_render_views_sh_two_level_record", "gsmi_render_views_backward_sh",
+             "gsmi_render_views_backward_sh_lists")
+    orig = {k: getattr(L, k) for k in names}
 
     def wrap(name, timer):
         def f(*a):
@@ -183,8 +185,8 @@ def gpu_bench_views(scene, steps, warmup):
                 return orig[name](*a)
         return f
 
-    L.gsmi_render_views_sh = wrap("gsmi_render_views_sh", t_fwd)
-    L.gsmi_render_views_backward_sh = wrap("gsmi_render_views_backward_sh", t_bwd)
+    for k in names:  # the per-tile or the two-level entries, whichever the step calls
+        setattr(L, k, wrap(k, t_fwd if "backward" not in k else t_bwd))
 
     def step():
         out, st = multiview.forward_views(*params, scene.cameras, scene.C, cfg)
@@ -212,7 +214,7 @@ def gpu_bench_views(scene, steps, warmup):
     P_exam = sum(examined_pairs(st.n_done[m["pix_base"]:m["pix_base"] + m["H"] * m["W"]], m["H"], m["W"])
                  for m in b.meta)
     return dict(ms_step=(t1 - t0) * 1e3 / steps, ms_composite_fwd=t_fwd.mean_ms(), ms_composite_bwd=t_bwd.mean_ms(),
-                N=b.N, V=b.V, P=b.P, P_exam=P_exam, n_pix=n_pix, views=b.B, C=scene.C,
+                N=b.N, V=b.V, P=b.P, shift=b.shift, P_exam=P_exam, n_pix=n_pix, views=b.B, C=scene.C,
                 H=scene.cameras[0][1].h, W=scene.cameras[0][1].w)
 
 
@@ -553,7 +555,7 @@ def main():
         rv = gpu_bench_views(mv, 3, 1)
         line["multiview_6m"] = {"workload": mv.name, "ms_per_step": round(rv["ms_step"], 3),
                                 "mpix_s": round(rv["n_pix"] / (rv["ms_step"] / 1e3) / 1e6, 3), "views": rv["views"],
-                                "pairs": rv["P"], "kernels_ms": {"composite_fwd": round(rv["ms_composite_fwd"], 3),
+                                ("block_pairs" if rv["shift"] else "pairs"): rv["P"], "kernels_ms": {"composite_fwd": round(rv["ms_composite_fwd"], 3),
                                                                  "composite_bwd": round(rv["ms_composite_bwd"], 3)},
                                 "steps": 3, "warmup": 1}
         print(f"[bench] multiview: {line['multiview_6m']}", file=sys.stderr, flush=True)

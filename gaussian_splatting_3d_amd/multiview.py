@@ -16,6 +16,12 @@ every launch:
 backward:
   6. gsmi_render_views_backward_sh — one adjoint launch; SH gradients of a
      Gaussian seen in several views accumulate in the raw-layout buffer;
+With two-level lists (the default without background) steps 1 and 4 count
+and bin per 64-px block, and step 5 is gsmi_render_views_sh_two_level_record:
+every 16-px tile filters its block's list while compositing and records what
+it examined in its view's own array (view-relative int32 offsets from
+gsmi_two_level_offsets_views), which step 6 reads
+(gsmi_render_views_backward_sh_lists).
   7. gsmi_project_backward_views — per Gaussian, the views' gradients are
      summed in view order and chained to the raw parameters once.
 
@@ -25,6 +31,7 @@ per-view gradients (up to float summation order).
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
@@ -97,10 +104,32 @@ class ViewsBinning:
     ids: torch.Tensor
     start: torch.Tensor
     end: torch.Tensor
+    # two-level lists: pairs (P, bases) are per 64-px block; block_views_d,
+    # n_blocks describe the block grids; after the forward ids / start / end
+    # are the recorded per-tile lists, view b's at view_ids_d[b]
+    shift: int = 0
+    block_views_d: torch.Tensor | None = None
+    n_blocks: int = 0
+    view_ids_d: torch.Tensor | None = None
+    view_fids: list | None = None
 
 
-def preprocess_and_bin_views(mean, qvec, log_s, logit, cams, cfg) -> ViewsBinning:
-    """Steps 1-4 of the module docstring."""
+def block_view_records(rec, shift):
+    """gsmi_view records of the views' block grids (2^shift tiles a side),
+    blocks numbered across the batch."""
+    brec = rec.copy()
+    ints = brec.view(np.int32)
+    base = 0
+    for b in range(rec.shape[0]):
+        ntw, nth = int(ints[b, 58]), int(ints[b, 59])
+        bw, bh = (ntw + (1 << shift) - 1) >> shift, (nth + (1 << shift) - 1) >> shift
+        ints[b, 58], ints[b, 59], ints[b, 60] = bw, bh, base
+        base += bw * bh
+    return brec, base
+
+
+def preprocess_and_bin_views(mean, qvec, log_s, logit, cams, cfg, block_shift=0) -> ViewsBinning:
+    """Steps 1-4 of the module docstring (per 64-px block with block_shift)."""
     L = _lib.load()
     dev = mean.device
     st = stream_of(dev)
@@ -113,9 +142,10 @@ def preprocess_and_bin_views(mean, qvec, log_s, logit, cams, cfg) -> ViewsBinnin
     bases_d = torch.empty(2 * (B + 1), dtype=torch.int64, device=dev)
     ws = torch.empty(max(int(L.gsmi_preprocess_views_workspace_size(N, B)), 1), dtype=torch.uint8, device=dev)
     D = float(cfg.tile_culling_radius)
-    check(L.gsmi_preprocess_views_count(N, B, ptr(views_d), ptr(mean), ptr(qvec), ptr(log_s), D,
-                                        float(cfg.frustum_culling_radius), 1 if cfg.skip_frustum_culling else 0,
-                                        ptr(vis), ptr(vis_off), ptr(bases_d), ptr(ws), ws.numel(), st),
+    check(L.gsmi_preprocess_views_count_blocks(N, B, ptr(views_d), ptr(mean), ptr(qvec), ptr(log_s), D,
+                                               float(cfg.frustum_culling_radius),
+                                               1 if cfg.skip_frustum_culling else 0, int(block_shift), ptr(vis),
+                                               ptr(vis_off), ptr(bases_d), ptr(ws), ws.numel(), st),
           "preprocess_views_count")
     bases = bases_d.cpu().numpy().reshape(B + 1, 2)
     V, P = int(bases[B, 0]), int(bases[B, 1])
@@ -133,13 +163,21 @@ def preprocess_and_bin_views(mean, qvec, log_s, logit, cams, cfg) -> ViewsBinnin
                                        ptr(vis), ptr(vis_off), ptr(mean2d), ptr(cov2d), ptr(depth), ptr(alpha),
                                        ptr(tl), ptr(br), ptr(orig), st), "preprocess_views_emit")
     ids = torch.empty(P, dtype=torch.int32, device=dev)
-    start = torch.empty(n_tiles, dtype=torch.int32, device=dev)
-    end = torch.empty(n_tiles, dtype=torch.int32, device=dev)
-    bws = torch.empty(max(int(L.gsmi_binning_workspace_size(V, P, n_tiles)), 1), dtype=torch.uint8, device=dev)
-    check(L.gsmi_bin_views(V, P, B, ptr(views_d), ptr(bases_d), n_tiles, ptr(tl), ptr(br), ptr(depth), ptr(ids),
+    bin_views_d, n_bin, btl, bbr = views_d, n_tiles, tl, br
+    bviews_d = None
+    if block_shift:
+        brec, n_bin = block_view_records(rec, block_shift)
+        bviews_d = bin_views_d = torch.from_numpy(brec).to(dev)
+        btl = torch.empty_like(tl)
+        bbr = torch.empty_like(br)
+        check(L.gsmi_shift_rects(V, ptr(tl), ptr(br), int(block_shift), ptr(btl), ptr(bbr), st), "shift_rects")
+    start = torch.empty(n_bin, dtype=torch.int32, device=dev)
+    end = torch.empty(n_bin, dtype=torch.int32, device=dev)
+    bws = torch.empty(max(int(L.gsmi_binning_workspace_size(V, P, n_bin)), 1), dtype=torch.uint8, device=dev)
+    check(L.gsmi_bin_views(V, P, B, ptr(bin_views_d), ptr(bases_d), n_bin, ptr(btl), ptr(bbr), ptr(depth), ptr(ids),
                            ptr(start), ptr(end), ptr(bws), bws.numel(), st), "bin_views")
     return ViewsBinning(N, B, V, P, bases, views_d, meta, n_tiles, n_pix, vis, vis_off, mean2d, cov2d, depth, alpha,
-                        tl, br, orig, ids, start, end)
+                        tl, br, orig, ids, start, end, int(block_shift), bviews_d, n_bin if block_shift else 0)
 
 
 @dataclass
@@ -160,10 +198,46 @@ class ViewsState:
         return src.view(-1, 3)[m["pix_base"]:m["pix_base"] + m["H"] * m["W"]].view(m["H"], m["W"], 3)
 
 
+BLOCK_SHIFT = 2  # two-level lists: blocks of 4 x 4 tiles
+
+
+def _two_level_views(b: ViewsBinning, sh_coeffs, C, cfg, out, T, nd, exact):
+    """Two-level forward of a batch: per-tile offsets within each view, one
+    list array per view (sized for the view's block pairs), the recording
+    forward; afterwards b.ids / b.start / b.end / b.view_ids_d are the
+    recorded per-tile lists."""
+    L = _lib.load()
+    dev = out.device
+    st = stream_of(dev)
+    view_tiles = np.array([m["n_tiles"] for m in b.meta], np.int32)
+    foff = torch.empty(b.n_tiles, dtype=torch.int32, device=dev)
+    fend = torch.empty(b.n_tiles, dtype=torch.int32, device=dev)
+    ows = torch.empty(max(int(L.gsmi_two_level_offsets_workspace_size(int(view_tiles.max()))), 1), dtype=torch.uint8,
+                      device=dev)
+    check(L.gsmi_two_level_offsets_views(b.B, ptr(b.views_d), ptr(b.block_views_d),
+                                         view_tiles.ctypes.data_as(ctypes.c_void_p), b.shift, ptr(b.start),
+                                         ptr(b.end), ptr(foff), ptr(ows), ows.numel(), st), "two_level_offsets_views")
+    per_view = np.diff(b.bases[:, 1])  # block pairs of each view
+    caps = [int((1 << (2 * b.shift)) * p) for p in per_view]  # >= the view's sum of tile caps
+    if max(caps) >= 2 ** 31:
+        raise RuntimeError(f"a view needs {max(caps)} recorded list entries, beyond the int32 range")
+    fids = [torch.empty(max(c, 1), dtype=torch.int32, device=dev) for c in caps]
+    view_ids_d = torch.tensor([f.data_ptr() for f in fids], dtype=torch.int64, device=dev)
+    cmax2 = sh_coeffs.shape[-1]
+    check(L.gsmi_render_views_sh_two_level_record(
+        b.V, b.B, ptr(b.views_d), ptr(b.block_views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d), ptr(sh_coeffs),
+        3 * cmax2, cmax2, ptr(b.orig), ptr(b.alpha), ptr(b.tl), ptr(b.br), ptr(b.start), ptr(b.end), ptr(b.ids),
+        b.shift, ptr(foff), ptr(view_ids_d), ptr(fend), ptr(out), int(C), float(cfg.T_thresh), ptr(T), ptr(nd),
+        1 if exact else 0, st), "render_views_sh_two_level_record")
+    b.start, b.end, b.view_fids, b.view_ids_d = foff, fend, fids, view_ids_d
+    b.ids = None
+
+
 def forward_views(mean, qvec, log_s, logit, sh_coeffs, cams, C, cfg, exact=False):
     """Steps 1-5: returns (flat image [sum(H*W)*3], ViewsState)."""
     L = _lib.load()
-    b = preprocess_and_bin_views(mean, qvec, log_s, logit, cams, cfg)
+    two = bool(getattr(cfg, "two_level", False)) and not cfg.bg and cfg.tile_size == TILE
+    b = preprocess_and_bin_views(mean, qvec, log_s, logit, cams, cfg, block_shift=BLOCK_SHIFT if two else 0)
     dev = mean.device
     bg = torch.tensor(cfg.bg_rgb, dtype=torch.float32, device=dev) if cfg.bg else None
     out = torch.empty(b.n_pix * 3, dtype=torch.float32, device=dev)
@@ -173,6 +247,14 @@ def forward_views(mean, qvec, log_s, logit, sh_coeffs, cams, C, cfg, exact=False
     cmax2 = sh_coeffs.shape[-1]
     if not 1 <= C <= 4 or C * C > cmax2:
         raise RuntimeError(f"SH order C={C} does not fit sh_coeffs with {cmax2} coefficients per channel")
+    if two:
+        if b.V > 0:
+            _two_level_views(b, sh_coeffs, C, cfg, out, T, nd, exact)
+        else:  # nothing visible: every pixel of every view is empty
+            out.zero_()
+            T.fill_(1.0)
+            nd.zero_()
+        return out, ViewsState(b, int(C), cfg, bg, out, T, nd, pre)
     check(L.gsmi_render_views_sh(b.V, b.P, b.B, ptr(b.views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d),
                                  ptr(sh_coeffs), 3 * cmax2, cmax2, ptr(b.orig), ptr(b.alpha), ptr(b.start),
                                  ptr(b.end), ptr(b.ids), ptr(out), int(C), float(cfg.T_thresh), ptr(bg), ptr(T),
@@ -203,7 +285,14 @@ def backward_views(s: ViewsState, mean, qvec, log_s, logit, sh_coeffs, grad_out,
         return (*g, bd) if boundary else g
     nws = int(L.gsmi_backward_order_workspace_size(b.n_tiles))  # launch order, most expensive tiles first
     ows = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
-    check(L.gsmi_render_views_backward_sh(b.V, b.P, b.B, ptr(b.views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d),
+    if b.view_ids_d is not None:  # two-level: the recorded per-view lists
+        check(L.gsmi_render_views_backward_sh_lists(
+            b.V, b.P, b.B, ptr(b.views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d), ptr(sh_coeffs), 3 * cmax2, cmax2,
+            ptr(b.orig), ptr(b.alpha), ptr(b.start), ptr(b.end), ptr(b.view_ids_d), ptr(s.T), ptr(s.n_done), ptr(go),
+            ptr(g_m2d), ptr(g_cov), ptr(g_sh), ptr(g_al), s.C, float(s.cfg.T_thresh), 0, ptr(ows), nws, st),
+            "render_views_backward_sh_lists")
+    else:
+        check(L.gsmi_render_views_backward_sh(b.V, b.P, b.B, ptr(b.views_d), b.n_tiles, ptr(b.mean2d), ptr(b.cov2d),
                                           ptr(sh_coeffs), 3 * cmax2, cmax2, ptr(b.orig), ptr(b.alpha), ptr(b.start),
                                           ptr(b.end), ptr(b.ids), ptr(s.out), ptr(s.T), ptr(s.n_done),
                                           ptr(s.pre_bg), ptr(go), ptr(g_m2d), ptr(g_cov), ptr(g_sh), ptr(g_al), s.C,

@@ -11,7 +11,7 @@ import torch
 
 from gaussian_splatting_3d_amd import multiview, rasterize, scenes
 from gradmetrics import elementwise_excess, exact_excess, rel_err
-from tilesample import oracle_records, sample_tiles, tile_lists, tile_max, tile_pixels
+from tilesample import oracle_records, sample_tiles, tile_counts, tile_lists, tile_max, tile_pixels
 
 pytestmark = pytest.mark.gpu
 
@@ -44,31 +44,38 @@ def cameras():
     return cams
 
 
-@pytest.mark.parametrize("C", [1, 4])
-def test_views_match_single_view(C):
+@pytest.mark.parametrize("C,two_level", [(1, False), (4, False), (1, True), (4, True)])
+def test_views_match_single_view(C, two_level):
+    """Per view: record and pair counts (per 64-px block with two-level
+    lists), every tile's list (two-level: the list the forward recorded,
+    which ends where the tile saturated) with the same Gaussians in the same
+    order, image and final transmittance, all equal to the single-view
+    pipeline with the same list kind."""
     p = scene(C)
     cams = cameras()
-    cfg = scenes.RasterConfig()
+    cfg = scenes.RasterConfig(two_level=two_level)
     flat, st = multiview.forward_views(*p, cams, C, cfg)
     b = st.b
-    assert b.B == len(cams)
-    cfg1 = scenes.RasterConfig(two_level=False)  # the single view's own tile lists
+    assert b.B == len(cams) and b.shift == (2 if two_level else 0)
+    starts, ends = b.start.cpu().numpy(), b.end.cpu().numpy()
+    orig_b = b.orig.cpu().numpy()
     for v, (c2w, cam) in enumerate(cams):
-        out1, s1 = rasterize.forward_view(*p, c2w, cam, C, cfg1)
+        out1, s1 = rasterize.forward_view(*p, c2w, cam, C, cfg)
+        assert s1.b.shift == b.shift
         m = b.meta[v]
-        # records and tile lists: same Gaussians in the same order per tile
         assert int(b.bases[v + 1, 0] - b.bases[v, 0]) == s1.b.V
         assert int(b.bases[v + 1, 1] - b.bases[v, 1]) == s1.b.P
-        sb = b.start[m["tile_base"]:m["tile_base"] + m["n_tiles"]].cpu().numpy()
-        eb = b.end[m["tile_base"]:m["tile_base"] + m["n_tiles"]].cpu().numpy()
+        sb = starts[m["tile_base"]:m["tile_base"] + m["n_tiles"]]
+        eb = ends[m["tile_base"]:m["tile_base"] + m["n_tiles"]]
         s_single, e_single = s1.b.start.cpu().numpy(), s1.b.end.cpu().numpy()
-        assert np.array_equal(sb >= 0, s_single >= 0)
-        nz = s_single >= 0
-        assert np.array_equal((eb - sb)[nz], (e_single - s_single)[nz])
-        orig_b = b.orig.cpu().numpy()[b.ids.cpu().numpy()]
-        orig_1 = s1.b.orig.cpu().numpy()[s1.b.ids.cpu().numpy()]
-        for t in np.nonzero(nz)[0]:
-            assert np.array_equal(orig_b[sb[t]:eb[t]], orig_1[s_single[t]:e_single[t]])
+        ids_b = (b.view_fids[v] if two_level else b.ids).cpu().numpy()
+        ids_1 = s1.b.ids.cpu().numpy()
+        orig_1 = s1.b.orig.cpu().numpy()
+        len_b = np.where(sb >= 0, eb - sb, 0)
+        len_1 = np.where(s_single >= 0, e_single - s_single, 0)
+        assert np.array_equal(len_b, len_1)
+        for t in np.nonzero(len_1)[0]:
+            assert np.array_equal(orig_b[ids_b[sb[t]:eb[t]]], orig_1[ids_1[s_single[t]:e_single[t]]]), (v, t)
         # image bit-identical
         img = st.image(v).cpu().numpy()
         assert np.array_equal(img, out1.view(cam.h, cam.w, 3).cpu().numpy()), f"view {v}"
@@ -119,19 +126,20 @@ def test_views_autograd_matches_forward():
 
 
 def test_full_size_views_match_single_view():
-    """The 8-view 3840x2160 workload of 6M Gaussians (964M pairs, 18 bits of
-    tile id): the first and last view of the batch equal single-view renders
-    bit for bit (image, final transmittance, entries examined, per-view pair
-    counts)."""
+    """The 8-view 3840x2160 workload of 6M Gaussians (964M tile pairs, binned
+    as two-level lists per 64-px block): the first and last view of the batch
+    equal single-view renders bit for bit (image, final transmittance,
+    entries examined, per-view block-pair counts)."""
     sc = scenes.multiview_6m()
     p = [t.to(DEV).contiguous() for t in (sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation,
                                           sc.sh_coeffs)]
-    cfg = scenes.RasterConfig()
+    cfg = scenes.RasterConfig()  # two-level lists, per view
     flat, st = multiview.forward_views(*p, sc.cameras, sc.C, cfg)
     b = st.b
+    assert b.shift == 2
     for v in (0, len(sc.cameras) - 1):
         c2w, cam = sc.cameras[v]
-        out1, s1 = rasterize.forward_view(*p, c2w, cam, sc.C, scenes.RasterConfig(two_level=False))
+        out1, s1 = rasterize.forward_view(*p, c2w, cam, sc.C, cfg)
         m = b.meta[v]
         assert int(b.bases[v + 1, 1] - b.bases[v, 1]) == s1.b.P
         assert torch.equal(st.image(v), out1.view(cam.h, cam.w, 3)), f"view {v}"
@@ -168,8 +176,11 @@ def test_full_size_4k_batched_adjoint(orc):
     ref = oracle_records(orc, sc.mean, sc.qvec, sc.svec_before_activation, sc.alpha_before_activation, c2w, cam)
     ic = ref["ic"]
     H, W, nth, ntw = ic["H"], ic["W"], ic["n_tiles_h"], ic["n_tiles_w"]
+    assert b.shift == 2  # two-level lists: pairs counted per 64-px block
     V0, P0 = int(b.bases[1, 0]), int(b.bases[1, 1])
-    assert (V0, P0) == (ref["idx"].size, ref["P"])
+    tl, br = ref["tl"], ref["br"]
+    block_pairs = int((((br[:, 0] >> 2) - (tl[:, 0] >> 2) + 1) * ((br[:, 1] >> 2) - (tl[:, 1] >> 2) + 1)).sum())
+    assert (V0, P0) == (ref["idx"].size, block_pairs)
     assert np.array_equal(b.orig[:V0].cpu().numpy(), ref["idx"])
     for name, got, want in (("mean2d", b.mean2d[:V0], ref["m2d"]), ("cov2d", b.cov2d[:V0], ref["cov"]),
                             ("depth", b.depth[:V0], ref["depth"]), ("tl", b.tl[:V0], ref["tl"]),
@@ -178,15 +189,20 @@ def test_full_size_4k_batched_adjoint(orc):
         w = np.asarray(want).astype(g.dtype).reshape(-1)
         assert np.array_equal(g.view(np.uint32), w.view(np.uint32)), name
     n0 = nth * ntw
-    start0, end0 = b.start[:n0].cpu().numpy(), b.end[:n0].cpu().numpy()
-    counts = np.where(start0 >= 0, end0 - start0, 0).astype(np.int64)
-    assert counts.sum() == P0
+    counts = tile_counts(tl, br, nth, ntw)  # 16-px tile lists, from the rectangles
+    assert counts.sum() == ref["P"]
     nd0 = st.n_done[:H * W].cpu().numpy()
-    tiles = sample_tiles(counts, tile_max(nd0, H, W), every=100, top=16)
-    st_o, en_o, ids_o = tile_lists(ref["tl"], ref["br"], ref["depth"], tiles, ntw, n0)
-    ids_g = b.ids.cpu().numpy()
+    nd_tile = tile_max(nd0, H, W)
+    tiles = sample_tiles(counts, nd_tile, every=100, top=16)
+    st_o, en_o, ids_o = tile_lists(tl, br, ref["depth"], tiles, ntw, n0)
+    # the lists the forward recorded: the head of each tile's own list, up to
+    # (at least) the last entry any of its pixels examined
+    start0, end0 = b.start[:n0].cpu().numpy(), b.end[:n0].cpu().numpy()
+    ids_g = b.view_fids[0].cpu().numpy()
     for t in tiles:
-        assert np.array_equal(ids_g[start0[t]:end0[t]], ids_o[st_o[t]:en_o[t]]), f"tile {t}"
+        rec = ids_g[start0[t]:end0[t]]
+        assert nd_tile[t] <= rec.size <= en_o[t] - st_o[t], f"tile {t}"
+        assert np.array_equal(rec, ids_o[st_o[t]:st_o[t] + rec.size]), f"tile {t}"
     sh_v = np.ascontiguousarray(sc.sh_coeffs.numpy()[ref["idx"]][..., : C * C])
     al = ref["a"][ref["idx"]]
     o_out, o_T, o_nd, _ = orc.render_sh_fwd(ref["m2d"], ref["cov"], sh_v, al, st_o, en_o, ids_o, ic["topleft"], c2w,
