@@ -83,7 +83,7 @@ def examined_pairs(n_done, H, W, tile=16):
     return int(nd.view(nth, tile, ntw, tile).amax(dim=(1, 3)).sum())
 
 
-def gpu_bench(scene, steps, warmup, exact=False, split=False):
+def gpu_bench(scene, steps, warmup, exact=False, split=False, cmf=False):
     from gaussian_splatting_3d_amd import _lib, rasterize, scenes, training
     from gaussian_splatting_3d_amd.backend import _backend
 
@@ -121,7 +121,7 @@ def gpu_bench(scene, steps, warmup, exact=False, split=False):
         out, st = rasterize.forward_view(*params, c2w, cam, scene.C, cfg, exact=exact, prepare_backward=True)
         # mean |out - target| and its gradient sign(out - target) / numel, one fused pass
         _, grad_out = training.photometric_loss(out, target.view_as(out), "l1")
-        grads = rasterize.backward_view(st, *params, grad_out, split_bf16=split)
+        grads = rasterize.backward_view(st, *params, grad_out, split_bf16=split, colour_mfma=cmf)
         return st, grads
 
     try:
@@ -370,15 +370,15 @@ def _kernel_values(run, kernel):
 VALU_PEAK_INSTR_S = 256 * 4 * 2.4e9 / 4
 
 
-def _child_args(workload, exact, split):
-    return ["--pmc-child", "--workload", workload] + (["--exact"] if exact else []) + (["--bwd-split"] if split else [])
+def _child_args(workload, exact, bwd_opts=()):
+    return ["--pmc-child", "--workload", workload] + (["--exact"] if exact else []) + list(bwd_opts)
 
 
-def measure_valu(workload, exact, kernel, kernel_ms, split=False):
+def measure_valu(workload, exact, kernel, kernel_ms, bwd_opts=()):
     """Wave-level VALU instructions per launch of `kernel` (rocprofv3
     SQ_INSTS_VALU, summed over the chip) and their issue rate over the
     kernel's HIP-event time, against the non-packed issue peak."""
-    vals = _kernel_values(_pmc_pass("SQ_INSTS_VALU", _child_args(workload, exact, split)), kernel)
+    vals = _kernel_values(_pmc_pass("SQ_INSTS_VALU", _child_args(workload, exact, bwd_opts)), kernel)
     n = vals["SQ_INSTS_VALU"]
     rate = n / (kernel_ms / 1e3)
     return dict(instructions_per_launch=int(n), achieved_g_instr_s=round(rate / 1e9, 2),
@@ -388,7 +388,7 @@ def measure_valu(workload, exact, kernel, kernel_ms, split=False):
 FP32_PEAK_FLOP_S = 157.3e12  # MI355X FP32 vector peak (published)
 
 
-def measure_flops(workload, exact, kernel, kernel_ms, split=False):
+def measure_flops(workload, exact, kernel, kernel_ms, bwd_opts=()):
     """FP32 floating-point operations per launch of `kernel` from one
     counter pass, as rocprofv3's TOTAL_32_OPS metric counts them without the
     integer term: 64 lanes x (2 SQ_INSTS_VALU_FMA_F32 + SQ_INSTS_VALU_ADD_F32 +
@@ -397,7 +397,7 @@ def measure_flops(workload, exact, kernel, kernel_ms, split=False):
     kernel's HIP-event time, against the 157.3 TFLOP/s FP32 peak."""
     names = ["SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_TRANS_F32",
              "SQ_INSTS_VALU_MFMA_MOPS_F32"]
-    vals = _kernel_values(_pmc_pass(names, _child_args(workload, exact, split)), kernel)
+    vals = _kernel_values(_pmc_pass(names, _child_args(workload, exact, bwd_opts)), kernel)
     fma, add, mul, trans, mops = (vals.get(k, 0.0) for k in names)
     vec = 64.0 * (2.0 * fma + add + mul + trans)
     mfma = 512.0 * mops
@@ -408,13 +408,13 @@ def measure_flops(workload, exact, kernel, kernel_ms, split=False):
                 frac=round(rate / FP32_PEAK_FLOP_S, 4))
 
 
-def measure_traffic(workload, exact, kernel, split=False):
+def measure_traffic(workload, exact, kernel, bwd_opts=()):
     """HBM bytes per launch of `kernel` from rocprofv3 FETCH_SIZE and
     WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
     for a 1 GiB device-to-device copy (bytes moved / counter KiB)."""
     copy_bytes = float(1 << 30)
     res = {}
-    child = _child_args(workload, exact, split)
+    child = _child_args(workload, exact, bwd_opts)
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         cal = _pmc_pass(counter, ["--copy-calibration"])
         copy_kib = [v[counter] for k, v in cal.items() if "copyBuffer" in k and counter in v]
@@ -439,6 +439,8 @@ def main():
     ap.add_argument("--bwd-split", action="store_true",
                     help="opt-in: SH-gradient products on bf16 matrix cores as a three-term split (not fp32; the "
                          "line's dtype says so)")
+    ap.add_argument("--bwd-colour-mfma", action="store_true",
+                    help="opt-in: the backward's colours on fp32 matrix cores (five entries at a time)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (scales rocprof byte counters)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -469,7 +471,7 @@ def main():
         if len(sc.cameras) > 1:
             gpu_bench_views(sc, 1, 1)
         else:
-            gpu_bench(sc, 1, 1, exact=args.exact, split=args.bwd_split)
+            gpu_bench(sc, 1, 1, exact=args.exact, split=args.bwd_split, cmf=args.bwd_colour_mfma)
         return
     if args.workload == "serve":
         serve_main(args)
@@ -478,12 +480,13 @@ def main():
         train_main(args)
         return
     scene = make_scene(args.workload)
+    bwd_opts = (["--bwd-split"] if args.bwd_split else []) + (["--bwd-colour-mfma"] if args.bwd_colour_mfma else [])
     print(f"[bench] workload {scene.name}: N={scene.N}", file=sys.stderr, flush=True)
     views = len(scene.cameras)
     if views > 1:
         r = gpu_bench_views(scene, args.steps, args.warmup)
     else:
-        r = gpu_bench(scene, args.steps, args.warmup, exact=args.exact, split=args.bwd_split)
+        r = gpu_bench(scene, args.steps, args.warmup, exact=args.exact, split=args.bwd_split, cmf=args.bwd_colour_mfma)
         r["n_pix"], r["views"] = r["H"] * r["W"], 1
     print(f"[bench] gpu done: {r}", file=sys.stderr, flush=True)
     # compositing bytes are counted over the pairs the launches process (the
@@ -524,7 +527,7 @@ def main():
         line["fwd_only_mpix_s"] = round(r["n_pix"] / (r["ms_fwd_only"] / 1e3) / 1e6, 3)
     if not args.no_traffic:
         try:
-            tr = measure_traffic(args.workload, args.exact, dom, args.bwd_split)
+            tr = measure_traffic(args.workload, args.exact, dom, bwd_opts)
             line["roofline"]["traffic"] = int(tr["bytes"])
             line["roofline"]["traffic_detail"] = {k: (round(v, 3) if isinstance(v, float) else v)
                                                   for k, v in tr.items()}
@@ -537,13 +540,13 @@ def main():
         # the compositing kernels are VALU-issue bound, not HBM bound: report that rate and the FP32 FLOP roof
         try:
             line["roofline"]["valu_issue"] = measure_valu(args.workload, args.exact, dom, line["kernels_ms"][dom],
-                                                          args.bwd_split)
+                                                          bwd_opts)
         except Exception as e:
             line["roofline"]["valu_issue"] = {"error": str(e)}
         print(f"[bench] valu: {line['roofline']['valu_issue']}", file=sys.stderr, flush=True)
         try:
             line["roofline"]["fp32_flops"] = measure_flops(args.workload, args.exact, dom, line["kernels_ms"][dom],
-                                                           args.bwd_split)
+                                                           bwd_opts)
         except Exception as e:
             line["roofline"]["fp32_flops"] = {"error": str(e)}
         print(f"[bench] flops: {line['roofline']['fp32_flops']}", file=sys.stderr, flush=True)

@@ -16,6 +16,7 @@ from ._lib import check, ptr, stream_of
 
 TILE = 16
 GSMI_BWD_SPLIT_BF16 = 1  # include/gsmi.h
+GSMI_BWD_COLOUR_MFMA = 2  # include/gsmi.h
 
 
 def _check_cuda(name, t):
@@ -138,12 +139,13 @@ class _Backend:
     def sh_backward(self, mean, cov, sh_coeffs, alpha, start, end, gaussian_ids, out, T_final, n_done, pre_bg,
                     grad_out, grad_mean, grad_cov, grad_sh_coeffs, grad_alpha, topleft, c2w, tile_size, n_tiles_h,
                     n_tiles_w, pixel_size_x, pixel_size_y, H, W, C, thresh, bg_rgb=None, sh_stride_g=None,
-                    sh_stride_c=None, sh_index=None, split_bf16=False):
+                    sh_stride_c=None, sh_index=None, split_bf16=False, colour_mfma=False):
         """Reverse-order adjoint (accumulates).  From the saved per-pixel
         state, with the tiles launched most expensive first (a small
         workspace holds the order); with T_final = n_done = None the launch
         recomputes that state itself.  split_bf16: the SH-gradient products
-        as a three-term bf16 split (opt-in, not fp32)."""
+        as a three-term bf16 split (opt-in, not fp32).  colour_mfma: the
+        colours formed on fp32 matrix cores (opt-in)."""
         L = _lib.load()
         C = int(C)
         sg = 3 * C * C if sh_stride_g is None else int(sh_stride_g)
@@ -157,7 +159,8 @@ class _Backend:
             ptr(alpha), ptr(start), ptr(end), ptr(gaussian_ids), ptr(out), ptr(T_final), ptr(n_done), ptr(pre_bg),
             ptr(grad_out), ptr(grad_mean), ptr(grad_cov), ptr(grad_sh_coeffs), ptr(grad_alpha), ptr(topleft),
             ptr(c2w), int(tile_size), int(n_tiles_h), int(n_tiles_w), float(pixel_size_x), float(pixel_size_y),
-            int(H), int(W), C, float(thresh), ptr(bg_rgb), GSMI_BWD_SPLIT_BF16 if split_bf16 else 0, ptr(ws), nws,
+            int(H), int(W), C, float(thresh), ptr(bg_rgb),
+            (GSMI_BWD_SPLIT_BF16 if split_bf16 else 0) | (GSMI_BWD_COLOUR_MFMA if colour_mfma else 0), ptr(ws), nws,
             stream_of(grad_out.device)), "render_sh_backward")
 
     # gs/src/render.cu:483-544

@@ -378,7 +378,10 @@ __device__ __forceinline__ int filter_batch(const RasterArgs& a, int e0, int tx,
 // ---------------------------------------------------------------------------
 // TWO: the tile's list is filtered on the fly from its block's list (see
 // filter_batch) and recorded for the backward; single view, no background.
-template <int C, bool BG, bool EXACT, bool TWO>
+// PV: per-view SH rows, output image, list entries and recorded lists
+// (view_sh / view_out / view_ids / view_fids; each null falls back to the
+// shared array); without it the shared arrays, and no registers for them.
+template <int C, bool BG, bool EXACT, bool TWO, bool PV>
 __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
@@ -394,12 +397,12 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
   const int tile = tc.tile;
   const bool inside = tc.inside;
   const int64_t pix = tc.pix;
-  // the view's SH rows and output image (mixed-scene batches), else the shared ones
-  const float* shb = a.view_sh ? a.view_sh[tc.view] : a.sh;
-  float* outb = a.view_out ? a.view_out[tc.view] : a.out;
-  const int64_t opix = a.view_out ? tc.lpix : pix;
-  const int32_t* idsb = a.view_ids ? a.view_ids[tc.view] : a.ids;
-  int32_t* fidsb = a.view_fids ? a.view_fids[tc.view] : a.fids;  // TWO: where the tile's list is recorded
+  // the view's SH rows, output image and lists (PV), else the shared ones
+  const float* shb = (PV && a.view_sh) ? a.view_sh[tc.view] : a.sh;
+  float* outb = (PV && a.view_out) ? a.view_out[tc.view] : a.out;
+  const int64_t opix = (PV && a.view_out) ? tc.lpix : pix;
+  const int32_t* idsb = (PV && a.view_ids) ? a.view_ids[tc.view] : a.ids;
+  int32_t* fidsb = (PV && a.view_fids) ? a.view_fids[tc.view] : a.fids;  // TWO: where the tile's list is recorded
   const int tx = tc.gx / kTile, ty = tc.gy / kTile;  // the view's own tile coordinates (TWO only)
   int s0, e0;
   if constexpr (TWO) {
@@ -575,19 +578,23 @@ __device__ __forceinline__ int next_queued_tile(const RasterArgs& a, int& band, 
   }
 }
 
-template <int C, bool BG, bool EXACT, bool TWO = false>
+template <int C, bool BG, bool EXACT, bool TWO = false, bool PV = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_fwd_sh(RasterArgs a) {
   TileCtx tc;
-  if (!a.work) {
-    if (tile_setup(a, tc)) fwd_sh_tile<C, BG, EXACT, TWO>(a, tc);
-    return;
-  }
+  if (tile_setup(a, tc)) fwd_sh_tile<C, BG, EXACT, TWO, PV>(a, tc);
+}
+
+// The work-queue form (two-level lists, per-view arrays): a persistent
+// launch whose workgroups composite tiles until the queues are empty.
+template <int C, bool EXACT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_fwd_sh_queued(RasterArgs a) {
   __shared__ int s_item;
+  TileCtx tc;
   int band = 0;
   for (;;) {
     const int tile = next_queued_tile(a, band, &s_item);
     if (tile < 0) return;
-    if (tile_setup_at(a, tc, tile)) fwd_sh_tile<C, BG, EXACT, TWO>(a, tc);
+    if (tile_setup_at(a, tc, tile)) fwd_sh_tile<C, false, EXACT, true, true>(a, tc);
   }
 }
 
@@ -1054,12 +1061,22 @@ __device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) 
   lo = __float_as_uint(x - __uint_as_float(b & 0xffff0000u)) >> 16;
 }
 
-template <int C, bool BG, bool SPLIT, bool RECOMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bwd_sh_mfma(RasterArgs a) {
+// CMF: the colours (SH dot products) of up to kChunk relevant entries at a
+// time are one 16x16 (rows ch * kChunk + slot) by 16x64 product on fp32
+// matrix cores, written into the slots' staged rows, where each entry reads
+// its three dot products just before it overwrites them with its
+// coefficients; the SH-gradient product then runs on the same rows.  A chunk
+// is then kChunk relevant entries (contributing or not) instead of kChunk
+// contributing ones.  fp32 products, summed in another order than the VALU
+// dot product.
+template <int C, bool BG, bool SPLIT, bool RECOMP, bool CMF>
+__device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
   constexpr int NCOMP = 6 + SHN;
+  constexpr int KS = (CC + 3) / 4;  // CMF: K-steps of 4 basis functions
   static_assert(CC <= 16, "SH basis wider than the MFMA tile");
+  static_assert(!(CMF && SPLIT), "matrix-core colours with the fp32 reduction only");
   __shared__ GeoA s_ga[kMfmaBatch];
   __shared__ GeoB s_gb[kMfmaBatch];
   __shared__ float2 s_gc[kMfmaBatch];
@@ -1134,6 +1151,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   float bY[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) bY[k] = 0.0f;
+  // CMF B operand: yB[4 s + b] = Y_{4 s + g}(pixel 16 b + j) for lane j + 16 g
+  float yB[CMF ? 4 * KS : 1];
+#pragma unroll
+  for (int k = 0; k < (CMF ? 4 * KS : 1); ++k) yB[k] = 0.0f;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
 #pragma unroll
@@ -1157,6 +1178,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
       } else {
 #pragma unroll
         for (int k = 0; k < 16; ++k) bY[k] = tr[(16 * (lane >> 4) + k) * 8 + col];
+      }
+    }
+    if constexpr (CMF) {
+#pragma unroll
+      for (int st = 0; st < KS; ++st) {
+        const int e = 4 * st + (lane >> 4) - 8 * h;  // basis function of this lane, within this half
+        if (e >= 0 && e < 8 && 8 * h + e < CC) {
+#pragma unroll
+          for (int b = 0; b < 4; ++b) yB[4 * st + b] = tr[(16 * b + (lane & 15)) * 8 + e];
+        }
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -1197,9 +1228,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int le = lane < cnt ? lane : 0;
     uint32_t rel =
         (uint32_t)__ballot(lane < cnt && may_reach(s_ga[le], s_gb[le], s_gc[le].y, wr.x, wr.y, wr.z, wr.w));
+    uint32_t cmask = 0;  // CMF: chunk slots with a contributing lane
     while (rel) {
-      const int j = 31 - __builtin_clz(rel);
-      rel &= ~(1u << j);
+      int gj[kChunk];  // CMF: batch positions of the chunk's entries, last first (wave-uniform)
+      int gn = 1;
+      if constexpr (CMF) {
+        gn = 0;
+#pragma unroll
+        for (int m = 0; m < kChunk; ++m) {
+          gj[m] = 0;
+          if (rel) {
+            gj[m] = 31 - __builtin_clz(rel);
+            rel &= ~(1u << gj[m]);
+            gn = m + 1;
+          }
+        }
+        // A operand: row ch * kChunk + m (this lane's row i = lane % 16), basis 4 s + lane / 16
+        const int ai = lane & 15, am = ai % kChunk, ach = ai / kChunk;
+        int aj = gj[0];
+#pragma unroll
+        for (int m = 1; m < kChunk; ++m) aj = am == m ? gj[m] : aj;
+        const bool arow_ok = ai < kCcRows && am < gn;
+        float av[KS];
+#pragma unroll
+        for (int st = 0; st < KS; ++st) {
+          const int k = 4 * st + (lane >> 4);
+          av[st] = (arow_ok && k < CC) ? s_sh[aj * SHN + ach * CC + k] : 0.0f;
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int st = 0; st < KS; ++st)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[st], yB[4 * st + b], acc, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = 4 * (lane >> 4) + r;
+            if (row < kCcRows) cc_f[cc_pos_f32(row, 16 * b + (lane & 15))] = acc[r];
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      for (int gi = 0; gi < gn; ++gi) {
+      int j;
+      if constexpr (CMF) {
+        j = gj[gi];
+      } else {
+        j = 31 - __builtin_clz(rel);
+        rel &= ~(1u << j);
+      }
       const int idx = base + j;
       // Per-lane decisions are selects, not branches: the wave runs the
       // entry when any lane needs it, and lanes that do not contribute keep
@@ -1230,11 +1308,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
           const float Tj = T * inv_om;
           float w = al * Tj * G;
           w = isnan(w) ? 0.0f : w;
-          const float* shj = s_sh + j * SHN;
+          const int m = CMF ? gi : n_slots;
+          float d0, d1, d2;
+          if constexpr (CMF) {  // this entry's dot products, from the chunk's matrix-core product
+            d0 = cc_f[cc_pos_f32(0 * kChunk + m, lane)];
+            d1 = cc_f[cc_pos_f32(1 * kChunk + m, lane)];
+            d2 = cc_f[cc_pos_f32(2 * kChunk + m, lane)];
+          } else {
+            const float* shj = s_sh + j * SHN;
+            d0 = dot_fast<CC>(shj, Y);
+            d1 = dot_fast<CC>(shj + CC, Y);
+            d2 = dot_fast<CC>(shj + 2 * CC, Y);
+          }
           // w is finite, a sigmoid NaN or in [0, 1]: isnan(y * w) == isnan(y)
-          const float y0 = fmaxf(sigmoid_fast(dot_fast<CC>(shj, Y)), 0.0f);
-          const float y1 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + CC, Y)), 0.0f);
-          const float y2 = fmaxf(sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y)), 0.0f);
+          const float y0 = fmaxf(sigmoid_fast(d0), 0.0f);
+          const float y1 = fmaxf(sigmoid_fast(d1), 0.0f);
+          const float y2 = fmaxf(sigmoid_fast(d2), 0.0f);
           const float cc0 = contrib ? w * (y0 * (1.0f - y0)) * g0 : 0.0f;
           const float cc1 = contrib ? w * (y1 * (1.0f - y1)) * g1 : 0.0f;
           const float cc2 = contrib ? w * (y2 * (1.0f - y2)) * g2 : 0.0f;
@@ -1258,7 +1347,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
           v[5] = contrib ? P * G : 0.0f;
           v[6] = 0.0f;
           v[7] = 0.0f;
-          const int m = n_slots;
           if constexpr (SPLIT) {
             uint32_t h, l;
             split_bf16(cc0, h, l);
@@ -1275,14 +1363,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             cc_f[cc_pos_f32(1 * kChunk + m, lane)] = cc1;
             cc_f[cc_pos_f32(2 * kChunk + m, lane)] = cc2;
           }
-          slots |= static_cast<uint32_t>(j) << (kSlotBits * m);
-          n_slots = m + 1;
+          if constexpr (CMF) {
+            cmask |= 1u << m;
+          } else {
+            slots |= static_cast<uint32_t>(j) << (kSlotBits * m);
+            n_slots = m + 1;
+          }
           if (lane == 0) atomicOr(&s_touch, 1u << j);
           const float r = wave_reduce_scatter8_swap(v, lane);
           if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r);
         }
       }
-      if (n_slots == kChunk || (rel == 0 && n_slots > 0)) {
+      }  // entries of the chunk
+      if (CMF ? cmask != 0u : (n_slots == kChunk || (rel == 0 && n_slots > 0))) {
         // LDS operations of one wave complete in order; the barrier keeps the
         // compiler from moving the loads above the stores.
         __builtin_amdgcn_wave_barrier();
@@ -1315,7 +1408,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
           for (int r = 0; r < 4; ++r) {
             const int row = 4 * d_ch + r;  // = ch * kChunk + slot
             const int ch = row / kChunk, m = row - ch * kChunk;
-            if (row < kCcRows && m < n_slots) {
+            if constexpr (CMF) {
+              if (row < kCcRows && ((cmask >> m) & 1u)) {
+                int jj = gj[0];
+#pragma unroll
+                for (int mm = 1; mm < kChunk; ++mm) jj = m == mm ? gj[mm] : jj;
+                shacc_w[jj * SHN + ch * CC + d_k] = acc[r];
+              }
+            } else if (row < kCcRows && m < n_slots) {
               const int jj = (slots >> (kSlotBits * m)) & ((1u << kSlotBits) - 1);
               shacc_w[jj * SHN + ch * CC + d_k] = acc[r];
             }
@@ -1324,6 +1424,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         __builtin_amdgcn_wave_barrier();
         n_slots = 0;
         slots = 0;
+        cmask = 0u;
       }
     }
     __syncthreads();
@@ -1347,6 +1448,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
       if (v != 0.0f) emit_grad<SHN>(a, s_id[jj], c, v);
     }
   }
+}
+
+template <int C, bool BG, bool SPLIT, bool RECOMP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bwd_sh_mfma(RasterArgs a) {
+  bwd_sh_mfma_body<C, BG, SPLIT, RECOMP, false>(a);
+}
+
+// The matrix-core colours need the 16 basis values of the product's B
+// operand in registers as well: four waves per SIMD instead of five.
+template <int C, bool BG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bwd_sh_mfma_cmf(RasterArgs a) {
+  bwd_sh_mfma_body<C, BG, false, false, true>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1679,11 +1792,13 @@ static int launch_fwd_sh(const RasterArgs& a, int exact, hipStream_t st) {
 // recomp: no saved forward state (a.T_saved == null); split: bf16 split of
 // the SH-gradient reduction (orders 3 and 4, saved state only).
 template <int C, bool BG>
-static int launch_bwd_sh(const RasterArgs& a, bool split, bool recomp, hipStream_t st) {
+static int launch_bwd_sh(const RasterArgs& a, bool split, bool recomp, hipStream_t st, bool cmf = false) {
   const int grid = xcd_grid(a.n_tiles_total);
   if constexpr (C >= 3) {
     if (recomp)
       hipLaunchKernelGGL((k_bwd_sh_mfma<C, BG, false, true>), dim3(grid), dim3(256), 0, st, a);
+    else if (cmf)
+      hipLaunchKernelGGL((k_bwd_sh_mfma_cmf<C, BG>), dim3(grid), dim3(256), 0, st, a);
     else if (split)
       hipLaunchKernelGGL((k_bwd_sh_mfma<C, BG, true, false>), dim3(grid), dim3(256), 0, st, a);
     else
@@ -1698,14 +1813,16 @@ static int launch_bwd_sh(const RasterArgs& a, bool split, bool recomp, hipStream
   return GSMI_OK;
 }
 
-static int dispatch_bwd_sh(const RasterArgs& a, int C, bool split, bool recomp, hipStream_t st) {
+static int dispatch_bwd_sh(const RasterArgs& a, int C, int flags, bool recomp, hipStream_t st) {
   const bool B = a.bg != nullptr;
+  const bool split = (flags & GSMI_BWD_SPLIT_BF16) != 0, cmf = (flags & GSMI_BWD_COLOUR_MFMA) != 0;
   switch (C) {
     case 1: return B ? launch_bwd_sh<1, true>(a, split, recomp, st) : launch_bwd_sh<1, false>(a, split, recomp, st);
     case 2: return B ? launch_bwd_sh<2, true>(a, split, recomp, st) : launch_bwd_sh<2, false>(a, split, recomp, st);
-    case 3: return B ? launch_bwd_sh<3, true>(a, split, recomp, st) : launch_bwd_sh<3, false>(a, split, recomp, st);
+    case 3:
+      return B ? launch_bwd_sh<3, true>(a, split, recomp, st, cmf) : launch_bwd_sh<3, false>(a, split, recomp, st, cmf);
     default:
-      return B ? launch_bwd_sh<4, true>(a, split, recomp, st) : launch_bwd_sh<4, false>(a, split, recomp, st);
+      return B ? launch_bwd_sh<4, true>(a, split, recomp, st, cmf) : launch_bwd_sh<4, false>(a, split, recomp, st, cmf);
   }
 }
 
@@ -1723,8 +1840,9 @@ static int check_bwd_state(const float* out, const float* T_final, const int32_t
                            const float* bg, int flags, const void* workspace, bool* recomp) {
   if ((T_final == nullptr) != (n_done == nullptr)) return GSMI_ERR_ARG;
   *recomp = T_final == nullptr;
-  if (flags & ~GSMI_BWD_SPLIT_BF16) return GSMI_ERR_ARG;
-  if (*recomp && (flags & GSMI_BWD_SPLIT_BF16)) return GSMI_ERR_UNSUPPORTED;
+  if (flags & ~(GSMI_BWD_SPLIT_BF16 | GSMI_BWD_COLOUR_MFMA)) return GSMI_ERR_ARG;
+  if (*recomp && flags) return GSMI_ERR_UNSUPPORTED;
+  if ((flags & GSMI_BWD_SPLIT_BF16) && (flags & GSMI_BWD_COLOUR_MFMA)) return GSMI_ERR_UNSUPPORTED;
   if (*recomp && workspace) return GSMI_ERR_ARG;  // the launch order is derived from the saved n_done
   if (bg && (!out || (!*recomp && !pre_bg))) return GSMI_ERR_ARG;
   return GSMI_OK;
@@ -1847,7 +1965,7 @@ extern "C" int gsmi_render_sh_backward(int64_t V, int64_t P, const float* mean, 
     rc = make_tile_order(a, workspace, st);
     if (rc) return rc;
   }
-  return dispatch_bwd_sh(a, C, (flags & GSMI_BWD_SPLIT_BF16) != 0, recomp, st);
+  return dispatch_bwd_sh(a, C, flags, recomp, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -2180,7 +2298,7 @@ static int views_backward_impl(int64_t V, int64_t P, int B, const gsmi_view* vie
     rc = make_tile_order(a, workspace, st);
     if (rc) return rc;
   }
-  return dispatch_bwd_sh(a, C, (flags & GSMI_BWD_SPLIT_BF16) != 0, recomp, st);
+  return dispatch_bwd_sh(a, C, flags, recomp, st);
 }
 
 extern "C" int gsmi_render_views_backward_sh(int64_t V, int64_t P, int B, const gsmi_view* views,
@@ -2254,16 +2372,30 @@ __global__ void k_zero_work(int* w) {
 // grid_cap > 0 with a.work: a work-queue launch of at most grid_cap workgroups.
 static int launch_fwd_two(const RasterArgs& a, int C, int exact, hipStream_t st, int grid_cap = 0) {
   int grid = xcd_grid(a.n_tiles_total);
-  if (a.work) {
+  const bool queued = a.work != nullptr;
+  const bool pv = a.view_sh || a.view_out || a.view_ids || a.view_fids;
+  if (queued) {
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, st, a.work);  // a fill kernel, not a memset node
     grid = std::min(grid, std::max(grid_cap, 8));
   }
-#define GSMI_FWD2(CC_)                                                                      \
-  do {                                                                                       \
-    if (exact)                                                                               \
-      hipLaunchKernelGGL((k_fwd_sh<CC_, false, true, true>), dim3(grid), dim3(256), 0, st, a);  \
-    else                                                                                     \
-      hipLaunchKernelGGL((k_fwd_sh<CC_, false, false, true>), dim3(grid), dim3(256), 0, st, a); \
+#define GSMI_FWD2(CC_)                                                                                     \
+  do {                                                                                                      \
+    if (queued) {                                                                                           \
+      if (exact)                                                                                            \
+        hipLaunchKernelGGL((k_fwd_sh_queued<CC_, true>), dim3(grid), dim3(256), 0, st, a);                  \
+      else                                                                                                  \
+        hipLaunchKernelGGL((k_fwd_sh_queued<CC_, false>), dim3(grid), dim3(256), 0, st, a);                 \
+    } else if (pv) {                                                                                        \
+      if (exact)                                                                                            \
+        hipLaunchKernelGGL((k_fwd_sh<CC_, false, true, true, true>), dim3(grid), dim3(256), 0, st, a);      \
+      else                                                                                                  \
+        hipLaunchKernelGGL((k_fwd_sh<CC_, false, false, true, true>), dim3(grid), dim3(256), 0, st, a);     \
+    } else {                                                                                                \
+      if (exact)                                                                                            \
+        hipLaunchKernelGGL((k_fwd_sh<CC_, false, true, true>), dim3(grid), dim3(256), 0, st, a);            \
+      else                                                                                                  \
+        hipLaunchKernelGGL((k_fwd_sh<CC_, false, false, true>), dim3(grid), dim3(256), 0, st, a);           \
+    }                                                                                                       \
   } while (0)
   switch (C) {
     case 1: GSMI_FWD2(1); break;

@@ -257,13 +257,15 @@ def forward_view(mean, qvec, log_s, logit, sh_coeffs, c2w, camera, C, cfg, exact
     return out, ViewState(b, int(C), cfg, topleft, c2w_d, bg, out, T, nd, pre, early.get("g_sh"), early.get("bnd"))
 
 
-def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g_sh=None, split_bf16=False):
+def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g_sh=None, split_bf16=False,
+                  colour_mfma=False):
     """Steps 7-8: gradients w.r.t. the raw parameters.  SH gradients are
     accumulated into g_sh when given (several views), else a new buffer.
     Returns (g_mean, g_qvec, g_log_s, g_logit, g_sh, boundary) where boundary
     holds the compositing gradients (mean2d, cov2d, alpha) of visible
     Gaussians.  split_bf16: SH-gradient products on bf16 matrix cores as a
-    three-term split (opt-in; the default forms them in fp32)."""
+    three-term split (opt-in; the default forms them in fp32); colour_mfma:
+    the colours formed on fp32 matrix cores (opt-in)."""
     b, cfg, ic = s.b, s.cfg, s.b.consts
     dev = mean.device
     L = _lib.load()
@@ -286,7 +288,7 @@ def backward_view(s: ViewState, mean, qvec, log_s, logit, sh_coeffs, grad_out, g
                          s.pre_bg, grad_out.contiguous().view(-1), g_m2d, g_cov, g_sh, g_al, s.topleft, s.c2w_d,
                          cfg.tile_size, ic["n_tiles_h"], ic["n_tiles_w"], ic["pixel_size_x"], ic["pixel_size_y"],
                          ic["H"], ic["W"], s.C, cfg.T_thresh, bg_rgb=s.bg, sh_stride_g=3 * cmax2, sh_stride_c=cmax2,
-                         sh_index=b.orig, split_bf16=split_bf16)
+                         sh_index=b.orig, split_bf16=split_bf16, colour_mfma=colour_mfma)
     g_mean = torch.empty_like(mean)
     g_q = torch.empty_like(qvec)
     g_ls = torch.empty_like(log_s)

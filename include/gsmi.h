@@ -277,6 +277,11 @@ int gsmi_render_sh_forward(int64_t V, int64_t P, const float *mean, const float 
  * error up to 2^-15, biased toward zero) instead of fp32.  Off by default;
  * not fp32 arithmetic. */
 #define GSMI_BWD_SPLIT_BF16 1
+/* flags bit of the SH backward entries (saved state, SH orders 3 and 4, not
+ * with GSMI_BWD_SPLIT_BF16): the colours (SH dot products) of the entries
+ * are formed on fp32 matrix cores, five list entries at a time, instead of
+ * per entry on the vector units; fp32 products, summed in another order. */
+#define GSMI_BWD_COLOUR_MFMA 2
 
 /*
  * Extended SH backward (reverse-order adjoint).  With the forward's T_final
@@ -287,7 +292,8 @@ int gsmi_render_sh_forward(int64_t V, int64_t P, const float *mean, const float 
  * atomics) into grad_mean [V,2], grad_cov [V,4], grad_sh (same addressing as
  * sh) and grad_alpha [V].  out is the forward's output (read with bg only:
  * the blended colour) and pre_bg the colour before blending (bg with saved
- * state only).  flags: 0 or GSMI_BWD_SPLIT_BF16 (saved state only).
+ * state only).  flags: 0, GSMI_BWD_SPLIT_BF16 or GSMI_BWD_COLOUR_MFMA (saved
+ * state only).
  * workspace (saved state only; NULL for none) of
  * gsmi_backward_order_workspace_size(n_tiles_h * n_tiles_w) bytes: the tiles
  * are launched most expensive first (cost from n_done), which shortens the

@@ -12,7 +12,9 @@
    28-36); on a scene of mostly non-overlapping Gaussians with a mixed-sign
    gradient image (the per-Gaussian sums cancel), every SH-gradient element
    is within 1e-5 of the oracle relative to itself, plus 2e-6 of its
-   absolute-term sum.  The opt-in bf16 split misses that bound.
+   absolute-term sum.  The opt-in bf16 split misses that bound; the opt-in
+   colours on fp32 matrix cores (GSMI_BWD_COLOUR_MFMA) keep it and every
+   bound of item 1.
 """
 import ctypes
 
@@ -106,13 +108,16 @@ def test_reference_signature_sh_entries(orc, C, with_bg):
     exact = orc.render_sh_bwd_f64(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]], ref["start"],
                                   ref["end"], ref["ids"], gout_np, ref["topleft"], c2w, 16, nth, ntw, ref["psx"],
                                   ref["psy"], H, W, C, 1e-4, mode=1 if with_bg else 0, bg=bg)
-    for name, got in zip(("mean", "cov", "sh", "alpha"), g):
-        want = np.asarray(rg[name]).reshape(got.shape)
-        assert rel_err(got.cpu(), want) <= GRAD_RTOL, name
-        ex = elementwise_excess(got, want, ab[name], GRAD_RTOL, 1e-5)
-        assert ex <= 1.0, (name, ex)
-        ex = exact_excess(got, exact[name], GRAD_RTOL, 1e-6)
-        assert ex <= 1.0, (name, "vs float64", ex)
+    def check(g):
+        for name, got in zip(("mean", "cov", "sh", "alpha"), g):
+            want = np.asarray(rg[name]).reshape(got.shape)
+            assert rel_err(got.cpu(), want) <= GRAD_RTOL, name
+            ex = elementwise_excess(got, want, ab[name], GRAD_RTOL, 1e-5)
+            assert ex <= 1.0, (name, ex)
+            ex = exact_excess(got, exact[name], GRAD_RTOL, 1e-6)
+            assert ex <= 1.0, (name, "vs float64", ex)
+
+    check(g)
     assert float(g[2].abs().max()) > 0
 
     # the saved-state backward (forward state from the extended forward) agrees
@@ -130,6 +135,12 @@ def test_reference_signature_sh_entries(orc, C, with_bg):
                          bg_rgb=bg_d)
     for a, b in zip(g, g2):
         assert rel_err(a.cpu(), b.cpu()) <= 1e-5
+    # the opt-in colours on fp32 matrix cores: the same bounds
+    g4 = grads()
+    _backend.sh_backward(t["mean"], t["cov"], t["sh"], t["alpha"], t["start"], t["end"], t["ids"], out, T, nd, pre,
+                         gout, *g4, t["topleft"], t["c2w"], 16, nth, ntw, ref["psx"], ref["psy"], H, W, C, 1e-4,
+                         bg_rgb=bg_d, colour_mfma=True)
+    check(g4)
 
     # the reference-named in-place Python methods call the same entries
     out3 = torch.zeros_like(out)
@@ -193,7 +204,7 @@ def test_reference_signature_rgb_entries(orc):
         assert rel_err(a.cpu(), b.cpu()) <= 1e-5
 
 
-def _sh_grad_excess(orc, split):
+def _sh_grad_excess(orc, split, cmf=False):
     """(elementwise excess of the GPU SH gradients over the fp32 bound, the
     fraction of elements over it) on a sparse scene with a mixed-sign
     gradient image."""
@@ -214,7 +225,7 @@ def _sh_grad_excess(orc, split):
          torch.zeros(V, device=DEV)]
     _backend.sh_backward(t["mean"], t["cov"], t["sh"], t["alpha"], t["start"], t["end"], t["ids"], out, T, nd, None,
                          torch.from_numpy(gout_np).to(DEV), *g, t["topleft"], t["c2w"], 16, nth, ntw, ref["psx"],
-                         ref["psy"], H, W, 4, 1e-4, split_bf16=split)
+                         ref["psy"], H, W, 4, 1e-4, split_bf16=split, colour_mfma=cmf)
     rg, _, ab = orc.render_sh_bwd(ref["mean2d"], ref["cov2d"], ref["sh_v"], ref["a"][ref["idx"]], ref["start"],
                                   ref["end"], ref["ids"], ref["out"], gout_np, ref["topleft"], c2w, 16, nth, ntw,
                                   ref["psx"], ref["psy"], H, W, 4, 1e-4, want_abs=True)
@@ -230,6 +241,13 @@ def _sh_grad_excess(orc, split):
 def test_sh_gradient_fp32_elementwise(orc):
     excess, frac = _sh_grad_excess(orc, split=False)
     print(f"fp32 SH-gradient excess {excess:.3g} (fraction over the bound {frac:.3g})")
+    assert excess <= 1.0
+
+
+def test_sh_gradient_colour_mfma_fp32_elementwise(orc):
+    """The opt-in colour products on fp32 matrix cores keep the fp32 bound."""
+    excess, frac = _sh_grad_excess(orc, split=False, cmf=True)
+    print(f"colour-MFMA SH-gradient excess {excess:.3g} (fraction over the bound {frac:.3g})")
     assert excess <= 1.0
 
 
