@@ -175,9 +175,10 @@ def plain_curve(pool, cap, levels, n_scenes, seed=7, n=2000):
     for lv in levels:
         reqs = request_stream(n, lv * cap, n_scenes, seed + 10 + int(lv * 10))
         svc = np.array([rng.choice(pool[_req_class(r)]) for r in reqs])
-        lat = fifo_latencies(np.array([r.t for r in reqs]), svc) * 1e3
-        span = max(r.t for r in reqs) + svc[-1]
-        out.append(dict(offered_qps=round(lv * cap, 2), views_s=round(min(len(reqs) / span, lv * cap), 2),
+        arrivals = np.array([r.t for r in reqs])
+        lat = fifo_latencies(arrivals, svc) * 1e3
+        span = float((arrivals + lat / 1e3).max() - arrivals.min())  # first arrival to last completion
+        out.append(dict(offered_qps=round(lv * cap, 2), views_s=round(len(reqs) / span, 2),
                         p50_ms=round(float(np.percentile(lat, 50)), 3), p99_ms=round(float(np.percentile(lat, 99)), 3)))
     return out
 
