@@ -911,7 +911,7 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
     __syncthreads();
     for (int j = cnt - 1; j >= 0; --j) {
       const int idx = base + j;
-      if (!__any(idx < nd)) continue;
+      if (!wave_any(idx < nd)) continue;
       float cc0 = 0.0f, cc1 = 0.0f, cc2 = 0.0f, Ds = 0.0f, Du = 0.0f, Dss = 0.0f, Dsu = 0.0f, Duu = 0.0f,
             PG = 0.0f;
       bool contrib = false;
@@ -962,7 +962,7 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
           }
         }
       }
-      if (!__any(contrib)) continue;
+      if (!wave_any(contrib)) continue;
       float v[NV];
       v[0] = Ds;
       v[1] = Du;
@@ -1049,10 +1049,13 @@ __device__ __forceinline__ int cc_pos(int row, int p) {
   return row * kCcStride + (p ^ ((row & 7) << 3));  // p < 64: block p / 8 -> (p / 8) ^ (row & 7)
 }
 
-// Position (floats) of pixel p in staged fp32 row `row`: 4-float blocks XOR-swizzled by row.
-__device__ __forceinline__ int cc_pos_f32(int row, int p) {
-  return row * 64 + (p ^ ((row & 15) << 2));  // p < 64: block p / 4 -> (p / 4) ^ (row & 15)
-}
+// Position (floats) of pixel p in the staged fp32 rows: pixel-major with the
+// odd stride kCcRows, so a row written by the 64 lanes (15 p + row) and one
+// pixel of 16 rows x 4 pixel groups read by the matrix-core operand
+// (15 (16 g + k) + i) both fall in distinct banks, and every row of a lane
+// is its base plus a constant.
+__device__ __forceinline__ int cc_pos_f32(int row, int p) { return p * kCcRows + row; }
+static_assert(kCcRows % 2 == 1 && 64 * kCcRows <= kCcWords, "fp32 staging layout");
 
 // Upper 16 bits of x (x truncated to bf16) and of the remainder x - hi.
 __device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) {
@@ -1114,7 +1117,9 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   static_assert(kMfmaBatch <= 32, "one batch entry per lane in the relevance mask");
   const float4 wr = wave_pixel_rect(px, py);
 
-  float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
+  // gS = sum_c g_c S_c: the colour behind the entry (S_c, accumulated in
+  // reverse order) enters the adjoint only through this grad_out-weighted sum.
+  float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, gS = 0.0f;
   int nd = 0;
   float pre0 = 0.0f, pre1 = 0.0f, pre2 = 0.0f;
   if constexpr (RECOMP)
@@ -1134,9 +1139,10 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
     g1 = a.grad_out[3 * pix + 1];
     g2 = a.grad_out[3 * pix + 2];
     if (BG) {
-      S0 = a.out_saved[3 * pix + 0] - pre0;
-      S1 = a.out_saved[3 * pix + 1] - pre1;
-      S2 = a.out_saved[3 * pix + 2] - pre2;
+      const float S0 = a.out_saved[3 * pix + 0] - pre0;
+      const float S1 = a.out_saved[3 * pix + 1] - pre1;
+      const float S2 = a.out_saved[3 * pix + 2] - pre2;
+      gS = fmaf(g2, S2, fmaf(g1, S1, g0 * S0));
     }
   }
 
@@ -1196,16 +1202,18 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   const int nmax = block_max(nd, s_tmp);
   const int my_comp = lane >> 3;                      // scalar reduce-scatter (8 wide)
   const bool writer = (lane & 7) == 0 && my_comp < 6;
+  // the covariance components (2..4) carry a factor 1/2, applied to the wave
+  // sums (a power-of-two scale commutes with rounding)
+  const float comp_scale = (my_comp >= 2 && my_comp <= 4) ? 0.5f : 1.0f;
   // MFMA A operand: lane i + 16g reads row i (row 15 repeats row 0; its
   // output row is discarded).  SPLIT: pixels 32t + 8g .. +7 of each plane.
-  // FP32: pixels 16g .. 16g + 15, blocks 4g + q (q = 0..3) of the row.
+  // FP32: pixels 16g .. 16g + 15 of the row.
   const int arow = (lane & 15) < kCcRows ? (lane & 15) : 0;
   uint16_t* cc_w = reinterpret_cast<uint16_t*>(cc_raw);
   float* cc_f = reinterpret_cast<float*>(cc_raw);
   const u32x4* a_src0 = reinterpret_cast<const u32x4*>(cc_w + cc_pos(arow, 8 * (lane >> 4)));
   const u32x4* a_src1 = reinterpret_cast<const u32x4*>(cc_w + cc_pos(arow, 32 + 8 * (lane >> 4)));
-  const floatx4* a_row = reinterpret_cast<const floatx4*>(cc_f + arow * 64);
-  const int a_blk = (4 * (lane >> 4)) ^ (arow & 15);  // block 4g + q sits at a_blk ^ q
+  const float* a_col = cc_f + cc_pos_f32(arow, 16 * (lane >> 4));  // FP32: pixel 16g + k at a_col[k kCcRows]
   float* shacc_w = s_shacc[wv];
   const int d_ch = lane >> 4;                          // channel of this lane's MFMA output rows
   const int d_k = lane & 15;                           // basis function of this lane's output column
@@ -1229,6 +1237,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
     uint32_t rel =
         (uint32_t)__ballot(lane < cnt && may_reach(s_ga[le], s_gb[le], s_gc[le].y, wr.x, wr.y, wr.z, wr.w));
     uint32_t cmask = 0;  // CMF: chunk slots with a contributing lane
+    uint32_t wtouch = 0;  // batch entries this wave contributed to (wave-uniform)
     while (rel) {
       int gj[kChunk];  // CMF: batch positions of the chunk's entries, last first (wave-uniform)
       int gn = 1;
@@ -1284,7 +1293,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
       // their state (a divergent branch here cost more in exec-mask handling
       // and re-zeroing than the masked arithmetic it skipped).
       const bool live = idx < nd;
-      if (__any(live)) {
+      if (wave_any(live)) {
         const GeoA ga = s_ga[j];
         const GeoB gb = s_gb[j];
         const float2 gc = s_gc[j];
@@ -1297,12 +1306,12 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
         const bool near = live && !(rq > gc.y);
         float G = gauss_fast(rq);
         const bool exact = near && gauss_needs_exact(al * G, q);
-        if (__any(exact)) {
+        if (wave_any(exact)) {
           if (exact) G = gauss_exact(q, gb.det);
         }
         const float aG = al * G;
         const bool contrib = near && !(aG < kMinRenderAlpha);
-        if (__any(contrib)) {
+        if (wave_any(contrib)) {
           const float om = 1.0f - aG;
           const float inv_om = __builtin_amdgcn_rcpf(om);
           const float Tj = T * inv_om;
@@ -1324,15 +1333,13 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           const float y0 = fmaxf(sigmoid_fast(d0), 0.0f);
           const float y1 = fmaxf(sigmoid_fast(d1), 0.0f);
           const float y2 = fmaxf(sigmoid_fast(d2), 0.0f);
-          const float cc0 = contrib ? w * (y0 * (1.0f - y0)) * g0 : 0.0f;
-          const float cc1 = contrib ? w * (y1 * (1.0f - y1)) * g1 : 0.0f;
-          const float cc2 = contrib ? w * (y2 * (1.0f - y2)) * g2 : 0.0f;
-          float P = g0 * (y0 * Tj - S0 * inv_om);
-          P = fmaf(g1, y1 * Tj - S1 * inv_om, P);
-          P = fmaf(g2, y2 * Tj - S2 * inv_om, P);
-          S0 = contrib ? fmaf(w, y0, S0) : S0;
-          S1 = contrib ? fmaf(w, y1, S1) : S1;
-          S2 = contrib ? fmaf(w, y2, S2) : S2;
+          const float cc0 = contrib ? w * fmaf(-y0, y0, y0) * g0 : 0.0f;  // y (1 - y): the sigmoid's slope
+          const float cc1 = contrib ? w * fmaf(-y1, y1, y1) * g1 : 0.0f;
+          const float cc2 = contrib ? w * fmaf(-y2, y2, y2) * g2 : 0.0f;
+          const float gy = fmaf(g2, y2, fmaf(g1, y1, g0 * y0));
+          // sum_c g_c (y_c T_j - S_c / (1 - aG))
+          const float P = fmaf(gy, Tj, -(gS * inv_om));
+          gS = contrib ? fmaf(w, gy, gS) : gS;
           T = contrib ? Tj : T;
           const float D = P * aG;
           const float sx = tmpx * gb.inv_det;
@@ -1341,9 +1348,9 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           float v[8];
           v[0] = contrib ? Ds : 0.0f;
           v[1] = contrib ? Du : 0.0f;
-          v[2] = contrib ? 0.5f * Ds * sx : 0.0f;
-          v[3] = contrib ? 0.5f * Ds * uy : 0.0f;
-          v[4] = contrib ? 0.5f * Du * uy : 0.0f;
+          v[2] = contrib ? Ds * sx : 0.0f;  // x 1/2 after the reduction (comp_scale)
+          v[3] = contrib ? Ds * uy : 0.0f;
+          v[4] = contrib ? Du * uy : 0.0f;
           v[5] = contrib ? P * G : 0.0f;
           v[6] = 0.0f;
           v[7] = 0.0f;
@@ -1369,9 +1376,9 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
             slots |= static_cast<uint32_t>(j) << (kSlotBits * m);
             n_slots = m + 1;
           }
-          if (lane == 0) atomicOr(&s_touch, 1u << j);
+          wtouch |= 1u << j;
           const float r = wave_reduce_scatter8_swap(v, lane);
-          if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r);
+          if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r * comp_scale);
         }
       }
       }  // entries of the chunk
@@ -1396,12 +1403,11 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al1),
                                                         __builtin_bit_cast(bf16x8, bh[1]), acc, 0, 0, 0);
         } else {
-          floatx4 av[4];
+          float av[16];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) av[q] = a_row[a_blk ^ q];
+          for (int k = 0; k < 16; ++k) av[k] = a_col[k * kCcRows];
 #pragma unroll
-          for (int k = 0; k < 16; ++k)
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[k >> 2][k & 3], bY[k], acc, 0, 0, 0);
+          for (int k = 0; k < 16; ++k) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[k], bY[k], acc, 0, 0, 0);
         }
         if (d_k < CC) {
 #pragma unroll
@@ -1427,6 +1433,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
         cmask = 0u;
       }
     }
+    if (lane == 0 && wtouch) atomicOr(&s_touch, wtouch);
     __syncthreads();
     const uint32_t touch = s_touch;
     if (t < kMfmaBatch && ((touch >> t) & 1u)) s_tlist[__popc(touch & ((1u << t) - 1u))] = t;
@@ -1610,7 +1617,7 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
     __syncthreads();
     for (int j = cnt - 1; j >= 0; --j) {
       const int idx = base + j;
-      if (!__any(idx < nd)) continue;
+      if (!wave_any(idx < nd)) continue;
       float c0v = 0.0f, c1v = 0.0f, c2v = 0.0f, Ds = 0.0f, Du = 0.0f, Dss = 0.0f, Dsu = 0.0f, Duu = 0.0f,
             PG = 0.0f;
       bool contrib = false;
@@ -1656,7 +1663,7 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
           }
         }
       }
-      if (!__any(contrib)) continue;
+      if (!wave_any(contrib)) continue;
       float v[NV] = {Ds, Du, Dss, Dsu, Duu, PG, c0v, c1v, c2v, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       const float r = wave_reduce_scatter<NV>(v);
       if (writer && r != 0.0f) atomicAdd(s_acc + j * NCOMP + my_comp, r);

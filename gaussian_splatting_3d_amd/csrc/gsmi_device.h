@@ -138,6 +138,9 @@ __device__ __forceinline__ int xcd_tile(int b, int n_tiles) {
 inline int xcd_grid(int n_tiles) { return ((n_tiles + 7) >> 3) << 3; }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+// Whether p holds in some active lane: a ballot of the condition mask
+// itself (HIP's __any widens p to an int and compares it again per lane).
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
 static_assert(sizeof(gsmi_view) == 256, "gsmi_view is a 256-byte record");
 
