@@ -512,7 +512,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+bf16x3-split" if args.bwd_split else "f32",
         "data": "synthetic (seeded scene and target, random-init Gaussians)",
-        "config": {"workload": scene.name, "gaussians": r["N"], "visible": r["V"], "pairs": r["P"],
+        "config": {"workload": scene.name, "gaussians": r["N"], "visible": r["V"],
+                   ("block_pairs" if r.get("shift") else "pairs"): r["P"],
                    "width": r["W"], "height": r["H"], "sh_degree": r["C"] - 1, "views": r["views"]},
         "gaussians_per_s_fwd_bwd": round(r["N"] / (r["ms_step"] / 1e3), 1),
         "kernels_ms": {"composite_fwd": round(r["ms_composite_fwd"], 4),

@@ -1683,24 +1683,37 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
 // within a level the tiles keep image order, so neighbours (which share
 // Gaussians) still run close together.
 // ---------------------------------------------------------------------------
+// Cost of each tile: the sum over its four waves of the most entries a pixel
+// of the wave examined, plus a fixed 32 (0 for an empty list).  Workgroups
+// take tiles in a grid-stride loop (a launch of one small workgroup per tile
+// is bound by workgroup dispatch on grids of 10^5 tiles).
+constexpr int kTileCostGrid = 2048;
 __global__ __launch_bounds__(256) void k_tile_cost(RasterArgs a, uint32_t* __restrict__ cost) {
-  __shared__ int s_w[4];
+  __shared__ int s_w[2][4];
   TileCtx tc;
-  if (!tile_setup(a, tc)) return;
-  int nd = tc.inside ? a.n_done_saved[tc.pix] : 0;
+  for (int tile = blockIdx.x, par = 0; tile < a.n_tiles_total; tile += gridDim.x, par ^= 1) {
+    tile_setup_at(a, tc, tile);
+    int nd = tc.inside ? a.n_done_saved[tc.pix] : 0;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) nd = max(nd, __shfl_xor(nd, o));
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = nd;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int s0 = a.start[tc.tile], e0 = a.end[tc.tile];
-    const bool empty = s0 == -1 || e0 <= s0;
-    cost[tc.tile] = empty ? 0u : (uint32_t)(s_w[0] + s_w[1] + s_w[2] + s_w[3]) + 32u;
+    for (int o = 32; o > 0; o >>= 1) nd = max(nd, __shfl_xor(nd, o));
+    if ((threadIdx.x & 63) == 0) s_w[par][threadIdx.x >> 6] = nd;  // alternate slots: one barrier per tile
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int s0 = a.start[tile], e0 = a.end[tile];
+      const bool empty = s0 == -1 || e0 <= s0;
+      cost[tile] = empty ? 0u : (uint32_t)(s_w[par][0] + s_w[par][1] + s_w[par][2] + s_w[par][3]) + 32u;
+    }
   }
 }
 
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderLevels = 16;  // cost levels; within a level tiles keep their index order
+constexpr int kOrderSingleMax = 1 << 15;  // grids up to this many tiles are ordered by one workgroup
+
+// Cost level of a tile: 0 holds the most expensive (mx = the largest cost, >= 1).
+__device__ __forceinline__ uint32_t order_level(uint32_t c, uint64_t mx) {
+  return (uint32_t)((kOrderLevels - 1) - (int)(((uint64_t)c * (uint64_t)(kOrderLevels - 1)) / mx));
+}
 
 __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __restrict__ cost, int n,
                                                               int32_t* __restrict__ order) {
@@ -1717,10 +1730,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
   atomicMax(&s_max, m);
   __syncthreads();
   const uint64_t mx = s_max;
-  // level 0 holds the most expensive tiles
-  auto level_of = [mx](uint32_t c) {
-    return (kOrderLevels - 1) - (int)(((uint64_t)c * (uint64_t)(kOrderLevels - 1)) / mx);
-  };
+  auto level_of = [mx](uint32_t c) { return (int)order_level(c, mx); };
   for (int i = t; i < n; i += kOrderThreads) atomicAdd(&s_cursor[level_of(cost[i])], 1u);
   __syncthreads();
   if (t == 0) {  // exclusive prefix over the levels
@@ -1767,16 +1777,108 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
   }
 }
 
+// Larger grids: the same levels, placed by many workgroups.  Counter words
+// (cw): [0] the largest cost, [1 + l] the tiles of level l, [1 + kOrderLevels
+// + l] level l's next free position.  Each workgroup places kOrderThreads
+// tiles, reserving its run of each level with one atomic, so within a level
+// tiles keep their index order inside a workgroup and workgroups follow in
+// the order they reserve.
+constexpr int kOrderWords = 1 + 2 * kOrderLevels;
+
+__global__ void k_order_zero(uint32_t* __restrict__ cw) {
+  if ((int)threadIdx.x < kOrderWords) cw[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_order_max(const uint32_t* __restrict__ cost, int n, uint32_t* __restrict__ cw) {
+  uint32_t m = 0u;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = max(m, cost[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(cw, m);
+}
+
+__global__ __launch_bounds__(256) void k_order_count(const uint32_t* __restrict__ cost, int n,
+                                                     uint32_t* __restrict__ cw) {
+  __shared__ uint32_t s_c[kOrderLevels];
+  if (threadIdx.x < kOrderLevels) s_c[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint64_t mx = max(cw[0], 1u);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) atomicAdd(&s_c[order_level(cost[i], mx)], 1u);
+  __syncthreads();
+  if (threadIdx.x < kOrderLevels && s_c[threadIdx.x]) atomicAdd(&cw[1 + threadIdx.x], s_c[threadIdx.x]);
+}
+
+__global__ void k_order_scan(uint32_t* __restrict__ cw) {
+  if (threadIdx.x == 0) {
+    uint32_t run = 0u;
+    for (int l = 0; l < kOrderLevels; ++l) {
+      cw[1 + kOrderLevels + l] = run;
+      run += cw[1 + l];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kOrderThreads) void k_order_place(const uint32_t* __restrict__ cost, int n,
+                                                               uint32_t* __restrict__ cw, int32_t* __restrict__ order) {
+  constexpr int NW = kOrderThreads / 64;
+  __shared__ uint32_t s_wc[NW][kOrderLevels];
+  __shared__ uint32_t s_base[kOrderLevels];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const uint64_t mx = max(cw[0], 1u);
+  const int i = blockIdx.x * kOrderThreads + t;
+  const bool valid = i < n;
+  const uint32_t lv = valid ? order_level(cost[i], mx) : (uint32_t)kOrderLevels;  // 16: no level
+  uint64_t peers = ~0ull;
+#pragma unroll
+  for (int bit = 0; bit < 5; ++bit) {
+    const bool on = (lv >> bit) & 1u;
+    const uint64_t bal = __ballot(on);
+    peers &= on ? bal : ~bal;
+  }
+  const uint32_t below =
+      __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+  if (lane < kOrderLevels) s_wc[w][lane] = 0u;
+  __syncthreads();
+  if (valid && below == 0) s_wc[w][lv] = (uint32_t)__popcll(peers);
+  __syncthreads();
+  if (t < kOrderLevels) {
+    uint32_t tot = 0u;
+    for (int v = 0; v < NW; ++v) tot += s_wc[v][t];
+    s_base[t] = tot ? atomicAdd(&cw[1 + kOrderLevels + t], tot) : 0u;
+  }
+  __syncthreads();
+  if (valid) {
+    uint32_t pos = s_base[lv] + below;
+    for (int v = 0; v < w; ++v) pos += s_wc[v][lv];
+    order[pos] = i;
+  }
+}
+
 static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
-static size_t order_ws_bytes(int n_tiles) { return 2 * align_up((size_t)(n_tiles > 0 ? n_tiles : 0) * 4); }
+// costs, order, then the counter words of large grids
+static size_t order_ws_bytes(int n_tiles) {
+  return 2 * align_up((size_t)(n_tiles > 0 ? n_tiles : 0) * 4) + align_up(kOrderWords * 4);
+}
 
 // Fills a.tile_order from the saved n_done (workspace: costs, then order).
 static int make_tile_order(RasterArgs& a, void* ws, hipStream_t st) {
+  const int n = a.n_tiles_total;
   uint32_t* cost = reinterpret_cast<uint32_t*>(ws);
-  int32_t* order = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ws) + align_up((size_t)a.n_tiles_total * 4));
+  int32_t* order = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ws) + align_up((size_t)n * 4));
+  uint32_t* cw = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + 2 * align_up((size_t)n * 4));
   a.tile_order = nullptr;
-  hipLaunchKernelGGL(k_tile_cost, dim3(xcd_grid(a.n_tiles_total)), dim3(256), 0, st, a, cost);
-  hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, st, cost, a.n_tiles_total, order);
+  hipLaunchKernelGGL(k_tile_cost, dim3(std::min(xcd_grid(n), kTileCostGrid)), dim3(256), 0, st, a, cost);
+  if (n <= kOrderSingleMax) {
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, st, cost, n, order);
+  } else {
+    const int g = std::min((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_order_zero, dim3(1), dim3(64), 0, st, cw);
+    hipLaunchKernelGGL(k_order_max, dim3(g), dim3(256), 0, st, cost, n, cw);
+    hipLaunchKernelGGL(k_order_count, dim3(g), dim3(256), 0, st, cost, n, cw);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, st, cw);
+    hipLaunchKernelGGL(k_order_place, dim3((n + kOrderThreads - 1) / kOrderThreads), dim3(kOrderThreads), 0, st,
+                       cost, n, cw, order);
+  }
   GSMI_CHECK_LAUNCH();
   a.tile_order = order;
   return GSMI_OK;
@@ -1981,12 +2083,13 @@ extern "C" int gsmi_render_sh_backward(int64_t V, int64_t P, const float* mean, 
 // Each GPU holds its own copy of `scratch` at its own address; the address is
 // looked up once per device (with that device current) and only that address
 // is handed to kernels.  Every call on a GPU shares the array and the order is
-// rebuilt in full by each call (every tile's cost and position is written),
-// so nothing needs clearing; calls that overlap on one GPU (other streams or
-// threads) race on it.
+// rebuilt in full by each call (every tile's cost and position is written,
+// and the counter words of grids beyond 32768 tiles are cleared by a kernel
+// first); calls that overlap on one GPU (other streams or threads) race on
+// it.
 // ---------------------------------------------------------------------------
 constexpr int kNoAllocMaxTiles = 1 << 18;  // 262,144 tiles: 8 views of 3840x2160
-static __device__ __align__(256) uint8_t scratch[2 * kNoAllocMaxTiles * 4];
+static __device__ __align__(256) uint8_t scratch[2 * kNoAllocMaxTiles * 4 + 256];  // = order_ws_bytes(max)
 constexpr int kMaxDevices = 64;
 static void* g_scratch_ptr[kMaxDevices];
 static std::mutex g_scratch_mu;

@@ -318,7 +318,7 @@ int gsmi_render_sh_backward(int64_t V, int64_t P, const float *mean, const float
  * for callers that cannot allocate (code under HIP graph capture, serving
  * loops, harnesses that forbid allocation).  Requires the saved state
  * (T_final and n_done; GSMI_ERR_ARG otherwise) and at most 262,144 tiles
- * (n_tiles_h * n_tiles_w; GSMI_ERR_WORKSPACE beyond).  The array (2 MiB per
+ * (n_tiles_h * n_tiles_w; GSMI_ERR_WORKSPACE beyond).  The array (2 MiB + 256 bytes per
  * GPU, held until the process ends) is shared by every call on a GPU: calls
  * that overlap on one GPU race.
  */

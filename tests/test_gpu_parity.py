@@ -385,12 +385,20 @@ def test_backward_boundary_gradients(orc, which, bg):
     assert rel_err(gsh, rg["sh"]) <= GRAD_RTOL
 
 
-@pytest.mark.parametrize("C", [1, 4])
-def test_backward_launch_order_entry_points_agree(C):
+def _large_grid_scene(C=4, n=60_000, W=3328, H=2720, seed=31):
+    """Tile grid 208 x 170 = 35,360 tiles: more than one workgroup orders
+    (32,768), so the launch order is placed by many workgroups."""
+    sc = small_sh_scene(C, n=n, seed=seed)
+    sc.cameras = [scenes.orbit_camera(4.0, 0.7, 3000.0, 3000.0, W, H, near=0.01, far=100.0, height=0.3)]
+    return sc
+
+
+@pytest.mark.parametrize("C,grid", [(1, "small"), (4, "small"), (4, "large")])
+def test_backward_launch_order_entry_points_agree(C, grid):
     """The backward without a workspace (image-order launch) and with one
     (tiles launched most expensive first) give the same gradients (to the
     float atomics' summation order); a short workspace is refused."""
-    sc = small_sh_scene(C)
+    sc = small_sh_scene(C) if grid == "small" else _large_grid_scene(C)
     c2w, cam = sc.cameras[0]
     cfg = scenes.RasterConfig()
     p = dev_params(sc)
@@ -399,6 +407,7 @@ def test_backward_launch_order_entry_points_agree(C):
     gout = torch.rand(out.shape, generator=torch.Generator().manual_seed(3)).to(DEV) - 0.5
     L = _lib.load()
     ntiles = ic["n_tiles_h"] * ic["n_tiles_w"]
+    assert (ntiles > 32768) == (grid == "large")
     cmax2 = p[4].shape[-1]
 
     def run(ws=None, nws=0):
