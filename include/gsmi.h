@@ -277,10 +277,12 @@ int gsmi_render_sh_forward(int64_t V, int64_t P, const float *mean, const float 
  * error up to 2^-15, biased toward zero) instead of fp32.  Off by default;
  * not fp32 arithmetic. */
 #define GSMI_BWD_SPLIT_BF16 1
-/* flags bit of the SH backward entries (saved state, SH orders 3 and 4, not
- * with GSMI_BWD_SPLIT_BF16): the colours (SH dot products) of the entries
- * are formed on fp32 matrix cores, five list entries at a time, instead of
- * per entry on the vector units; fp32 products, summed in another order. */
+/* flags bit of the SH backward entries (saved state, not with
+ * GSMI_BWD_SPLIT_BF16): for SH orders 3 and 4 the colours (SH dot products)
+ * of the entries are formed on fp32 matrix cores, five list entries at a
+ * time, instead of per entry on the vector units; fp32 products, summed in
+ * another order.  Orders 1 and 2 run on the vector units either way.  Off by
+ * default. */
 #define GSMI_BWD_COLOUR_MFMA 2
 
 /*
