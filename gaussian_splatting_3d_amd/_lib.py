@@ -127,6 +127,7 @@ SIGNATURES = {
     "gsmi_server_stats_ex": (_I32, [_P, _P, _I32]),
     "gsmi_server_destroy": (_I32, [_P]),
     "gsmi_exp_spec": (_I32, [_I64, _P, _P, _P]),
+    "gsmi_exp_spec_nonpos": (_I32, [_I64, _P, _P, _P]),
 }
 
 _lib = None

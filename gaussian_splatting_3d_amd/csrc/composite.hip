@@ -242,7 +242,7 @@ __device__ __forceinline__ float4 wave_pixel_rect(float px, float py) {
 __device__ __forceinline__ float gauss_exact(float q, float det) {
   float radial = q / det;
   if (radial < 0.0f) radial = 1000.0f;
-  return exp_spec(-0.5f * radial);
+  return exp_spec_nonpos(-0.5f * radial);  // radial >= 0 or NaN
 }
 
 // Gaussian value for the backward pass: the hardware exponential of -r/2 with
@@ -310,6 +310,22 @@ __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const floa
   }
   if constexpr (!WITH_SH) return;
   __syncthreads();
+  if constexpr (SHN % 4 == 0) {
+    // Rows of SHN contiguous floats (the stored order is the rendered one):
+    // 16-byte pieces, SHN / 4 per entry.  Block-uniform test.
+    if (a.sh_sc == CC && a.sh_sg == SHN && (reinterpret_cast<uintptr_t>(shb) & 15) == 0) {
+      constexpr int Q = SHN / 4;
+      const float4* sh4 = reinterpret_cast<const float4*>(shb);
+      float4* d4 = reinterpret_cast<float4*>(s_sh);
+      for (int e = t; e < cnt * Q; e += kTilePixels) {
+        const int j = e / Q;
+        const int g = s_id[j];
+        const int64_t row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
+        d4[e] = sh4[row * Q + (e - j * Q)];
+      }
+      return;
+    }
+  }
   for (int e = t; e < cnt * SHN; e += kTilePixels) {
     const int j = e / SHN;
     const int r = e - j * SHN;
@@ -390,7 +406,7 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
   __shared__ GeoB s_gb[kBatch];
   __shared__ float2 s_gc[kBatch];
   __shared__ int s_id[kBatch];
-  __shared__ float s_sh[kBatch * SHN];
+  __shared__ __align__(16) float s_sh[kBatch * SHN];
   __shared__ int s_fq[TWO ? kFilterQueue : 1];
   __shared__ int s_fwh[8];
 
@@ -847,7 +863,7 @@ __global__ __launch_bounds__(256) void k_bwd_sh(RasterArgs a) {
   __shared__ GeoB s_gb[kBatch];
   __shared__ float2 s_gc[kBatch];
   __shared__ int s_id[kBatch];
-  __shared__ float s_sh[kBatch * SHN];
+  __shared__ __align__(16) float s_sh[kBatch * SHN];
   __shared__ float s_acc[kBatch * NCOMP];
   __shared__ int s_tmp[4];
 
@@ -1084,7 +1100,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   __shared__ GeoB s_gb[kMfmaBatch];
   __shared__ float2 s_gc[kMfmaBatch];
   __shared__ int s_id[kMfmaBatch];
-  __shared__ float s_sh[kMfmaBatch * SHN];
+  __shared__ __align__(16) float s_sh[kMfmaBatch * SHN];
   __shared__ float s_acc[kMfmaBatch * 6];                      // scalar components, shared
   __shared__ __align__(16) float s_shacc[4][kMfmaBatch * SHN];  // SH components, per wave
   __shared__ __align__(16) uint32_t s_cc[4][kCcWords];         // per wave: staged coefficient rows

@@ -43,6 +43,27 @@ __device__ __forceinline__ float exp_spec(float x) {
   return ldexpf(p, (int)kf);
 }
 
+// exp_spec for x <= 0 or NaN (a Gaussian's exponent, -r/2 with r >= 0):
+// the same value, without the overflow test and with the underflow and NaN
+// tests as selects instead of branches.  Inputs below -110 (including -inf)
+// are clamped before the reduction so that every intermediate is finite.
+__device__ __forceinline__ float exp_spec_nonpos(float x) {
+  const float xc = fmaxf(x, -120.0f);  // NaN -> -120; the result is replaced below
+  const float kf = rintf(xc * 0x1.715476p+0f);
+  float r = fmaf(kf, -0x1.62ep-1f, xc);
+  r = fmaf(kf, -0x1.0bfbe8p-15f, r);
+  float p = 0x1.a01a02p-13f;
+  p = fmaf(p, r, 0x1.6c16c2p-10f);
+  p = fmaf(p, r, 0x1.111112p-7f);
+  p = fmaf(p, r, 0x1.555556p-5f);
+  p = fmaf(p, r, 0x1.555556p-3f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  const float e = ldexpf(p, (int)kf);
+  return x != x ? x : (x < -110.0f ? 0.0f : e);
+}
+
 // Binary64 exponential, same scheme (32-bit hi part of ln2, degree-13
 // Taylor polynomial).
 __device__ __forceinline__ double exp_spec_d(double x) {

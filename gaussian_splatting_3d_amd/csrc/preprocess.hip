@@ -106,9 +106,12 @@ __global__ void k_split_int2(const Int2* __restrict__ in, int64_t n, int32_t* __
   }
 }
 
+template <bool NONPOS>
 __global__ void k_exp_spec(int64_t n, const float* __restrict__ x, float* __restrict__ y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = exp_spec(x[i]);
+  if (i >= n) return;
+  const float v = x[i];
+  y[i] = NONPOS && !(v > 0.0f) ? exp_spec_nonpos(v) : exp_spec(v);
 }
 
 static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
@@ -163,12 +166,24 @@ extern "C" int gsmi_culling_gaussian_bsphere(int64_t N, const float* mean, const
   return GSMI_OK;
 }
 
-extern "C" int gsmi_exp_spec(int64_t n, const float* x, float* y, void* stream) {
+static int launch_exp_spec(bool nonpos, int64_t n, const float* x, float* y, void* stream) {
   if (n < 0 || (n > 0 && (!x || !y))) return GSMI_ERR_ARG;
   if (n == 0) return GSMI_OK;
-  hipLaunchKernelGGL(k_exp_spec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, x, y);
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (nonpos)
+    hipLaunchKernelGGL(k_exp_spec<true>, grid, dim3(256), 0, (hipStream_t)stream, n, x, y);
+  else
+    hipLaunchKernelGGL(k_exp_spec<false>, grid, dim3(256), 0, (hipStream_t)stream, n, x, y);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
+}
+
+extern "C" int gsmi_exp_spec(int64_t n, const float* x, float* y, void* stream) {
+  return launch_exp_spec(false, n, x, y, stream);
+}
+
+extern "C" int gsmi_exp_spec_nonpos(int64_t n, const float* x, float* y, void* stream) {
+  return launch_exp_spec(true, n, x, y, stream);
 }
 
 // Workspace of gsmi_tile_culling_aabb_count: counts [V] + scan partials + total.

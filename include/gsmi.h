@@ -685,6 +685,11 @@ int gsmi_server_destroy(gsmi_server *s);
  * the tests to check it bit for bit against the oracle. */
 int gsmi_exp_spec(int64_t n, const float *x, float *y, void *stream);
 
+/* The same exponential in the branch-free form the compositing kernels use
+ * for a Gaussian's exponent (x <= 0 or NaN); positive x[i] take the general
+ * form.  Used by the tests to check the two agree bit for bit. */
+int gsmi_exp_spec_nonpos(int64_t n, const float *x, float *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

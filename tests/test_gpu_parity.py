@@ -69,6 +69,24 @@ def test_exp_spec_bit_exact(orc):
     assert same.all(), (x[~same][:5], got[~same][:5], want[~same][:5])
 
 
+def test_exp_spec_nonpos_bit_exact(orc):
+    """The branch-free exponential of a Gaussian's exponent (x <= 0 or NaN)
+    equals the oracle's bit for bit, across the underflow edge and at -0,
+    -inf and NaN."""
+    bits = np.arange(0, 0x7F800000, 131, dtype=np.uint32)
+    edge = np.array([-110.0, -109.99999, -110.00001, -103.97, -87.33, -88.73, -0.0, 0.0, -1e-45, -np.inf, np.nan,
+                     -1000.0, -500.0, -3.4e38], np.float32)
+    x = np.concatenate([(bits | np.uint32(0x80000000)).view(np.float32), edge])
+    xt = torch.from_numpy(x).to(DEV)
+    yt = torch.empty_like(xt)
+    L = _lib.load()
+    _lib.check(L.gsmi_exp_spec_nonpos(xt.numel(), _lib.ptr(xt), _lib.ptr(yt), _lib.stream_of()), "exp_spec_nonpos")
+    got = yt.cpu().numpy()
+    want = orc.expf(x)
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), (x[~same][:5], got[~same][:5], want[~same][:5])
+
+
 @pytest.mark.parametrize("which", ["c1", "small4"])
 def test_cull_bit_exact(orc, which):
     sc = scenes.plain_forward_10k() if which == "c1" else small_sh_scene(4)
