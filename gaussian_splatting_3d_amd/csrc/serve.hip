@@ -45,65 +45,11 @@
 
 #include "../../include/gsmi.h"
 #include "serve_internal.h"
+#include "serve_sched.h"
 
 namespace gsmi_serve {
 
 using Clock = std::chrono::steady_clock;
-
-static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
-
-// First-fit allocator over one device allocation (offsets, 256-byte aligned).
-class Arena {
- public:
-  void init(char* base, size_t size) {
-    base_ = base;
-    size_ = size;
-    free_.clear();
-    if (size) free_[0] = size;
-  }
-  bool alloc(size_t n, size_t* off) {
-    n = align_up(std::max<size_t>(n, 1));
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= n) {
-        *off = it->first;
-        const size_t rest = it->second - n;
-        const size_t at = it->first + n;
-        free_.erase(it);
-        if (rest) free_[at] = rest;
-        return true;
-      }
-    }
-    return false;
-  }
-  void release(size_t off, size_t n) {
-    n = align_up(std::max<size_t>(n, 1));
-    auto it = free_.emplace(off, n).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-  size_t largest() const {
-    size_t m = 0;
-    for (auto& kv : free_) m = std::max(m, kv.second);
-    return m;
-  }
-  size_t size() const { return size_; }
-  char* ptr(size_t off) const { return base_ + off; }
-
- private:
-  char* base_ = nullptr;
-  size_t size_ = 0;
-  std::map<size_t, size_t> free_;
-};
 
 struct Scene {
   int64_t N;
@@ -213,34 +159,18 @@ static void arena_give(gsmi_server* s, size_t off, size_t n) {
 
 static bool same_layout(const Scene& a, const Scene& b) { return a.C == b.C && a.sh_stride_c == b.sh_stride_c; }
 
-// Takes a batch from the queue (caller holds the lock; pending not empty):
-// the oldest request, then in arrival order the requests of scenes with the
-// same SH layout while the view and pixel budgets allow.  Sorted by scene
-// (stable), so each scene's views are contiguous.
+// Takes a batch from the queue (caller holds the lock; pending not empty);
+// the policy is select_batch (serve_sched.h).
 static std::vector<Request> take_batch(gsmi_server* s) {
-  std::vector<Request> batch;
-  const Scene& lead = s->scenes[s->pending.front().scene];
-  int64_t pixels = 0;
-  for (auto it = s->pending.begin(); it != s->pending.end() && (int)batch.size() < s->max_views;) {
-    const bool fits = batch.empty() || pixels + it->pixels <= s->max_batch_pixels;
-    if (fits && same_layout(s->scenes[it->scene], lead)) {
-      pixels += it->pixels;
-      batch.push_back(*it);
-      it = s->pending.erase(it);
-    } else {
-      ++it;
-    }
-  }
-  std::stable_sort(batch.begin(), batch.end(), [](const Request& a, const Request& b) { return a.scene < b.scene; });
-  return batch;
+  return select_batch(
+      s->pending, s->max_views, s->max_batch_pixels, [](const Request& r) { return r.scene; },
+      [](const Request& r) { return r.pixels; },
+      [s](int a, int b) { return same_layout(s->scenes[a], s->scenes[b]); });
 }
 
-// Phase-B bytes for the views of the batch prefix described by V, P,
-// n_blocks and B.
-static size_t phase_b_bytes(int64_t V, int64_t P, int64_t n_blocks, int B) {
-  return align_up((size_t)(2 * (B + 1)) * 8) + align_up((size_t)(2 * B) * sizeof(void*)) + align_up(8 * 4) +
-         align_up(V * 8) + align_up(V * 16) + align_up(V * 4) * 2 + align_up(V * 8) * 4 + align_up(V * 4) +
-         align_up(P * 4) + align_up(n_blocks * 4) * 2 + align_up(gsmi_binning_workspace_size(V, P, (int)n_blocks));
+// Phase-B bytes of a batch prefix with its binning workspace.
+static size_t batch_phase_b_bytes(int64_t V, int64_t P, int64_t n_blocks, int B) {
+  return phase_b_bytes(V, P, n_blocks, B, gsmi_binning_workspace_size(V, P, (int)n_blocks));
 }
 
 // Runs one batch to completion on worker w (returns when its images are
@@ -340,20 +270,11 @@ static void run_batch(gsmi_server* s, Worker& w, std::vector<Request> reqs) {
     std::lock_guard<std::mutex> g(s->mu);
     largest = s->arena.largest();
   }
-  auto need = [&](int k, bool* fits_int) {
-    int64_t V = 0, P = 0, nb = 0;
-    for (int b = 0; b < k; ++b) {
-      V += vV[b];
-      P += vP[b];
-      nb += (int64_t)w.h_views[B0 + b].n_tiles_w * w.h_views[B0 + b].n_tiles_h;
-    }
-    *fits_int = P < (int64_t(1) << 31) && V < (int64_t(1) << 31);
-    return phase_b_bytes(V, P, nb, k);
-  };
-  int B = B0;
+  std::vector<int64_t> vNb(B0);
+  for (int b = 0; b < B0; ++b) vNb[b] = (int64_t)w.h_views[B0 + b].n_tiles_w * w.h_views[B0 + b].n_tiles_h;
   bool fits_int = true;
-  size_t bytesB = need(B, &fits_int);
-  while (B > 1 && !(fits_int && bytesB <= largest)) bytesB = need(--B, &fits_int);
+  size_t bytesB = 0;
+  const int B = admit_prefix(B0, vV.data(), vP.data(), vNb.data(), largest, batch_phase_b_bytes, &fits_int, &bytesB);
   if (B < B0) {
     std::lock_guard<std::mutex> g(s->mu);
     for (int b = B0 - 1; b >= B; --b) s->pending.push_front(reqs[b]);
@@ -713,5 +634,62 @@ extern "C" int gsmi_server_destroy(gsmi_server* s) {
   }
   if (s->arena_mem) (void)hipFree(s->arena_mem);
   delete s;
+  return GSMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host-only test hooks of the scheduling policy (no device is touched).
+// ---------------------------------------------------------------------------
+
+extern "C" int gsmi_sched_arena_run(int64_t size, int n_ops, const int64_t* ops, int64_t* out) {
+  if (size < 0 || n_ops < 0 || (n_ops > 0 && (!ops || !out))) return GSMI_ERR_ARG;
+  Arena a;
+  a.init(nullptr, (size_t)size);
+  for (int i = 0; i < n_ops; ++i) {
+    const int64_t op = ops[3 * i], x = ops[3 * i + 1], y = ops[3 * i + 2];
+    if (op == 0) {
+      size_t off = 0;
+      out[i] = x >= 0 && a.alloc((size_t)x, &off) ? (int64_t)off : -1;
+    } else if (op == 1) {
+      if (x < 0 || y < 0) return GSMI_ERR_ARG;
+      a.release((size_t)x, (size_t)y);
+      out[i] = 0;
+    } else if (op == 2) {
+      out[i] = (int64_t)a.largest();
+    } else {
+      return GSMI_ERR_ARG;
+    }
+  }
+  return GSMI_OK;
+}
+
+extern "C" int gsmi_sched_select_batch(int n, const int32_t* scene, const int64_t* pixels, int n_scenes,
+                                       const int32_t* layout, int max_views, int64_t max_pixels, int32_t* batch,
+                                       int32_t* n_batch, int32_t* rest) {
+  if (n <= 0 || n_scenes <= 0 || max_views <= 0 || !scene || !pixels || !layout || !batch || !n_batch || !rest)
+    return GSMI_ERR_ARG;
+  std::deque<int> q;
+  for (int i = 0; i < n; ++i) {
+    if (scene[i] < 0 || scene[i] >= n_scenes || pixels[i] < 0) return GSMI_ERR_ARG;
+    q.push_back(i);
+  }
+  const std::vector<int> b = select_batch(
+      q, max_views, max_pixels, [&](int r) { return scene[r]; }, [&](int r) { return pixels[r]; },
+      [&](int s0, int s1) { return layout[s0] == layout[s1]; });
+  *n_batch = (int32_t)b.size();
+  for (size_t k = 0; k < b.size(); ++k) batch[k] = b[k];
+  int k = 0;
+  for (int r : q) rest[k++] = r;
+  return GSMI_OK;
+}
+
+extern "C" int gsmi_sched_admit(int B0, const int64_t* vV, const int64_t* vP, const int64_t* n_blocks,
+                                uint64_t largest, int32_t* B, int32_t* fits_int, uint64_t* bytes) {
+  if (B0 <= 0 || !vV || !vP || !n_blocks || !B || !fits_int || !bytes) return GSMI_ERR_ARG;
+  bool fi = true;
+  size_t by = 0;
+  *B = admit_prefix(B0, vV, vP, n_blocks, (size_t)largest, batch_phase_b_bytes, &fi, &by);
+  *fits_int = fi ? 1 : 0;
+  *bytes = (uint64_t)by;
   return GSMI_OK;
 }

@@ -690,6 +690,23 @@ int gsmi_exp_spec(int64_t n, const float *x, float *y, void *stream);
  * form.  Used by the tests to check the two agree bit for bit. */
 int gsmi_exp_spec_nonpos(int64_t n, const float *x, float *y, void *stream);
 
+/* The render server's scheduling policy on the host alone (no device use);
+ * no counterpart in the reference.  gsmi_sched_arena_run: an arena of `size`
+ * bytes driven by n_ops operations (op, x, y) — op 0 allocates x bytes (out:
+ * offset, or -1), op 1 frees (offset x, bytes y), op 2 reads the largest
+ * free range.  gsmi_sched_select_batch: requests 0..n-1 in arrival order
+ * (scene[i], pixels[i]; layout[s] is scene s's SH layout key): the batch the
+ * server takes (request indices, in batch order) and the queue left behind.
+ * gsmi_sched_admit: the prefix of a counted batch (records vV, pairs vP,
+ * blocks n_blocks per view) admitted against `largest` free bytes, whether
+ * its totals fit int32 and its phase-B bytes. */
+int gsmi_sched_arena_run(int64_t size, int n_ops, const int64_t *ops, int64_t *out);
+int gsmi_sched_select_batch(int n, const int32_t *scene, const int64_t *pixels, int n_scenes,
+                            const int32_t *layout, int max_views, int64_t max_pixels, int32_t *batch,
+                            int32_t *n_batch, int32_t *rest);
+int gsmi_sched_admit(int B0, const int64_t *vV, const int64_t *vP, const int64_t *n_blocks, uint64_t largest,
+                     int32_t *B, int32_t *fits_int, uint64_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif
