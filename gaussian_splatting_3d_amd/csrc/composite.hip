@@ -295,11 +295,13 @@ __device__ __forceinline__ float dot_fast(const float* a, const float* b) {
 template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true>
 __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const float* __restrict__ shb,
                                                  const int32_t* __restrict__ idsb, int s0, int base, int cnt,
-                                                 int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc, float* s_sh) {
+                                                 int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc, float* s_sh,
+                                                 int* s_row = nullptr) {
   const int t = threadIdx.x;
   if (t < cnt) {
     const int g = IDS_IN_LDS ? s_id[t] : idsb[s0 + base + t];
     s_id[t] = g;
+    if (s_row) s_row[t] = a.sh_index ? a.sh_index[g] : g;  // the entry's SH row
     const float2 m = reinterpret_cast<const float2*>(a.mean)[g];
     const float4 c = reinterpret_cast<const float4*>(a.cov)[g];
     const float det = c.x * c.w - c.y * c.z;
@@ -1093,7 +1095,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
   constexpr int NCOMP = 6 + SHN;
-  constexpr int KS = (CC + 3) / 4;  // CMF: K-steps of 4 basis functions
+    constexpr int KS = (CC + 3) / 4;  // CMF: K-steps of 4 basis functions
   static_assert(CC <= 16, "SH basis wider than the MFMA tile");
   static_assert(!(CMF && SPLIT), "matrix-core colours with the fp32 reduction only");
   __shared__ GeoA s_ga[kMfmaBatch];
@@ -1107,6 +1109,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   __shared__ int s_tmp[4];
   __shared__ uint32_t s_touch;          // batch entries with a contribution in some wave
   __shared__ int s_tlist[kMfmaBatch];   // their positions, ascending
+  __shared__ int s_row[kMfmaBatch];     // SH row of each batch entry
 
   TileCtx tc;
   if (!tile_setup(a, tc)) return;
@@ -1243,7 +1246,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
     const int cnt = min(kMfmaBatch, nmax - base);
     __syncthreads();
     if (t == 0) s_touch = 0u;
-    stage_batch_from<SHN, CC>(a, a.sh, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+    stage_batch_from<SHN, CC>(a, a.sh, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh, s_row);
     __syncthreads();
     int n_slots = 0;      // filled slots of the current chunk (wave-uniform)
     uint32_t slots = 0;   // batch position of slot m in bits [kSlotBits m, kSlotBits (m + 1))
@@ -1459,16 +1462,18 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
       const int k = e / NCOMP;
       const int c = e - k * NCOMP;
       const int jj = s_tlist[k];
-      float v;
       if (c < 6) {
-        v = s_acc[jj * 6 + c];
+        const float v = s_acc[jj * 6 + c];
         s_acc[jj * 6 + c] = 0.0f;
+        if (v != 0.0f) emit_grad<SHN>(a, s_id[jj], c, v);
       } else {
-        const int o = jj * SHN + c - 6;
-        v = ((s_shacc[0][o] + s_shacc[1][o]) + s_shacc[2][o]) + s_shacc[3][o];
+        const int r = c - 6;
+        const int o = jj * SHN + r;
+        const float v = ((s_shacc[0][o] + s_shacc[1][o]) + s_shacc[2][o]) + s_shacc[3][o];
         s_shacc[0][o] = s_shacc[1][o] = s_shacc[2][o] = s_shacc[3][o] = 0.0f;
+        const int ch = r / CC;
+        if (v != 0.0f) unsafeAtomicAdd(a.grad_sh + (int64_t)s_row[jj] * a.sh_sg + ch * a.sh_sc + (r - ch * CC), v);
       }
-      if (v != 0.0f) emit_grad<SHN>(a, s_id[jj], c, v);
     }
   }
 }
