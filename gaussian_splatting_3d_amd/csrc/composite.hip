@@ -321,8 +321,13 @@ __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const floa
       float4* d4 = reinterpret_cast<float4*>(s_sh);
       for (int e = t; e < cnt * Q; e += kTilePixels) {
         const int j = e / Q;
-        const int g = s_id[j];
-        const int64_t row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
+        int64_t row;
+        if (s_row) {
+          row = s_row[j];  // recorded with the geometry, before the barrier
+        } else {
+          const int g = s_id[j];
+          row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
+        }
         d4[e] = sh4[row * Q + (e - j * Q)];
       }
       return;
@@ -409,6 +414,7 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
   __shared__ float2 s_gc[kBatch];
   __shared__ int s_id[kBatch];
   __shared__ __align__(16) float s_sh[kBatch * SHN];
+  __shared__ int s_row[kBatch];  // SH row of each batch entry
   __shared__ int s_fq[TWO ? kFilterQueue : 1];
   __shared__ int s_fwh[8];
 
@@ -488,11 +494,11 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
       cnt = filter_batch(a, e0, tx, ty, s_id, s_fq, s_fwh, fcur, fqh, fqn, fpar);
       if (cnt == 0) break;
       if (fidsb && (int)threadIdx.x < cnt) fidsb[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
-      stage_batch_from<SHN, CC, true>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+      stage_batch_from<SHN, CC, true>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh, s_row);
     } else {
       if (base >= n) break;
       cnt = min(kBatch, n - base);
-      stage_batch_from<SHN, CC>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
+      stage_batch_from<SHN, CC>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh, s_row);
     }
     __syncthreads();
     // entries of the batch that can reach a pixel of this wave, in order
