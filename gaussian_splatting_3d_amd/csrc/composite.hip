@@ -291,8 +291,10 @@ __device__ __forceinline__ float dot_fast(const float* a, const float* b) {
 }
 
 // Stage entries [s0+base, s0+base+cnt) of the tile list into LDS (without
-// WITH_SH only the geometry and opacity).
-template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true>
+// WITH_SH only the geometry and opacity).  NEG_LOG2E: the SH coefficients are
+// staged times -log2(e), so that a colour's dot product is the argument of
+// sigmoid_scaled (fast colours only).
+template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true, bool NEG_LOG2E = false>
 __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const float* __restrict__ shb,
                                                  const int32_t* __restrict__ idsb, int s0, int base, int cnt,
                                                  int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc, float* s_sh,
@@ -328,7 +330,14 @@ __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const floa
           const int g = s_id[j];
           row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
         }
-        d4[e] = sh4[row * Q + (e - j * Q)];
+        float4 v = sh4[row * Q + (e - j * Q)];
+        if constexpr (NEG_LOG2E) {
+          v.x *= kNegLog2e;
+          v.y *= kNegLog2e;
+          v.z *= kNegLog2e;
+          v.w *= kNegLog2e;
+        }
+        d4[e] = v;
       }
       return;
     }
@@ -340,7 +349,8 @@ __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const floa
     const int k = r - ch * CC;
     const int g = s_id[j];
     const int64_t row = a.sh_index ? (int64_t)a.sh_index[g] : (int64_t)g;
-    s_sh[e] = shb[row * a.sh_sg + ch * a.sh_sc + k];
+    const float v = shb[row * a.sh_sg + ch * a.sh_sc + k];
+    s_sh[e] = NEG_LOG2E ? v * kNegLog2e : v;
   }
 }
 
@@ -494,11 +504,13 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
       cnt = filter_batch(a, e0, tx, ty, s_id, s_fq, s_fwh, fcur, fqh, fqn, fpar);
       if (cnt == 0) break;
       if (fidsb && (int)threadIdx.x < cnt) fidsb[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
-      stage_batch_from<SHN, CC, true>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh, s_row);
+      stage_batch_from<SHN, CC, true, true, !EXACT>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh,
+                                                     s_row);
     } else {
       if (base >= n) break;
       cnt = min(kBatch, n - base);
-      stage_batch_from<SHN, CC>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh, s_row);
+      stage_batch_from<SHN, CC, false, true, !EXACT>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh,
+                                                      s_row);
     }
     __syncthreads();
     // entries of the batch that can reach a pixel of this wave, in order
@@ -530,9 +542,9 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
         y1 = sigmoid_exact(dot_exact<CC>(shj + CC, Y));
         y2 = sigmoid_exact(dot_exact<CC>(shj + 2 * CC, Y));
       } else {
-        y0 = sigmoid_fast(dot_fast<CC>(shj, Y));
-        y1 = sigmoid_fast(dot_fast<CC>(shj + CC, Y));
-        y2 = sigmoid_fast(dot_fast<CC>(shj + 2 * CC, Y));
+        y0 = sigmoid_scaled(dot_fast<CC>(shj, Y));  // coefficients staged times -log2(e)
+        y1 = sigmoid_scaled(dot_fast<CC>(shj + CC, Y));
+        y2 = sigmoid_scaled(dot_fast<CC>(shj + 2 * CC, Y));
       }
       if constexpr (EXACT) {
         if (isnan(y0 * coeff)) y0 = 0.0f;
@@ -1252,7 +1264,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
     const int cnt = min(kMfmaBatch, nmax - base);
     __syncthreads();
     if (t == 0) s_touch = 0u;
-    stage_batch_from<SHN, CC>(a, a.sh, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh, s_row);
+    stage_batch_from<SHN, CC, false, true, true>(a, a.sh, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh, s_row);
     __syncthreads();
     int n_slots = 0;      // filled slots of the current chunk (wave-uniform)
     uint32_t slots = 0;   // batch position of slot m in bits [kSlotBits m, kSlotBits (m + 1))
@@ -1355,9 +1367,10 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
             d2 = dot_fast<CC>(shj + 2 * CC, Y);
           }
           // w is finite, a sigmoid NaN or in [0, 1]: isnan(y * w) == isnan(y)
-          const float y0 = fmaxf(sigmoid_fast(d0), 0.0f);
-          const float y1 = fmaxf(sigmoid_fast(d1), 0.0f);
-          const float y2 = fmaxf(sigmoid_fast(d2), 0.0f);
+          // the staged coefficients carry -log2(e): d is the exponent's argument
+          const float y0 = fmaxf(sigmoid_scaled(d0), 0.0f);
+          const float y1 = fmaxf(sigmoid_scaled(d1), 0.0f);
+          const float y2 = fmaxf(sigmoid_scaled(d2), 0.0f);
           const float cc0 = contrib ? w * fmaf(-y0, y0, y0) * g0 : 0.0f;  // y (1 - y): the sigmoid's slope
           const float cc1 = contrib ? w * fmaf(-y1, y1, y1) * g1 : 0.0f;
           const float cc2 = contrib ? w * fmaf(-y2, y2, y2) * g2 : 0.0f;

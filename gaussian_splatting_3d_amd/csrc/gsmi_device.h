@@ -100,6 +100,13 @@ __device__ __forceinline__ float sigmoid_fast(float x) {
   return __builtin_amdgcn_rcpf(1.0f + e);
 }
 
+// The same sigmoid of x from x' = -log2(e) x (the SH coefficients staged
+// times -log2(e) give x' directly as the colour's dot product).
+constexpr float kNegLog2e = -1.44269504088896341f;
+__device__ __forceinline__ float sigmoid_scaled(float xs) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(xs));
+}
+
 // float -> int32 with truncation toward zero, saturation and NaN -> 0.
 __device__ __forceinline__ int32_t f2i_sat(float v) {
   if (v != v) return 0;
