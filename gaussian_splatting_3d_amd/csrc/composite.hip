@@ -1,7 +1,8 @@
 // Per-tile alpha compositing on gfx950: forward and reverse-order adjoint.
 //
-// One 256-thread workgroup (4 wave64) per 16x16 tile; wave w owns pixel rows
-// 4w..4w+3.  Gaussians of the tile's sorted list are staged in LDS in
+// One 256-thread workgroup (4 wave64) per 16x16 tile; wave w owns the 8x8
+// quadrant (w & 1, w >> 1) of the tile (squarer than rows of 16, so fewer
+// Gaussians reach a wave's pixels).  Gaussians of the tile's sorted list are staged in LDS in
 // batches; every wave walks the entries of the batch that can reach one of
 // its pixels (a relevance mask built once per batch, one entry per lane) in
 // lockstep (wave-uniform loop) and leaves the batch as soon as a ballot
@@ -160,8 +161,9 @@ __device__ __forceinline__ bool tile_setup_at(const RasterArgs& a, TileCtx& c, i
   const int tx = local % ntw, ty = local / ntw;
   const int t = threadIdx.x;
   c.tile = tile;
-  c.gx = tx * kTile + (t & 15);
-  c.gy = ty * kTile + (t >> 4);
+  const int w = t >> 6, l = t & 63;  // wave w: 8x8 quadrant, lane l: row l >> 3, column l & 7
+  c.gx = tx * kTile + ((w & 1) << 3) + (l & 7);
+  c.gy = ty * kTile + ((w >> 1) << 3) + (l >> 3);
   c.inside = c.gx < W && c.gy < H;
   c.lpix = (int64_t)c.gy * W + c.gx;
   c.pix = pix_base + c.lpix;
@@ -223,17 +225,17 @@ __device__ __forceinline__ bool may_reach(const GeoA& ga, const GeoB& gb, float 
   return !(well && m * gb.inv_det > thr);
 }
 
-// This wave's pixel rectangle: positions are monotone in the pixel index, so
-// lanes 0, 15 (first row) and 48 (last row) hold the extremes.  The values
-// are wave-uniform and kept in scalar registers.
+// This wave's pixel rectangle: positions are monotone in the pixel row and
+// column, so lanes 0, 7 (first row) and 56 (last row) hold the extremes.  The
+// values are wave-uniform and kept in scalar registers.
 __device__ __forceinline__ float uniform_f(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
 __device__ __forceinline__ float4 wave_pixel_rect(float px, float py) {
   const float xa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px), 0));
-  const float xb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px), 15));
+  const float xb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px), 7));
   const float ya = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), 0));
-  const float yb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), 48));
+  const float yb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(py), 56));
   return make_float4(uniform_f(fminf(xa, xb)), uniform_f(fmaxf(xa, xb)), uniform_f(fminf(ya, yb)),
                      uniform_f(fmaxf(ya, yb)));
 }
