@@ -84,32 +84,62 @@ struct AdamGroups {
   int count;
 };
 
-// w1 = 1 - b1, w2 = 1 - b2 rounded from double, as torch passes them.
+// One element in torch's operation order.
+__device__ __forceinline__ void adam_one(float& p, float gr, float& m, float& v, float w1, float b2, float w2,
+                                         float eps, float ss, float bs) {
+  const float m1 = m + w1 * (gr - m);
+  const float v1 = v * b2 + w2 * gr * gr;
+  const float denom = sqrtf(v1) / bs + eps;
+  p = p + (-ss) * (m1 / denom);
+  m = m1;
+  v = v1;
+}
+
+// w1 = 1 - b1, w2 = 1 - b2 rounded from double, as torch passes them.  Each
+// thread takes kAdamItems consecutive elements, as 16-byte vectors when they
+// are all in range and every array of the group is 16-byte aligned.
 __global__ __launch_bounds__(kAdamBlock) void k_adam(AdamGroups G, float w1, float b2, float w2, float eps,
                                                     int fresh) {
+  static_assert(kAdamItems == 4, "one float4 per thread");
   int gi = 0;
   while (gi + 1 < G.count && (int64_t)blockIdx.x >= G.first_block[gi + 1]) ++gi;
-  const int64_t base = ((int64_t)blockIdx.x - G.first_block[gi]) * kAdamChunk;
+  const int64_t i0 = ((int64_t)blockIdx.x - G.first_block[gi]) * kAdamChunk + (int64_t)kAdamItems * threadIdx.x;
+  const int64_t n = G.n[gi];
+  if (i0 >= n) return;
   float* __restrict__ p = G.p[gi];
   const float* __restrict__ g = G.g[gi];
   float* __restrict__ m = G.m[gi];
   float* __restrict__ v = G.v[gi];
-  const int64_t n = G.n[gi];
   const float ss = G.step_size[gi], bs = G.bc2_sqrt[gi];
-#pragma unroll
-  for (int k = 0; k < kAdamItems; ++k) {
-    const int64_t i = base + (int64_t)k * kAdamBlock + threadIdx.x;
-    if (i >= n) break;
-    const float gr = g[i];
-    const float m0 = fresh ? 0.0f : m[i];
-    const float v0 = fresh ? 0.0f : v[i];
-    const float m1 = m0 + w1 * (gr - m0);
-    const float v1 = v0 * b2 + w2 * gr * gr;
-    const float denom = sqrtf(v1) / bs + eps;
-    p[i] = p[i] + (-ss) * (m1 / denom);
+  uintptr_t al = reinterpret_cast<uintptr_t>(p) | reinterpret_cast<uintptr_t>(g);
+  if (!fresh) al |= reinterpret_cast<uintptr_t>(m) | reinterpret_cast<uintptr_t>(v);
+  if (i0 + kAdamItems <= n && (al & 15) == 0) {
+    const float4 gr = *reinterpret_cast<const float4*>(g + i0);
+    float4 pp = *reinterpret_cast<const float4*>(p + i0);
+    float4 mm = make_float4(0.f, 0.f, 0.f, 0.f), vv = make_float4(0.f, 0.f, 0.f, 0.f);
     if (!fresh) {
-      m[i] = m1;
-      v[i] = v1;
+      mm = *reinterpret_cast<const float4*>(m + i0);
+      vv = *reinterpret_cast<const float4*>(v + i0);
+    }
+    adam_one(pp.x, gr.x, mm.x, vv.x, w1, b2, w2, eps, ss, bs);
+    adam_one(pp.y, gr.y, mm.y, vv.y, w1, b2, w2, eps, ss, bs);
+    adam_one(pp.z, gr.z, mm.z, vv.z, w1, b2, w2, eps, ss, bs);
+    adam_one(pp.w, gr.w, mm.w, vv.w, w1, b2, w2, eps, ss, bs);
+    *reinterpret_cast<float4*>(p + i0) = pp;
+    if (!fresh) {
+      *reinterpret_cast<float4*>(m + i0) = mm;
+      *reinterpret_cast<float4*>(v + i0) = vv;
+    }
+    return;
+  }
+  for (int k = 0; k < kAdamItems && i0 + k < n; ++k) {
+    const int64_t i = i0 + k;
+    float pp = p[i], mm = fresh ? 0.0f : m[i], vv = fresh ? 0.0f : v[i];
+    adam_one(pp, g[i], mm, vv, w1, b2, w2, eps, ss, bs);
+    p[i] = pp;
+    if (!fresh) {
+      m[i] = mm;
+      v[i] = vv;
     }
   }
 }

@@ -52,6 +52,35 @@ def test_fused_adam_matches_torch_adam_over_steps():
             assert _close(a, b.detach())
 
 
+@pytest.mark.parametrize("fresh", [False, True])
+def test_fused_adam_ragged_and_unaligned_groups(fresh):
+    """Groups whose length is not a multiple of four (the last elements take
+    the scalar path) and a group starting 4 bytes into its storage (no
+    16-byte vectors at all) match torch's Adam like the aligned ones."""
+    g = torch.Generator().manual_seed(5)
+    base = torch.randn(10_003, generator=g).to(DEV)
+    ps = [torch.randn(4_999, 3, generator=g).to(DEV), base[1:], torch.randn(7, generator=g).to(DEV)]
+    lrs = [1e-3, 5e-3, 2e-2]
+    ref = [p.clone().requires_grad_(True) for p in ps]
+    mine = [p if k == 1 else p.clone() for k, p in enumerate(ps)]  # the offset view is updated in place
+    assert mine[1].data_ptr() % 16 == 4
+    fa = training.FusedAdam(list(zip(mine, lrs)), fresh=fresh)
+    opt = torch.optim.Adam([{"params": [p], "lr": lr} for p, lr in zip(ref, lrs)], betas=(0.9, 0.99), foreach=False)
+    gen = torch.Generator().manual_seed(13)
+    for _ in range(3):
+        grads = [torch.randn(p.shape, generator=gen).to(DEV) * 0.1 for p in ps]
+        if fresh:
+            opt = torch.optim.Adam([{"params": [p], "lr": lr} for p, lr in zip(ref, lrs)], betas=(0.9, 0.99),
+                                   foreach=False)
+        for p, gr in zip(ref, grads):
+            p.grad = gr.clone()
+        opt.step()
+        fa.step(grads)
+        for a_, b in zip(mine, ref):
+            assert _close(a_, b.detach())
+    assert float(base[0]) == float(torch.randn(10_003, generator=torch.Generator().manual_seed(5))[0])  # untouched
+
+
 def test_fused_adam_fresh_matches_rebuilt_optimizer():
     """The reference builds a new Adam every iteration (main_sh.py:238)."""
     ps, lrs = _groups(2)
