@@ -1726,25 +1726,40 @@ __global__ __launch_bounds__(256) void k_bwd_rgb(RasterArgs a) {
 // Gaussians) still run close together.
 // ---------------------------------------------------------------------------
 // Cost of each tile: the sum over its four waves of the most entries a pixel
-// of the wave examined, plus a fixed 32 (0 for an empty list).  Workgroups
-// take tiles in a grid-stride loop (a launch of one small workgroup per tile
-// is bound by workgroup dispatch on grids of 10^5 tiles).
-constexpr int kTileCostGrid = 2048;
+// of the wave examined (its largest n_done), plus a fixed 32 (0 for an empty
+// list).  One thread per 8x8 quadrant (the pixels of one compositing wave);
+// the four quadrants of a tile are consecutive lanes.
 __global__ __launch_bounds__(256) void k_tile_cost(RasterArgs a, uint32_t* __restrict__ cost) {
-  __shared__ int s_w[2][4];
-  TileCtx tc;
-  for (int tile = blockIdx.x, par = 0; tile < a.n_tiles_total; tile += gridDim.x, par ^= 1) {
-    tile_setup_at(a, tc, tile);
-    int nd = tc.inside ? a.n_done_saved[tc.pix] : 0;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) nd = max(nd, __shfl_xor(nd, o));
-    if ((threadIdx.x & 63) == 0) s_w[par][threadIdx.x >> 6] = nd;  // alternate slots: one barrier per tile
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int s0 = a.start[tile], e0 = a.end[tile];
-      const bool empty = s0 == -1 || e0 <= s0;
-      cost[tile] = empty ? 0u : (uint32_t)(s_w[par][0] + s_w[par][1] + s_w[par][2] + s_w[par][3]) + 32u;
+  const int64_t qd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int tile = (int)(qd >> 2);
+  const bool valid = tile < a.n_tiles_total;
+  int mx = 0;
+  if (valid) {
+    int local = tile, ntw = a.n_tiles_w, W = a.W, H = a.H;
+    int64_t pix_base = 0;
+    if (a.views) {
+      const gsmi_view& v = a.views[view_of_tile(a.views, a.n_views, tile)];
+      local = tile - v.tile_base;
+      ntw = v.n_tiles_w;
+      W = v.W;
+      H = v.H;
+      pix_base = v.pix_base;
     }
+    const int w = (int)(qd & 3);
+    const int gx0 = (local % ntw) * kTile + ((w & 1) << 3), gy0 = (local / ntw) * kTile + ((w >> 1) << 3);
+    for (int r = 0; r < 8 && gy0 + r < H; ++r) {
+      const int32_t* row = a.n_done_saved + pix_base + (int64_t)(gy0 + r) * W;
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (gx0 + c < W) mx = max(mx, row[gx0 + c]);
+    }
+  }
+  int sum = mx + __shfl_xor(mx, 1);
+  sum += __shfl_xor(sum, 2);
+  if (valid && (qd & 3) == 0) {
+    const int s0 = a.start[tile], e0 = a.end[tile];
+    const bool empty = s0 == -1 || e0 <= s0;
+    cost[tile] = empty ? 0u : (uint32_t)sum + 32u;
   }
 }
 
@@ -1909,7 +1924,7 @@ static int make_tile_order(RasterArgs& a, void* ws, hipStream_t st) {
   int32_t* order = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ws) + align_up((size_t)n * 4));
   uint32_t* cw = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + 2 * align_up((size_t)n * 4));
   a.tile_order = nullptr;
-  hipLaunchKernelGGL(k_tile_cost, dim3(std::min(xcd_grid(n), kTileCostGrid)), dim3(256), 0, st, a, cost);
+  hipLaunchKernelGGL(k_tile_cost, dim3((unsigned)((4 * (int64_t)n + 255) / 256)), dim3(256), 0, st, a, cost);
   if (n <= kOrderSingleMax) {
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, st, cost, n, order);
   } else {
