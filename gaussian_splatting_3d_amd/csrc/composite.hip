@@ -292,15 +292,12 @@ __device__ __forceinline__ float dot_fast(const float* a, const float* b) {
   return s;
 }
 
-// Stage entries [s0+base, s0+base+cnt) of the tile list into LDS (without
-// WITH_SH only the geometry and opacity).  NEG_LOG2E: the SH coefficients are
-// staged times -log2(e), so that a colour's dot product is the argument of
-// sigmoid_scaled (fast colours only).
-template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true, bool NEG_LOG2E = false>
-__device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const float* __restrict__ shb,
-                                                 const int32_t* __restrict__ idsb, int s0, int base, int cnt,
-                                                 int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc, float* s_sh,
-                                                 int* s_row = nullptr) {
+// First half of stage_batch_from: ids, SH row indices (when s_row is given)
+// and geometry of the batch's entries.
+template <bool IDS_IN_LDS = false>
+__device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_t* __restrict__ idsb, int s0,
+                                               int base, int cnt, int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc,
+                                               int* s_row) {
   const int t = threadIdx.x;
   if (t < cnt) {
     const int g = IDS_IN_LDS ? s_id[t] : idsb[s0 + base + t];
@@ -314,8 +311,16 @@ __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const floa
     s_gb[t] = GeoB{c.z, c.w, det, 1.0f / det};
     s_gc[t] = make_float2(al, reject_radius(al));
   }
-  if constexpr (!WITH_SH) return;
-  __syncthreads();
+}
+
+// Second half: the SH rows of the staged entries, once a barrier has
+// published stage_geometry's s_id / s_row.  NEG_LOG2E: the coefficients are
+// staged times -log2(e), so that a colour's dot product is the argument of
+// sigmoid_scaled (fast colours only).
+template <int SHN, int CC, bool NEG_LOG2E = false>
+__device__ __forceinline__ void stage_sh_rows(const RasterArgs& a, const float* __restrict__ shb, int cnt,
+                                              const int* s_id, float* s_sh, const int* s_row) {
+  const int t = threadIdx.x;
   if constexpr (SHN % 4 == 0) {
     // Rows of SHN contiguous floats (the stored order is the rendered one):
     // 16-byte pieces, SHN / 4 per entry.  Block-uniform test.
@@ -354,6 +359,21 @@ __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const floa
     const float v = shb[row * a.sh_sg + ch * a.sh_sc + k];
     s_sh[e] = NEG_LOG2E ? v * kNegLog2e : v;
   }
+}
+
+// Stage entries [s0+base, s0+base+cnt) of the tile list into LDS (without
+// WITH_SH only the geometry and opacity).  NEG_LOG2E: the SH coefficients are
+// staged times -log2(e), so that a colour's dot product is the argument of
+// sigmoid_scaled (fast colours only).
+template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true, bool NEG_LOG2E = false>
+__device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const float* __restrict__ shb,
+                                                 const int32_t* __restrict__ idsb, int s0, int base, int cnt,
+                                                 int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc, float* s_sh,
+                                                 int* s_row = nullptr) {
+  stage_geometry<IDS_IN_LDS>(a, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_row);
+  if constexpr (!WITH_SH) return;
+  __syncthreads();
+  stage_sh_rows<SHN, CC, NEG_LOG2E>(a, shb, cnt, s_id, s_sh, s_row);
 }
 
 template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true>
@@ -1128,7 +1148,6 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   __shared__ __align__(16) uint32_t s_cc[4][kCcWords];         // per wave: staged coefficient rows
   __shared__ int s_tmp[4];
   __shared__ uint32_t s_touch;          // batch entries with a contribution in some wave
-  __shared__ int s_tlist[kMfmaBatch];   // their positions, ascending
   __shared__ int s_row[kMfmaBatch];     // SH row of each batch entry
 
   TileCtx tc;
@@ -1262,11 +1281,61 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   for (int e = t; e < kMfmaBatch * 6; e += kTilePixels) s_acc[e] = 0.0f;
   for (int e = t; e < 4 * kMfmaBatch * SHN / 4; e += kTilePixels)
     reinterpret_cast<float4*>(&s_shacc[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int base = ((nmax - 1) / kMfmaBatch) * kMfmaBatch; nmax > 0 && base >= 0; base -= kMfmaBatch) {
+  // Flush of a batch: thread t takes elements e = t + i kTilePixels of the
+  // (entry, component) grid, entry e / NCOMP, and reads, zeroes and emits
+  // those of touched entries.  The ids and SH rows of its entries are kept
+  // in registers (keep_rows, at the end of the batch's compositing), so the
+  // flush of batch n - 1 can run while batch n is staged over s_id / s_row.
+  // The thread index passes through an empty asm in both, so that their
+  // index arithmetic, the same in every batch, is not hoisted out of the
+  // batch loop into registers the compositing needs.
+  constexpr int kFlushElems = (kMfmaBatch * NCOMP + kTilePixels - 1) / kTilePixels;
+  int f_id[kFlushElems], f_row[kFlushElems];
+  auto keep_rows = [&]() {
+    int tv = t;
+    asm volatile("" : "+v"(tv));
+#pragma unroll
+    for (int i = 0; i < kFlushElems; ++i) {
+      const int k = min((tv + i * kTilePixels) / NCOMP, kMfmaBatch - 1);
+      f_id[i] = s_id[k];
+      f_row[i] = s_row[k];
+    }
+  };
+  auto flush = [&](uint32_t touch) {
+    int tv = t;
+    asm volatile("" : "+v"(tv));
+#pragma unroll
+    for (int i = 0; i < kFlushElems; ++i) {
+      const int e = tv + i * kTilePixels;
+      const int k = e / NCOMP;
+      if (e >= kMfmaBatch * NCOMP || !((touch >> k) & 1u)) continue;
+      const int c = e - k * NCOMP;
+      if (c < 6) {
+        const float v = s_acc[k * 6 + c];
+        s_acc[k * 6 + c] = 0.0f;
+        if (v != 0.0f) emit_grad<SHN>(a, f_id[i], c, v);
+      } else {
+        const int r = c - 6;
+        const int o = k * SHN + r;
+        const float v = ((s_shacc[0][o] + s_shacc[1][o]) + s_shacc[2][o]) + s_shacc[3][o];
+        s_shacc[0][o] = s_shacc[1][o] = s_shacc[2][o] = s_shacc[3][o] = 0.0f;
+        const int ch = r / CC;
+        if (v != 0.0f) unsafeAtomicAdd(a.grad_sh + (int64_t)f_row[i] * a.sh_sg + ch * a.sh_sc + (r - ch * CC), v);
+      }
+    }
+  };
+  // Batch n, three barriers: stage its geometry | stage its SH rows, flush
+  // batch n - 1 (zeroing the accumulators before batch n writes them) |
+  // composite batch n, keep its ids and rows for its flush.
+  int nb = 0;
+  for (int base = ((nmax - 1) / kMfmaBatch) * kMfmaBatch; nmax > 0 && base >= 0; base -= kMfmaBatch, ++nb) {
     const int cnt = min(kMfmaBatch, nmax - base);
+    const uint32_t ptouch = nb > 0 ? s_touch : 0u;  // batch n - 1's, complete since the last barrier
+    stage_geometry<false>(a, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_row);
     __syncthreads();
-    if (t == 0) s_touch = 0u;
-    stage_batch_from<SHN, CC, false, true, true>(a, a.sh, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh, s_row);
+    if (t == 0) s_touch = 0u;  // every thread has read it before the barrier
+    stage_sh_rows<SHN, CC, true>(a, a.sh, cnt, s_id, s_sh, s_row);
+    flush(ptouch);
     __syncthreads();
     int n_slots = 0;      // filled slots of the current chunk (wave-uniform)
     uint32_t slots = 0;   // batch position of slot m in bits [kSlotBits m, kSlotBits (m + 1))
@@ -1354,8 +1423,9 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           const float om = 1.0f - aG;
           const float inv_om = __builtin_amdgcn_rcpf(om);
           const float Tj = T * inv_om;
+          // the weight, zero where the entry does not contribute (or is NaN)
           float w = al * Tj * G;
-          w = isnan(w) ? 0.0f : w;
+          w = (contrib && !isnan(w)) ? w : 0.0f;
           const int m = CMF ? gi : n_slots;
           float d0, d1, d2;
           if constexpr (CMF) {  // this entry's dot products, from the chunk's matrix-core product
@@ -1379,18 +1449,22 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           const float gy = fmaf(g2, y2, fmaf(g1, y1, g0 * y0));
           // sum_c g_c (y_c T_j - S_c / (1 - aG))
           const float P = fmaf(gy, Tj, -(gS * inv_om));
-          gS = contrib ? fmaf(w, gy, gS) : gS;
+          // w = 0 leaves gS unchanged where gy is finite; where it is not, so
+          // is P of every later contributing entry of the pixel
+          gS = fmaf(w, gy, gS);
           T = contrib ? Tj : T;
-          const float D = P * aG;
-          const float sx = tmpx * gb.inv_det;
-          const float uy = tmpy * gb.inv_det;
+          // zero factors, selected before the products, make the mean and
+          // covariance terms exact zeros where the entry does not contribute
+          const float D = contrib ? P * aG : 0.0f;
+          const float sx = contrib ? tmpx * gb.inv_det : 0.0f;
+          const float uy = contrib ? tmpy * gb.inv_det : 0.0f;
           const float Ds = D * sx, Du = D * uy;
           float v[8];
-          v[0] = contrib ? Ds : 0.0f;
-          v[1] = contrib ? Du : 0.0f;
-          v[2] = contrib ? Ds * sx : 0.0f;  // x 1/2 after the reduction (comp_scale)
-          v[3] = contrib ? Ds * uy : 0.0f;
-          v[4] = contrib ? Du * uy : 0.0f;
+          v[0] = Ds;
+          v[1] = Du;
+          v[2] = Ds * sx;  // x 1/2 after the reduction (comp_scale)
+          v[3] = Ds * uy;
+          v[4] = Du * uy;
           v[5] = contrib ? P * G : 0.0f;
           v[6] = 0.0f;
           v[7] = 0.0f;
@@ -1448,6 +1522,10 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           for (int k = 0; k < 16; ++k) av[k] = a_col[k * kCcRows];
 #pragma unroll
           for (int k = 0; k < 16; ++k) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[k], bY[k], acc, 0, 0, 0);
+          // all operand reads in flight before the first product (one LDS
+          // round trip per chunk instead of one per pair of products)
+          __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
         }
         if (d_k < CC) {
 #pragma unroll
@@ -1474,29 +1552,10 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
       }
     }
     if (lane == 0 && wtouch) atomicOr(&s_touch, wtouch);
+    keep_rows();
     __syncthreads();
-    const uint32_t touch = s_touch;
-    if (t < kMfmaBatch && ((touch >> t) & 1u)) s_tlist[__popc(touch & ((1u << t) - 1u))] = t;
-    __syncthreads();
-    // one thread per (touched entry, component): read, zero, emit
-    for (int e = t; e < __popc(touch) * NCOMP; e += kTilePixels) {
-      const int k = e / NCOMP;
-      const int c = e - k * NCOMP;
-      const int jj = s_tlist[k];
-      if (c < 6) {
-        const float v = s_acc[jj * 6 + c];
-        s_acc[jj * 6 + c] = 0.0f;
-        if (v != 0.0f) emit_grad<SHN>(a, s_id[jj], c, v);
-      } else {
-        const int r = c - 6;
-        const int o = jj * SHN + r;
-        const float v = ((s_shacc[0][o] + s_shacc[1][o]) + s_shacc[2][o]) + s_shacc[3][o];
-        s_shacc[0][o] = s_shacc[1][o] = s_shacc[2][o] = s_shacc[3][o] = 0.0f;
-        const int ch = r / CC;
-        if (v != 0.0f) unsafeAtomicAdd(a.grad_sh + (int64_t)s_row[jj] * a.sh_sg + ch * a.sh_sc + (r - ch * CC), v);
-      }
-    }
   }
+  if (nb > 0) flush(s_touch);  // the last batch
 }
 
 template <int C, bool BG, bool SPLIT, bool RECOMP>
