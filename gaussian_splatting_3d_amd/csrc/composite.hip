@@ -247,6 +247,28 @@ __device__ __forceinline__ float gauss_exact(float q, float det) {
   return exp_spec_nonpos(-0.5f * radial);  // radial >= 0 or NaN
 }
 
+// The quotient q / det of gauss_exact from the staged reciprocal y =
+// staged_recip(det) and q0 = q * y: div_staged, or the IEEE division where
+// that gives NaN or |q| < 2^-100 (wave-uniform branch; the empty asm keeps
+// the compiler from executing the division unconditionally and selecting).
+// Bit for bit the IEEE quotient.
+__device__ __forceinline__ float radial_staged(float q, float det, float y, float q0) {
+  float radial = div_staged(q, det, y, q0);
+  const bool ieee = radial != radial || fabsf(q) < 0x1p-100f;
+  if (wave_any(ieee)) {
+    asm volatile("");
+    if (ieee) radial = q / det;
+  }
+  return radial;
+}
+
+// gauss_exact from the staged reciprocal: the same value.
+__device__ __forceinline__ float gauss_staged(float q, float det, float y, float q0) {
+  float radial = radial_staged(q, det, y, q0);
+  if (radial < 0.0f) radial = 1000.0f;
+  return exp_spec_nonpos(-0.5f * radial);
+}
+
 // Gaussian value for the backward pass: the hardware exponential of -r/2 with
 // r = q/det from the staged reciprocal (1000 for r < 0, as the reference).
 // Values feed gradients only; decisions use gauss_needs_exact.
@@ -293,8 +315,10 @@ __device__ __forceinline__ float dot_fast(const float* a, const float* b) {
 }
 
 // First half of stage_batch_from: ids, SH row indices (when s_row is given)
-// and geometry of the batch's entries.
-template <bool IDS_IN_LDS = false>
+// and geometry of the batch's entries.  STAGED_RECIP: 1/det as staged_recip
+// (for gauss_staged; NaN outside its range, which the relevance and distance
+// tests read as "may reach").
+template <bool IDS_IN_LDS = false, bool STAGED_RECIP = false>
 __device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_t* __restrict__ idsb, int s0,
                                                int base, int cnt, int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc,
                                                int* s_row) {
@@ -308,7 +332,7 @@ __device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_
     const float det = c.x * c.w - c.y * c.z;
     const float al = fminf(a.alpha[g], 0.99f);
     s_ga[t] = GeoA{m.x, m.y, c.x, c.y};
-    s_gb[t] = GeoB{c.z, c.w, det, 1.0f / det};
+    s_gb[t] = GeoB{c.z, c.w, det, STAGED_RECIP ? staged_recip(det) : 1.0f / det};
     s_gc[t] = make_float2(al, reject_radius(al));
   }
 }
@@ -364,13 +388,14 @@ __device__ __forceinline__ void stage_sh_rows(const RasterArgs& a, const float* 
 // Stage entries [s0+base, s0+base+cnt) of the tile list into LDS (without
 // WITH_SH only the geometry and opacity).  NEG_LOG2E: the SH coefficients are
 // staged times -log2(e), so that a colour's dot product is the argument of
-// sigmoid_scaled (fast colours only).
-template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true, bool NEG_LOG2E = false>
+// sigmoid_scaled (fast colours only).  STAGED_RECIP as stage_geometry.
+template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true, bool NEG_LOG2E = false,
+          bool STAGED_RECIP = false>
 __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const float* __restrict__ shb,
                                                  const int32_t* __restrict__ idsb, int s0, int base, int cnt,
                                                  int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc, float* s_sh,
                                                  int* s_row = nullptr) {
-  stage_geometry<IDS_IN_LDS>(a, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_row);
+  stage_geometry<IDS_IN_LDS, STAGED_RECIP>(a, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_row);
   if constexpr (!WITH_SH) return;
   __syncthreads();
   stage_sh_rows<SHN, CC, NEG_LOG2E>(a, shb, cnt, s_id, s_sh, s_row);
@@ -526,13 +551,13 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
       cnt = filter_batch(a, e0, tx, ty, s_id, s_fq, s_fwh, fcur, fqh, fqn, fpar);
       if (cnt == 0) break;
       if (fidsb && (int)threadIdx.x < cnt) fidsb[a.foff[tile] + base + threadIdx.x] = s_id[threadIdx.x];
-      stage_batch_from<SHN, CC, true, true, !EXACT>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh,
-                                                     s_row);
+      stage_batch_from<SHN, CC, true, true, !EXACT, true>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc,
+                                                           s_sh, s_row);
     } else {
       if (base >= n) break;
       cnt = min(kBatch, n - base);
-      stage_batch_from<SHN, CC, false, true, !EXACT>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh,
-                                                      s_row);
+      stage_batch_from<SHN, CC, false, true, !EXACT, true>(a, shb, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc,
+                                                            s_sh, s_row);
     }
     __syncthreads();
     // entries of the batch that can reach a pixel of this wave, in order
@@ -550,8 +575,9 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
       const float tmpx = dx * gb.c3 - dy * gb.c2;
       const float tmpy = -dx * ga.c1 + dy * ga.c0;
       const float q = tmpx * dx + tmpy * dy;
-      if (q * gb.inv_det > gc.y) continue;
-      const float G = gauss_exact(q, gb.det);
+      const float rq = q * gb.inv_det;
+      if (rq > gc.y) continue;
+      const float G = gauss_staged(q, gb.det, gb.inv_det, rq);
       const float al = gc.x;
       float coeff = al * T;
       coeff *= G;
@@ -2861,4 +2887,24 @@ extern "C" int gsmi_render_views_sh_two_level(int64_t V, int B, const gsmi_view*
   a.rbr = br;
   a.super_shift = shift;
   return launch_fwd_two(a, C, exact, (hipStream_t)stream);
+}
+
+// Test hook: the forward's quotient q / det from the staged reciprocal
+// (radial_staged with y = staged_recip(det)), element-wise.
+__global__ void k_div_staged(int64_t n, const float* __restrict__ q, const float* __restrict__ det,
+                             float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float qi = q[i], di = det[i];
+  const float y = staged_recip(di);
+  out[i] = radial_staged(qi, di, y, qi * y);
+}
+
+extern "C" int gsmi_div_staged(int64_t n, const float* q, const float* det, float* out, void* stream) {
+  if (n < 0 || (n > 0 && (!q || !det || !out))) return GSMI_ERR_ARG;
+  if (n == 0) return GSMI_OK;
+  hipLaunchKernelGGL(k_div_staged, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, q, det,
+                     out);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
 }

@@ -90,6 +90,29 @@ __device__ __forceinline__ double exp_spec_d(double x) {
   return ldexp(p, (int)kd);
 }
 
+// Reciprocal of a Gaussian's determinant as staged for div_staged: RN(1/det)
+// where 2^-40 <= |det| <= 2^100 (then a normal number), else NaN, which
+// makes div_staged return NaN and its caller take the IEEE quotient.
+__device__ __forceinline__ float staged_recip(float det) {
+  const float ad = fabsf(det);
+  return (ad >= 0x1p-40f && ad <= 0x1p100f) ? 1.0f / det : __builtin_nanf("");
+}
+
+// q / det from y = staged_recip(det) and q0 = q * y, by two residual steps:
+// q0 is within 1.5 ulp of the quotient, r0 = q - det q0 (one FMA) corrects
+// it to a faithful q1, and with y within half an ulp of 1/det the second
+// step rounds correctly (Markstein's theorem; the remainder of a faithful
+// quotient is exact in an FMA while |q| >= 2^-103, where its bits stay
+// above the subnormal spacing).  So for |q| >= 2^-100 and finite inputs the
+// result is the IEEE quotient or NaN (an overflowed intermediate, a NaN y);
+// smaller |q| is left to the caller.
+__device__ __forceinline__ float div_staged(float q, float det, float y, float q0) {
+  const float r0 = fmaf(-q0, det, q);
+  const float q1 = fmaf(r0, y, q0);
+  const float r1 = fmaf(-q1, det, q);
+  return fmaf(r1, y, q1);
+}
+
 // shencoder.h:4 SIGMOID with the specified exponential (exact colour mode).
 __device__ __forceinline__ float sigmoid_exact(float x) { return 1.0f / (1.0f + exp_spec(-x)); }
 

@@ -690,6 +690,12 @@ int gsmi_exp_spec(int64_t n, const float *x, float *y, void *stream);
  * form.  Used by the tests to check the two agree bit for bit. */
 int gsmi_exp_spec_nonpos(int64_t n, const float *x, float *y, void *stream);
 
+/* The forward's quotient q[i] / det[i] from the staged reciprocal of det
+ * (two FMA residual steps; the IEEE division where they give NaN or
+ * |q| < 2^-100); no counterpart in the reference.  Used by the tests to
+ * check it against IEEE division bit for bit. */
+int gsmi_div_staged(int64_t n, const float *q, const float *det, float *out, void *stream);
+
 /* The render server's scheduling policy on the host alone (no device use);
  * no counterpart in the reference.  gsmi_sched_arena_run: an arena of `size`
  * bytes driven by n_ops operations (op, x, y) — op 0 allocates x bytes (out:

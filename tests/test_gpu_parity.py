@@ -87,6 +87,38 @@ def test_exp_spec_nonpos_bit_exact(orc):
     assert same.all(), (x[~same][:5], got[~same][:5], want[~same][:5])
 
 
+def test_div_staged_matches_ieee_division():
+    """The forward's q / det from the staged reciprocal (two FMA residual
+    steps, IEEE division where they give NaN or |q| < 2^-100) equals numpy's
+    IEEE float32 quotient bit for bit: quotients of the compositing range,
+    arbitrary and tiny numerators, determinants across [2^-40, 2^100] of
+    both signs, and out-of-range, zero and non-finite edges."""
+    rng = np.random.default_rng(17)
+    n = 1 << 22
+    det = (rng.uniform(1.0, 2.0, n) * np.exp2(rng.integers(-44, 105, n))).astype(np.float32)
+    det[rng.random(n) < 0.3] *= -1
+    t = rng.uniform(0.0, 300.0, n).astype(np.float32)
+    q = (t * det * (1.0 + rng.uniform(-1e-4, 1e-4, n))).astype(np.float32)
+    anyq = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    tiny = (rng.integers(0, 0x0E000000, n).astype(np.uint32) | (rng.integers(0, 2, n).astype(np.uint32) << 31)
+            ).view(np.float32)
+    edge_det = np.array([0.0, -0.0, 1e-45, 1e-30, 2.0 ** -41, 2.0 ** -40, 2.0 ** 100, 2.0 ** 101, 3e38, np.inf,
+                         -np.inf, np.nan, 1.0, 0.09], np.float32)
+    edge_q = np.array([0.0, -0.0, 1.0, -1.0, 1e-45, -1e-45, 3e38, np.inf, np.nan, 1e30, 0.5, -1e-20, 11.0, 2.0 ** -100],
+                      np.float32)
+    qs = np.concatenate([q, anyq, tiny, np.repeat(edge_q, edge_det.size)])
+    ds = np.concatenate([det, det, det, np.tile(edge_det, edge_q.size)])
+    qt, dt = torch.from_numpy(qs).to(DEV), torch.from_numpy(ds).to(DEV)
+    out = torch.empty_like(qt)
+    L = _lib.load()
+    _lib.check(L.gsmi_div_staged(qt.numel(), _lib.ptr(qt), _lib.ptr(dt), _lib.ptr(out), _lib.stream_of()), "div")
+    got = out.cpu().numpy()
+    with np.errstate(all="ignore"):
+        want = qs / ds
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), (qs[~same][:5], ds[~same][:5], got[~same][:5], want[~same][:5])
+
+
 @pytest.mark.parametrize("which", ["c1", "small4"])
 def test_cull_bit_exact(orc, which):
     sc = scenes.plain_forward_10k() if which == "c1" else small_sh_scene(4)
