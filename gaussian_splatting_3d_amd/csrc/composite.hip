@@ -247,6 +247,15 @@ __device__ __forceinline__ float gauss_exact(float q, float det) {
   return exp_spec_nonpos(-0.5f * radial);  // radial >= 0 or NaN
 }
 
+// The same exp(-radial / 2) for any radial with one select less: a negative
+// radial gives a positive exponent, mapped to -inf, whose exponential is 0
+// exactly as exp(-500) is; -0 and NaN pass unchanged.  (The backward keeps
+// gauss_exact: this form holds -inf in a register, which it has none for.)
+__device__ __forceinline__ float gauss_of_radial(float radial) {
+  const float x = -0.5f * radial;
+  return exp_spec_nonpos(x > 0.0f ? -__builtin_huge_valf() : x);
+}
+
 // The quotient q / det of gauss_exact from the staged reciprocal y =
 // staged_recip(det) and q0 = q * y: div_staged, or the IEEE division where
 // that gives NaN or |q| < 2^-100 (wave-uniform branch; the empty asm keeps
@@ -264,9 +273,7 @@ __device__ __forceinline__ float radial_staged(float q, float det, float y, floa
 
 // gauss_exact from the staged reciprocal: the same value.
 __device__ __forceinline__ float gauss_staged(float q, float det, float y, float q0) {
-  float radial = radial_staged(q, det, y, q0);
-  if (radial < 0.0f) radial = 1000.0f;
-  return exp_spec_nonpos(-0.5f * radial);
+  return gauss_of_radial(radial_staged(q, det, y, q0));
 }
 
 // Gaussian value for the backward pass: the hardware exponential of -r/2 with

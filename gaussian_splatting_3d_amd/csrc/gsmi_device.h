@@ -44,9 +44,11 @@ __device__ __forceinline__ float exp_spec(float x) {
 }
 
 // exp_spec for x <= 0 or NaN (a Gaussian's exponent, -r/2 with r >= 0):
-// the same value, without the overflow test and with the underflow and NaN
-// tests as selects instead of branches.  Inputs below -110 (including -inf)
-// are clamped before the reduction so that every intermediate is finite.
+// the same value, without the overflow test and with the NaN test as a
+// select.  Inputs below -120 (including -inf) are clamped first, so every
+// intermediate is finite; exp_spec's underflow test (x < -110 -> 0) needs
+// no select, since from -120 to -110 the scaled polynomial is below 2^-158
+// and ldexp rounds it to 0 as well.
 __device__ __forceinline__ float exp_spec_nonpos(float x) {
   const float xc = fmaxf(x, -120.0f);  // NaN -> -120; the result is replaced below
   const float kf = rintf(xc * 0x1.715476p+0f);
@@ -61,7 +63,7 @@ __device__ __forceinline__ float exp_spec_nonpos(float x) {
   p = fmaf(p, r, 1.0f);
   p = fmaf(p, r, 1.0f);
   const float e = ldexpf(p, (int)kf);
-  return x != x ? x : (x < -110.0f ? 0.0f : e);
+  return x != x ? x : e;
 }
 
 // Binary64 exponential, same scheme (32-bit hi part of ln2, degree-13
