@@ -292,6 +292,16 @@ __device__ __forceinline__ bool gauss_needs_exact(float aG, float q) {
          q == 0.0f;
 }
 
+// The wave's lanes where gauss_needs_exact holds, as a combination of the
+// ballots of its single comparisons: a ballot of one comparison is that
+// comparison's own lane mask, while a ballot of a combined condition goes
+// through a vector register (two more instructions).
+__device__ __forceinline__ uint64_t ballot_needs_exact(float aG, float q) {
+  return (__builtin_amdgcn_ballot_w64(aG > kMinRenderAlpha * (1.0f - 0x1p-12f)) &
+          __builtin_amdgcn_ballot_w64(aG < kMinRenderAlpha * (1.0f + 0x1p-12f))) |
+         __builtin_amdgcn_ballot_w64(!(aG == aG)) | __builtin_amdgcn_ballot_w64(q == 0.0f);
+}
+
 // The reference's kernel_gaussian_2d (double) from float inputs.
 __device__ __forceinline__ float gauss_exact_d(float dx, float dy, float c0, float c1, float c2, float c3) {
   const double C0 = c0, C1 = c1, C2 = c2, C3 = c3;
@@ -1444,15 +1454,18 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
         const float tmpy = -dx * ga.c1 + dy * ga.c0;
         const float q = tmpx * dx + tmpy * dy;
         const float rq = q * gb.inv_det;
-        const bool near = live && !(rq > gc.y);
+        // bitwise &, not &&: the conditions stay lane masks (no branch); the
+        // wave tests combine the ballots of single comparisons
+        const bool near = live & !(rq > gc.y);
+        const uint64_t b_near = __builtin_amdgcn_ballot_w64(live) & __builtin_amdgcn_ballot_w64(!(rq > gc.y));
         float G = gauss_fast(rq);
-        const bool exact = near && gauss_needs_exact(al * G, q);
-        if (wave_any(exact)) {
+        const bool exact = near & gauss_needs_exact(al * G, q);
+        if (b_near & ballot_needs_exact(al * G, q)) {
           if (exact) G = gauss_exact(q, gb.det);
         }
         const float aG = al * G;
-        const bool contrib = near && !(aG < kMinRenderAlpha);
-        if (wave_any(contrib)) {
+        const bool contrib = near & !(aG < kMinRenderAlpha);
+        if (b_near & __builtin_amdgcn_ballot_w64(!(aG < kMinRenderAlpha))) {
           const float om = 1.0f - aG;
           const float inv_om = __builtin_amdgcn_rcpf(om);
           const float Tj = T * inv_om;
@@ -1499,8 +1512,13 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           v[3] = Ds * uy;
           v[4] = Du * uy;
           v[5] = contrib ? P * G : 0.0f;
-          v[6] = 0.0f;
-          v[7] = 0.0f;
+          // components 6 and 7 are unused and their lanes never write, so
+          // any register serves as their partner in the exchanges (its
+          // value stays in those lanes' sums)
+          float spare;
+          asm volatile("" : "=v"(spare));
+          v[6] = spare;
+          v[7] = spare;
           if constexpr (SPLIT) {
             uint32_t h, l;
             split_bf16(cc0, h, l);
@@ -1525,7 +1543,18 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           }
           wtouch |= 1u << j;
           const float r = wave_reduce_scatter8_swap(v, lane);
-          if (writer && r != 0.0f) atomicAdd(s_acc + j * 6 + my_comp, r * comp_scale);
+          if (writer && r != 0.0f) {
+            // LDS float add at s_acc[6 j + my_comp]: the entry's byte offset
+            // formed in a scalar register (the empty asm keeps it there) and
+            // added to the lane's address; otherwise the compiler fuses both
+            // into a 64-bit multiply-add per entry
+            typedef __attribute__((address_space(3))) float lds_float;
+            typedef __attribute__((address_space(3))) char lds_char;
+            uint32_t jb = __builtin_amdgcn_readfirstlane(j * 24);  // j is wave-uniform
+            asm volatile("" : "+s"(jb));
+            lds_float* dst = (lds_float*)((lds_char*)((lds_float*)s_acc + my_comp) + jb);
+            __hip_atomic_fetch_add(dst, r * comp_scale, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
         }
       }
       }  // entries of the chunk
