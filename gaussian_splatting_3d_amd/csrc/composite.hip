@@ -559,6 +559,11 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
   static_assert(kBatch <= 64, "one batch entry per lane in the relevance mask");
   const float4 wr = wave_pixel_rect(px, py);
   const int lane = threadIdx.x & 63;
+  // A lane is done when outside the image or once T < thresh (T stays put
+  // from then on), so the wave is done when these two ballots of single
+  // comparisons cover it; a ballot of `done` itself would go through a
+  // vector register each entry.
+  const uint64_t outside = __builtin_amdgcn_ballot_w64(!inside);
   int fcur = s0, fqh = 0, fqn = 0, fpar = 0;  // filter state (TWO), block-uniform
   int base = 0;
   for (;;) {
@@ -583,7 +588,7 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
     while (rel) {
       const int j = __builtin_ctzll(rel);
       rel &= rel - 1;
-      if (__all(done)) break;
+      if ((__builtin_amdgcn_ballot_w64(T < thresh) | outside) == ~0ull) break;
       if (done) continue;
       const GeoA ga = s_ga[j];
       const GeoB gb = s_gb[j];
