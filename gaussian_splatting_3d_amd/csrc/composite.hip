@@ -585,6 +585,10 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
     // entries of the batch that can reach a pixel of this wave, in order
     const int le = lane < cnt ? lane : 0;
     uint64_t rel = __ballot(lane < cnt && may_reach(s_ga[le], s_gb[le], s_gc[le].y, wr.x, wr.y, wr.z, wr.w));
+    // entries examined when entry j saturates a pixel: base + j + 1, with
+    // base (block-uniform) in a scalar register, so that the per-entry
+    // update is one select
+    const int nd1 = __builtin_amdgcn_readfirstlane(base) + 1;
     while (rel) {
       const int j = __builtin_ctzll(rel);
       rel &= rel - 1;
@@ -593,6 +597,9 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
       const GeoA ga = s_ga[j];
       const GeoB gb = s_gb[j];
       const float2 gc = s_gc[j];
+      // det and alpha read with the rest of the record: deferred into the
+      // branch, their reads each cost an address move from the scalar index
+      asm volatile("" ::"v"(gb.det), "v"(gc.x));
       const float dx = px - ga.mx, dy = py - ga.my;
       const float tmpx = dx * gb.c3 - dy * gb.c2;
       const float tmpy = -dx * ga.c1 + dy * ga.c0;
@@ -633,7 +640,7 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
       T = T * (1.0f - al * G);
       if (T < thresh) {
         done = true;
-        ndone = base + j + 1;
+        ndone = nd1 + j;
       }
     }
     base += cnt;
