@@ -335,9 +335,14 @@ __device__ __forceinline__ float dot_fast(const float* a, const float* b) {
 // and geometry of the batch's entries.  STAGED_RECIP: 1/det as staged_recip
 // (for gauss_staged; NaN outside its range, which the relevance and distance
 // tests read as "may reach").
-template <bool IDS_IN_LDS = false, bool STAGED_RECIP = false>
+__device__ __forceinline__ void put_gc(float2* p, int t, float al, float r) { p[t] = make_float2(al, r); }
+__device__ __forceinline__ void put_gc(float4* p, int t, float al, float r) { p[t] = make_float4(al, r, 0.f, 0.f); }
+
+// GC: float2 {opacity, rejection radius}, or float4 with two unused words
+// (the stride of GeoA and GeoB, so that one address serves all three).
+template <bool IDS_IN_LDS = false, bool STAGED_RECIP = false, typename GC = float2>
 __device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_t* __restrict__ idsb, int s0,
-                                               int base, int cnt, int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc,
+                                               int base, int cnt, int* s_id, GeoA* s_ga, GeoB* s_gb, GC* s_gc,
                                                int* s_row) {
   const int t = threadIdx.x;
   if (t < cnt) {
@@ -350,7 +355,7 @@ __device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_
     const float al = fminf(a.alpha[g], 0.99f);
     s_ga[t] = GeoA{m.x, m.y, c.x, c.y};
     s_gb[t] = GeoB{c.z, c.w, det, STAGED_RECIP ? staged_recip(det) : 1.0f / det};
-    s_gc[t] = make_float2(al, reject_radius(al));
+    put_gc(s_gc, t, al, reject_radius(al));
   }
 }
 
@@ -407,10 +412,10 @@ __device__ __forceinline__ void stage_sh_rows(const RasterArgs& a, const float* 
 // staged times -log2(e), so that a colour's dot product is the argument of
 // sigmoid_scaled (fast colours only).  STAGED_RECIP as stage_geometry.
 template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true, bool NEG_LOG2E = false,
-          bool STAGED_RECIP = false>
+          bool STAGED_RECIP = false, typename GC = float2>
 __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const float* __restrict__ shb,
                                                  const int32_t* __restrict__ idsb, int s0, int base, int cnt,
-                                                 int* s_id, GeoA* s_ga, GeoB* s_gb, float2* s_gc, float* s_sh,
+                                                 int* s_id, GeoA* s_ga, GeoB* s_gb, GC* s_gc, float* s_sh,
                                                  int* s_row = nullptr) {
   stage_geometry<IDS_IN_LDS, STAGED_RECIP>(a, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_row);
   if constexpr (!WITH_SH) return;
@@ -485,7 +490,7 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
   static_assert(!(TWO && BG), "two-level lists without background");
   __shared__ GeoA s_ga[kBatch];
   __shared__ GeoB s_gb[kBatch];
-  __shared__ float2 s_gc[kBatch];
+  __shared__ float4 s_gc[kBatch];  // {opacity, rejection radius, -, -}: the stride of s_ga / s_gb
   __shared__ int s_id[kBatch];
   __shared__ __align__(16) float s_sh[kBatch * SHN];
   __shared__ int s_row[kBatch];  // SH row of each batch entry
@@ -596,7 +601,7 @@ __device__ __forceinline__ void fwd_sh_tile(const RasterArgs& a, TileCtx& tc) {
       if (done) continue;
       const GeoA ga = s_ga[j];
       const GeoB gb = s_gb[j];
-      const float2 gc = s_gc[j];
+      const float4 gc = s_gc[j];
       // det and alpha read with the rest of the record: deferred into the
       // branch, their reads each cost an address move from the scalar index
       asm volatile("" ::"v"(gb.det), "v"(gc.x));
