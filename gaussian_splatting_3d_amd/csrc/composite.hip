@@ -340,6 +340,19 @@ __device__ __forceinline__ void put_gc(float4* p, int t, float al, float r) { p[
 
 // GC: float2 {opacity, rejection radius}, or float4 with two unused words
 // (the stride of GeoA and GeoB, so that one address serves all three).
+// Geometry of Gaussian g into batch position t; returns its opacity as
+// composited (clamped at 0.99).
+template <bool STAGED_RECIP = false>
+__device__ __forceinline__ float stage_geo(const RasterArgs& a, int t, int g, GeoA* s_ga, GeoB* s_gb) {
+  const float2 m = reinterpret_cast<const float2*>(a.mean)[g];
+  const float4 c = reinterpret_cast<const float4*>(a.cov)[g];
+  const float det = c.x * c.w - c.y * c.z;
+  const float al = fminf(a.alpha[g], 0.99f);
+  s_ga[t] = GeoA{m.x, m.y, c.x, c.y};
+  s_gb[t] = GeoB{c.z, c.w, det, STAGED_RECIP ? staged_recip(det) : 1.0f / det};
+  return al;
+}
+
 template <bool IDS_IN_LDS = false, bool STAGED_RECIP = false, typename GC = float2>
 __device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_t* __restrict__ idsb, int s0,
                                                int base, int cnt, int* s_id, GeoA* s_ga, GeoB* s_gb, GC* s_gc,
@@ -349,12 +362,7 @@ __device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_
     const int g = IDS_IN_LDS ? s_id[t] : idsb[s0 + base + t];
     s_id[t] = g;
     if (s_row) s_row[t] = a.sh_index ? a.sh_index[g] : g;  // the entry's SH row
-    const float2 m = reinterpret_cast<const float2*>(a.mean)[g];
-    const float4 c = reinterpret_cast<const float4*>(a.cov)[g];
-    const float det = c.x * c.w - c.y * c.z;
-    const float al = fminf(a.alpha[g], 0.99f);
-    s_ga[t] = GeoA{m.x, m.y, c.x, c.y};
-    s_gb[t] = GeoB{c.z, c.w, det, STAGED_RECIP ? staged_recip(det) : 1.0f / det};
+    const float al = stage_geo<STAGED_RECIP>(a, t, g, s_ga, s_gb);
     put_gc(s_gc, t, al, reject_radius(al));
   }
 }
@@ -363,13 +371,51 @@ __device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_
 // published stage_geometry's s_id / s_row.  NEG_LOG2E: the coefficients are
 // staged times -log2(e), so that a colour's dot product is the argument of
 // sigmoid_scaled (fast colours only).
+template <int SHN, int CC, bool NEG_LOG2E = false, typename RowOf>
+__device__ __forceinline__ void stage_sh_rows_of(const RasterArgs& a, const float* __restrict__ shb, int cnt,
+                                                 RowOf row_of, float* s_sh) {
+  const int t = threadIdx.x;
+  if constexpr (SHN % 4 == 0) {
+    // Rows of SHN contiguous floats (the stored order is the rendered one):
+    // 16-byte pieces, SHN / 4 per entry.  Block-uniform test.
+    if (a.sh_sc == CC && a.sh_sg == SHN && (reinterpret_cast<uintptr_t>(shb) & 15) == 0) {
+      constexpr int Q = SHN / 4;
+      const float4* sh4 = reinterpret_cast<const float4*>(shb);
+      float4* d4 = reinterpret_cast<float4*>(s_sh);
+      for (int e = t; e < cnt * Q; e += kTilePixels) {
+        const int j = e / Q;
+        const int64_t row = row_of(j);
+        float4 v = sh4[row * Q + (e - j * Q)];
+        if constexpr (NEG_LOG2E) {
+          v.x *= kNegLog2e;
+          v.y *= kNegLog2e;
+          v.z *= kNegLog2e;
+          v.w *= kNegLog2e;
+        }
+        d4[e] = v;
+      }
+      return;
+    }
+  }
+  for (int e = t; e < cnt * SHN; e += kTilePixels) {
+    const int j = e / SHN;
+    const int r = e - j * SHN;
+    const int ch = r / CC;
+    const int k = r - ch * CC;
+    const int64_t row = row_of(j);
+    const float v = shb[row * a.sh_sg + ch * a.sh_sc + k];
+    s_sh[e] = NEG_LOG2E ? v * kNegLog2e : v;
+  }
+}
+
+// stage_sh_rows_of with the rows recorded in s_row by stage_geometry, or
+// looked up from the ids in s_id (the same loop written out: through the
+// row functor the forward compiles to a slower schedule).
 template <int SHN, int CC, bool NEG_LOG2E = false>
 __device__ __forceinline__ void stage_sh_rows(const RasterArgs& a, const float* __restrict__ shb, int cnt,
                                               const int* s_id, float* s_sh, const int* s_row) {
   const int t = threadIdx.x;
   if constexpr (SHN % 4 == 0) {
-    // Rows of SHN contiguous floats (the stored order is the rendered one):
-    // 16-byte pieces, SHN / 4 per entry.  Block-uniform test.
     if (a.sh_sc == CC && a.sh_sg == SHN && (reinterpret_cast<uintptr_t>(shb) & 15) == 0) {
       constexpr int Q = SHN / 4;
       const float4* sh4 = reinterpret_cast<const float4*>(shb);
@@ -407,6 +453,7 @@ __device__ __forceinline__ void stage_sh_rows(const RasterArgs& a, const float* 
   }
 }
 
+
 // Stage entries [s0+base, s0+base+cnt) of the tile list into LDS (without
 // WITH_SH only the geometry and opacity).  NEG_LOG2E: the SH coefficients are
 // staged times -log2(e), so that a colour's dot product is the argument of
@@ -423,9 +470,9 @@ __device__ __forceinline__ void stage_batch_from(const RasterArgs& a, const floa
   stage_sh_rows<SHN, CC, NEG_LOG2E>(a, shb, cnt, s_id, s_sh, s_row);
 }
 
-template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true>
+template <int SHN, int CC, bool IDS_IN_LDS = false, bool WITH_SH = true, typename GC = float2>
 __device__ __forceinline__ void stage_batch(const RasterArgs& a, int s0, int base, int cnt, int* s_id, GeoA* s_ga,
-                                            GeoB* s_gb, float2* s_gc, float* s_sh) {
+                                            GeoB* s_gb, GC* s_gc, float* s_sh) {
   stage_batch_from<SHN, CC, IDS_IN_LDS, WITH_SH>(a, a.sh, a.ids, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_sh);
 }
 
@@ -732,10 +779,10 @@ __device__ __forceinline__ void stage_rgb(const RasterArgs& a, int s0, int base,
 // kernel's staging arrays.  The reference recomputes the same front-to-back
 // state in its backward (vol_render_sh.h:268-351).
 // ---------------------------------------------------------------------------
-template <int NB, int C, bool COLOUR, bool RGB>
+template <int NB, int C, bool COLOUR, bool RGB, typename GC = float2>
 __device__ __forceinline__ void forward_state(const RasterArgs& a, int s0, int n, const float* Y, float px,
                                               float py, bool inside, int* s_id, GeoA* s_ga, GeoB* s_gb,
-                                              float2* s_gc, float* s_sh, float& T, int& nd, float& p0, float& p1,
+                                              GC* s_gc, float* s_sh, float& T, int& nd, float& p0, float& p1,
                                               float& p2) {
   constexpr int CC = C * C;
   constexpr int SHN = 3 * CC;
@@ -772,7 +819,7 @@ __device__ __forceinline__ void forward_state(const RasterArgs& a, int s0, int n
       if (done) continue;
       const GeoA ga = s_ga[j];
       const GeoB gb = s_gb[j];
-      const float2 gc = s_gc[j];
+      const auto gc = s_gc[j];
       const float dx = px - ga.mx, dy = py - ga.my;
       const float tmpx = dx * gb.c3 - dy * gb.c2;
       const float tmpy = -dx * ga.c1 + dy * ga.c0;
@@ -1200,15 +1247,15 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   static_assert(!(CMF && SPLIT), "matrix-core colours with the fp32 reduction only");
   __shared__ GeoA s_ga[kMfmaBatch];
   __shared__ GeoB s_gb[kMfmaBatch];
-  __shared__ float2 s_gc[kMfmaBatch];
-  __shared__ int s_id[kMfmaBatch];
+  // {opacity, rejection radius, id, SH row} (the last two as int bits): the
+  // stride of s_ga / s_gb, so one address serves an entry's three reads
+  __shared__ float4 s_gc[kMfmaBatch];
   __shared__ __align__(16) float s_sh[kMfmaBatch * SHN];
   __shared__ float s_acc[kMfmaBatch * 6];                      // scalar components, shared
   __shared__ __align__(16) float s_shacc[4][kMfmaBatch * SHN];  // SH components, per wave
   __shared__ __align__(16) uint32_t s_cc[4][kCcWords];         // per wave: staged coefficient rows
   __shared__ int s_tmp[4];
   __shared__ uint32_t s_touch;          // batch entries with a contribution in some wave
-  __shared__ int s_row[kMfmaBatch];     // SH row of each batch entry
 
   TileCtx tc;
   if (!tile_setup(a, tc)) return;
@@ -1241,7 +1288,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   int nd = 0;
   float pre0 = 0.0f, pre1 = 0.0f, pre2 = 0.0f;
   if constexpr (RECOMP)
-    forward_state<kMfmaBatch, C, BG, false>(a, s0, e0 - s0, Y, px, py, inside, s_id, s_ga, s_gb, s_gc, s_sh, T,
+    forward_state<kMfmaBatch, C, BG, false>(a, s0, e0 - s0, Y, px, py, inside, reinterpret_cast<int*>(s_acc), s_ga, s_gb, s_gc, s_sh, T,
                                             nd, pre0, pre1, pre2);
   if (inside) {
     if constexpr (!RECOMP) {
@@ -1345,7 +1392,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   // (entry, component) grid, entry e / NCOMP, and reads, zeroes and emits
   // those of touched entries.  The ids and SH rows of its entries are kept
   // in registers (keep_rows, at the end of the batch's compositing), so the
-  // flush of batch n - 1 can run while batch n is staged over s_id / s_row.
+  // flush of batch n - 1 can run while batch n is staged over them.
   // The thread index passes through an empty asm in both, so that their
   // index arithmetic, the same in every batch, is not hoisted out of the
   // batch loop into registers the compositing needs.
@@ -1357,8 +1404,9 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
 #pragma unroll
     for (int i = 0; i < kFlushElems; ++i) {
       const int k = min((tv + i * kTilePixels) / NCOMP, kMfmaBatch - 1);
-      f_id[i] = s_id[k];
-      f_row[i] = s_row[k];
+      const int4 r = reinterpret_cast<const int4*>(s_gc)[k];
+      f_id[i] = r.z;
+      f_row[i] = r.w;
     }
   };
   auto flush = [&](uint32_t touch) {
@@ -1391,10 +1439,16 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
   for (int base = ((nmax - 1) / kMfmaBatch) * kMfmaBatch; nmax > 0 && base >= 0; base -= kMfmaBatch, ++nb) {
     const int cnt = min(kMfmaBatch, nmax - base);
     const uint32_t ptouch = nb > 0 ? s_touch : 0u;  // batch n - 1's, complete since the last barrier
-    stage_geometry<false>(a, idsb, s0, base, cnt, s_id, s_ga, s_gb, s_gc, s_row);
+    if (t < cnt) {
+      const int g = idsb[s0 + base + t];
+      const int row = a.sh_index ? a.sh_index[g] : g;
+      const float al = stage_geo(a, t, g, s_ga, s_gb);
+      reinterpret_cast<int4*>(s_gc)[t] = make_int4(__float_as_int(al), __float_as_int(reject_radius(al)), g, row);
+    }
     __syncthreads();
     if (t == 0) s_touch = 0u;  // every thread has read it before the barrier
-    stage_sh_rows<SHN, CC, true>(a, a.sh, cnt, s_id, s_sh, s_row);
+    stage_sh_rows_of<SHN, CC, true>(
+        a, a.sh, cnt, [&](int j) -> int64_t { return reinterpret_cast<const int4*>(s_gc)[j].w; }, s_sh);
     flush(ptouch);
     __syncthreads();
     int n_slots = 0;      // filled slots of the current chunk (wave-uniform)
@@ -1464,7 +1518,7 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
       if (wave_any(live)) {
         const GeoA ga = s_ga[j];
         const GeoB gb = s_gb[j];
-        const float2 gc = s_gc[j];
+        const float4 gc = s_gc[j];
         const float al = gc.x;
         const float dx = px - ga.mx, dy = py - ga.my;
         const float tmpx = dx * gb.c3 - dy * gb.c2;
