@@ -280,16 +280,19 @@ __device__ __forceinline__ float gauss_staged(float q, float det, float y, float
 // r = q/det from the staged reciprocal (1000 for r < 0, as the reference).
 // Values feed gradients only; decisions use gauss_needs_exact.
 __device__ __forceinline__ float gauss_fast(float rq) {
-  return __builtin_amdgcn_exp2f((rq < 0.0f ? 1000.0f : rq) * -0.72134752044448170f);  // exp(-r/2)
+  // exp(-r/2) = 2^(-r log2(e) / 2); a negative rq (taken as 1000) gives a
+  // positive exponent, mapped to -inf: 0 either way, without a constant move
+  const float x = rq * -0.72134752044448170f;
+  return __builtin_amdgcn_exp2f(x > 0.0f ? -__builtin_huge_valf() : x);
 }
 
 // Where the fast value cannot take the forward's 1/255 skip decision: a*G
-// within a relative 2^-12 of the threshold, not a number, or q == 0; there
-// the exact sequence (gauss_exact) decides, so the decision always equals
-// the forward's.
+// within a relative 2^-12 of the threshold (far wider than the hardware
+// exponential's error), not a number, or q == 0; there the exact sequence
+// (gauss_exact) decides, so the decision always equals the forward's.  One
+// comparison covers the window and NaN (which fails >=).
 __device__ __forceinline__ bool gauss_needs_exact(float aG, float q) {
-  return (aG > kMinRenderAlpha * (1.0f - 0x1p-12f) && aG < kMinRenderAlpha * (1.0f + 0x1p-12f)) || !(aG == aG) ||
-         q == 0.0f;
+  return !(fabsf(aG - kMinRenderAlpha) >= kMinRenderAlpha * 0x1p-12f) || q == 0.0f;
 }
 
 // The wave's lanes where gauss_needs_exact holds, as a combination of the
@@ -297,9 +300,8 @@ __device__ __forceinline__ bool gauss_needs_exact(float aG, float q) {
 // comparison's own lane mask, while a ballot of a combined condition goes
 // through a vector register (two more instructions).
 __device__ __forceinline__ uint64_t ballot_needs_exact(float aG, float q) {
-  return (__builtin_amdgcn_ballot_w64(aG > kMinRenderAlpha * (1.0f - 0x1p-12f)) &
-          __builtin_amdgcn_ballot_w64(aG < kMinRenderAlpha * (1.0f + 0x1p-12f))) |
-         __builtin_amdgcn_ballot_w64(!(aG == aG)) | __builtin_amdgcn_ballot_w64(q == 0.0f);
+  return __builtin_amdgcn_ballot_w64(!(fabsf(aG - kMinRenderAlpha) >= kMinRenderAlpha * 0x1p-12f)) |
+         __builtin_amdgcn_ballot_w64(q == 0.0f);
 }
 
 // The reference's kernel_gaussian_2d (double) from float inputs.
@@ -1526,9 +1528,14 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
         const float q = tmpx * dx + tmpy * dy;
         const float rq = q * gb.inv_det;
         // bitwise &, not &&: the conditions stay lane masks (no branch); the
-        // wave tests combine the ballots of single comparisons
-        const bool near = live & !(rq > gc.y);
-        const uint64_t b_near = __builtin_amdgcn_ballot_w64(live) & __builtin_amdgcn_ballot_w64(!(rq > gc.y));
+        // wave tests combine the ballots of single comparisons.  `live` is
+        // compared again here (one instruction) rather than carried into the
+        // block, which the compiler does through a vector register (two).
+        int idx_s = __builtin_amdgcn_readfirstlane(idx);  // wave-uniform
+        asm volatile("" : "+s"(idx_s));  // a scalar copy: the compare is not merged with the first
+        const bool live_in = idx_s < nd;
+        const bool near = live_in & !(rq > gc.y);
+        const uint64_t b_near = __builtin_amdgcn_ballot_w64(live_in) & __builtin_amdgcn_ballot_w64(!(rq > gc.y));
         float G = gauss_fast(rq);
         const bool exact = near & gauss_needs_exact(al * G, q);
         if (b_near & ballot_needs_exact(al * G, q)) {
