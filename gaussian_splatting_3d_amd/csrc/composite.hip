@@ -1593,10 +1593,10 @@ __device__ __forceinline__ void bwd_sh_mfma_body(const RasterArgs& a) {
           // components 6 and 7 are unused and their lanes never write, so
           // any register serves as their partner in the exchanges (its
           // value stays in those lanes' sums)
-          float spare;
-          asm volatile("" : "=v"(spare));
-          v[6] = spare;
-          v[7] = spare;
+          float spare6, spare7;  // two: each exchange rewrites its partner
+          asm volatile("" : "=v"(spare6), "=v"(spare7));
+          v[6] = spare6;
+          v[7] = spare7;
           if constexpr (SPLIT) {
             uint32_t h, l;
             split_bf16(cc0, h, l);
