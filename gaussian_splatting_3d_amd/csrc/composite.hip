@@ -369,10 +369,10 @@ __device__ __forceinline__ void stage_geometry(const RasterArgs& a, const int32_
   }
 }
 
-// Second half: the SH rows of the staged entries, once a barrier has
-// published stage_geometry's s_id / s_row.  NEG_LOG2E: the coefficients are
-// staged times -log2(e), so that a colour's dot product is the argument of
-// sigmoid_scaled (fast colours only).
+// Second half of a batch's staging: the SH rows of its entries, row_of(j)
+// giving entry j's row index, once a barrier has published what row_of
+// reads.  NEG_LOG2E: the coefficients are staged times -log2(e), so that a
+// colour's dot product is the argument of sigmoid_scaled (fast colours only).
 template <int SHN, int CC, bool NEG_LOG2E = false, typename RowOf>
 __device__ __forceinline__ void stage_sh_rows_of(const RasterArgs& a, const float* __restrict__ shb, int cnt,
                                                  RowOf row_of, float* s_sh) {
