@@ -252,13 +252,14 @@ __device__ __forceinline__ float gauss_exact(float q, float det) {
   return exp_spec_nonpos(-0.5f * radial);  // radial >= 0 or NaN
 }
 
-// The same exp(-radial / 2) for any radial with one select less: a negative
-// radial gives a positive exponent, mapped to -inf, whose exponential is 0
-// exactly as exp(-500) is; -0 and NaN pass unchanged.  (The backward keeps
-// gauss_exact: this form holds -inf in a register, which it has none for.)
+// The same exp(-radial / 2) for any radial without a constant move: a
+// negative radial (taken as 1000) selects the exponent -inf, whose
+// exponential is 0 exactly as exp(-500) is; -0 and NaN pass unchanged.  The
+// test is on radial itself: -radial/2 of the smallest negative subnormal
+// rounds to +0.  (The backward keeps gauss_exact: this form holds -inf in a
+// register, which it has none for.)
 __device__ __forceinline__ float gauss_of_radial(float radial) {
-  const float x = -0.5f * radial;
-  return exp_spec_nonpos(x > 0.0f ? -__builtin_huge_valf() : x);
+  return exp_spec_nonpos(radial < 0.0f ? -__builtin_huge_valf() : -0.5f * radial);
 }
 
 // The quotient q / det of gauss_exact from the staged reciprocal y =
@@ -285,10 +286,10 @@ __device__ __forceinline__ float gauss_staged(float q, float det, float y, float
 // r = q/det from the staged reciprocal (1000 for r < 0, as the reference).
 // Values feed gradients only; decisions use gauss_needs_exact.
 __device__ __forceinline__ float gauss_fast(float rq) {
-  // exp(-r/2) = 2^(-r log2(e) / 2); a negative rq (taken as 1000) gives a
-  // positive exponent, mapped to -inf: 0 either way, without a constant move
-  const float x = rq * -0.72134752044448170f;
-  return __builtin_amdgcn_exp2f(x > 0.0f ? -__builtin_huge_valf() : x);
+  // exp(-r/2) = 2^(-r log2(e) / 2); a negative rq (taken as 1000) selects
+  // the exponent -inf: 0 either way, without a constant move.  The test is
+  // on rq itself: the smallest negative values of rq scale to +0.
+  return __builtin_amdgcn_exp2f(rq < 0.0f ? -__builtin_huge_valf() : rq * -0.72134752044448170f);
 }
 
 // Where the fast value cannot take the forward's 1/255 skip decision: a*G
@@ -3009,21 +3010,24 @@ extern "C" int gsmi_render_views_sh_two_level(int64_t V, int B, const gsmi_view*
 }
 
 // Test hook: the forward's quotient q / det from the staged reciprocal
-// (radial_staged with y = staged_recip(det)), element-wise.
+// (radial_staged with y = staged_recip(det)) and, when gauss is given, its
+// Gaussian value (gauss_staged), element-wise.
 __global__ void k_div_staged(int64_t n, const float* __restrict__ q, const float* __restrict__ det,
-                             float* __restrict__ out) {
+                             float* __restrict__ out, float* __restrict__ gauss) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float qi = q[i], di = det[i];
   const float y = staged_recip(di);
   out[i] = radial_staged(qi, di, y, qi * y);
+  if (gauss) gauss[i] = gauss_staged(qi, di, y, qi * y);
 }
 
-extern "C" int gsmi_div_staged(int64_t n, const float* q, const float* det, float* out, void* stream) {
+extern "C" int gsmi_div_staged(int64_t n, const float* q, const float* det, float* out, float* gauss,
+                               void* stream) {
   if (n < 0 || (n > 0 && (!q || !det || !out))) return GSMI_ERR_ARG;
   if (n == 0) return GSMI_OK;
   hipLaunchKernelGGL(k_div_staged, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, q, det,
-                     out);
+                     out, gauss);
   GSMI_CHECK_LAUNCH();
   return GSMI_OK;
 }

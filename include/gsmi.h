@@ -692,9 +692,10 @@ int gsmi_exp_spec_nonpos(int64_t n, const float *x, float *y, void *stream);
 
 /* The forward's quotient q[i] / det[i] from the staged reciprocal of det
  * (two FMA residual steps; the IEEE division where they give NaN or
- * |q| < 2^-100); no counterpart in the reference.  Used by the tests to
- * check it against IEEE division bit for bit. */
-int gsmi_div_staged(int64_t n, const float *q, const float *det, float *out, void *stream);
+ * |q| < 2^-100) and, when gauss is not null, its Gaussian value
+ * exp(-radial / 2) (radial < 0 taken as 1000); no counterpart in the
+ * reference.  Used by the tests to check both bit for bit. */
+int gsmi_div_staged(int64_t n, const float *q, const float *det, float *out, float *gauss, void *stream);
 
 /* The render server's scheduling policy on the host alone (no device use);
  * no counterpart in the reference.  gsmi_sched_arena_run: an arena of `size`

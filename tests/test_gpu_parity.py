@@ -87,12 +87,14 @@ def test_exp_spec_nonpos_bit_exact(orc):
     assert same.all(), (x[~same][:5], got[~same][:5], want[~same][:5])
 
 
-def test_div_staged_matches_ieee_division():
+def test_div_staged_matches_ieee_division(orc):
     """The forward's q / det from the staged reciprocal (two FMA residual
     steps, IEEE division where they give NaN or |q| < 2^-100) equals numpy's
     IEEE float32 quotient bit for bit: quotients of the compositing range,
     arbitrary and tiny numerators, determinants across [2^-40, 2^100] of
-    both signs, and out-of-range, zero and non-finite edges."""
+    both signs, and out-of-range, zero and non-finite edges; and the Gaussian
+    value from it (a negative radial mapped to a -inf exponent) equals the
+    oracle's exp(-radial / 2) with radial < 0 taken as 1000."""
     rng = np.random.default_rng(17)
     n = 1 << 22
     det = (rng.uniform(1.0, 2.0, n) * np.exp2(rng.integers(-44, 105, n))).astype(np.float32)
@@ -110,13 +112,22 @@ def test_div_staged_matches_ieee_division():
     ds = np.concatenate([det, det, det, np.tile(edge_det, edge_q.size)])
     qt, dt = torch.from_numpy(qs).to(DEV), torch.from_numpy(ds).to(DEV)
     out = torch.empty_like(qt)
+    gauss = torch.empty_like(qt)
     L = _lib.load()
-    _lib.check(L.gsmi_div_staged(qt.numel(), _lib.ptr(qt), _lib.ptr(dt), _lib.ptr(out), _lib.stream_of()), "div")
+    _lib.check(L.gsmi_div_staged(qt.numel(), _lib.ptr(qt), _lib.ptr(dt), _lib.ptr(out), _lib.ptr(gauss),
+                                 _lib.stream_of()), "div")
     got = out.cpu().numpy()
     with np.errstate(all="ignore"):
         want = qs / ds
     same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
     assert same.all(), (qs[~same][:5], ds[~same][:5], got[~same][:5], want[~same][:5])
+    # the Gaussian value: the reference's exp(-radial / 2) with radial < 0 taken as 1000, as the oracle's
+    # specified exponential computes it
+    radial = np.where(want < 0, np.float32(1000.0), want).astype(np.float32)
+    g_want = orc.expf((np.float32(-0.5) * radial).astype(np.float32))
+    g_got = gauss.cpu().numpy()
+    same = (g_got.view(np.uint32) == g_want.view(np.uint32)) | (np.isnan(g_got) & np.isnan(g_want))
+    assert same.all(), (radial[~same][:5], g_got[~same][:5], g_want[~same][:5])
 
 
 @pytest.mark.parametrize("which", ["c1", "small4"])
