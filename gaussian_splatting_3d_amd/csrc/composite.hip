@@ -1983,9 +1983,12 @@ constexpr int kLevelBits = 32 - __builtin_clz((unsigned)kOrderLevels);
 static_assert(kOrderLevels >= 2 && kOrderLevels <= 64, "one lane per level in the placement counts");
 constexpr int kOrderSingleMax = 1 << 15;  // grids up to this many tiles are ordered by one workgroup
 
-// Cost level of a tile: 0 holds the most expensive (mx = the largest cost, >= 1).
-__device__ __forceinline__ uint32_t order_level(uint32_t c, uint64_t mx) {
-  return (uint32_t)((kOrderLevels - 1) - (int)(((uint64_t)c * (uint64_t)(kOrderLevels - 1)) / mx));
+// Cost level of a tile: 0 holds the most expensive.  scale = (levels - 1) /
+// the largest cost (>= 1), from order_scale; the level is a launch-order
+// heuristic, so float rounding at a level's edge is of no consequence.
+__device__ __forceinline__ float order_scale(uint32_t mx) { return (float)(kOrderLevels - 1) / (float)max(mx, 1u); }
+__device__ __forceinline__ uint32_t order_level(uint32_t c, float scale) {
+  return (uint32_t)((kOrderLevels - 1) - min((int)((float)c * scale), kOrderLevels - 1));
 }
 
 __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __restrict__ cost, int n,
@@ -1994,6 +1997,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
   __shared__ uint32_t s_cursor[kOrderLevels];
   __shared__ uint32_t s_wc[NW][kOrderLevels];
   __shared__ uint32_t s_max;
+  __shared__ uint8_t s_lv[kOrderSingleMax];  // each tile's level, from the counting pass
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (t == 0) s_max = 1u;
   if (t < kOrderLevels) s_cursor[t] = 0u;
@@ -2002,26 +2006,31 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
   for (int i = t; i < n; i += kOrderThreads) m = max(m, cost[i]);
   atomicMax(&s_max, m);
   __syncthreads();
-  const uint64_t mx = s_max;
-  auto level_of = [mx](uint32_t c) { return (int)order_level(c, mx); };
-  for (int i = t; i < n; i += kOrderThreads) atomicAdd(&s_cursor[level_of(cost[i])], 1u);
+  const float scale = order_scale(s_max);
+  for (int i = t; i < n; i += kOrderThreads) {
+    const uint32_t lv = order_level(cost[i], scale);
+    s_lv[i] = (uint8_t)lv;
+    atomicAdd(&s_cursor[lv], 1u);
+  }
   __syncthreads();
-  if (t == 0) {  // exclusive prefix over the levels
-    uint32_t run = 0u;
-    for (int l = 0; l < kOrderLevels; ++l) {
-      const uint32_t c = s_cursor[l];
-      s_cursor[l] = run;
-      run += c;
+  if (w == 0) {  // exclusive prefix over the levels, one lane per level
+    const uint32_t c = lane < kOrderLevels ? s_cursor[lane] : 0u;
+    uint32_t inc = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = (uint32_t)__shfl_up((int)inc, d);
+      if (lane >= d) inc += o;
     }
+    if (lane < kOrderLevels) s_cursor[lane] = inc - c;
   }
   __syncthreads();
   // Stable placement, kOrderThreads tiles per round: rank among the lanes of
-  // the wave with the same level (ballot match on the 4 level bits), then the
+  // the wave with the same level (ballot match on the level bits), then the
   // counts of the waves before it, then the level's cursor.
   for (int c0 = 0; c0 < n; c0 += kOrderThreads) {
     const int i = c0 + t;
     const bool valid = i < n;
-    const uint32_t lv = valid ? (uint32_t)level_of(cost[i]) : (uint32_t)kOrderLevels;  // kOrderLevels: no level
+    const uint32_t lv = valid ? (uint32_t)s_lv[i] : (uint32_t)kOrderLevels;  // kOrderLevels: no level
     uint64_t peers = ~0ull;
 #pragma unroll
     for (int bit = 0; bit < kLevelBits; ++bit) {
@@ -2035,17 +2044,17 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
     __syncthreads();
     if (valid && below == 0) s_wc[w][lv] = (uint32_t)__popcll(peers);
     __syncthreads();
-    if (valid) {
-      uint32_t pos = s_cursor[lv] + below;
-      for (int v = 0; v < w; ++v) pos += s_wc[v][lv];
-      order[pos] = i;
+    if (t < kOrderLevels) {  // per level: the waves' counts -> their start positions
+      uint32_t run = s_cursor[t];
+      for (int v = 0; v < NW; ++v) {
+        const uint32_t c = s_wc[v][t];
+        s_wc[v][t] = run;
+        run += c;
+      }
+      s_cursor[t] = run;
     }
     __syncthreads();
-    if (t < kOrderLevels) {
-      uint32_t add = 0u;
-      for (int v = 0; v < NW; ++v) add += s_wc[v][t];
-      s_cursor[t] += add;
-    }
+    if (valid) order[s_wc[w][lv] + below] = i;
     __syncthreads();
   }
 }
@@ -2075,8 +2084,8 @@ __global__ __launch_bounds__(256) void k_order_count(const uint32_t* __restrict_
   __shared__ uint32_t s_c[kOrderLevels];
   if (threadIdx.x < kOrderLevels) s_c[threadIdx.x] = 0u;
   __syncthreads();
-  const uint64_t mx = max(cw[0], 1u);
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) atomicAdd(&s_c[order_level(cost[i], mx)], 1u);
+  const float scale = order_scale(cw[0]);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) atomicAdd(&s_c[order_level(cost[i], scale)], 1u);
   __syncthreads();
   if (threadIdx.x < kOrderLevels && s_c[threadIdx.x]) atomicAdd(&cw[1 + threadIdx.x], s_c[threadIdx.x]);
 }
@@ -2097,10 +2106,10 @@ __global__ __launch_bounds__(kOrderThreads) void k_order_place(const uint32_t* _
   __shared__ uint32_t s_wc[NW][kOrderLevels];
   __shared__ uint32_t s_base[kOrderLevels];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const uint64_t mx = max(cw[0], 1u);
+  const float scale = order_scale(cw[0]);
   const int i = blockIdx.x * kOrderThreads + t;
   const bool valid = i < n;
-  const uint32_t lv = valid ? order_level(cost[i], mx) : (uint32_t)kOrderLevels;  // kOrderLevels: no level
+  const uint32_t lv = valid ? order_level(cost[i], scale) : (uint32_t)kOrderLevels;  // kOrderLevels: no level
   uint64_t peers = ~0ull;
 #pragma unroll
   for (int bit = 0; bit < kLevelBits; ++bit) {
