@@ -129,6 +129,7 @@ SIGNATURES = {
     "gsmi_exp_spec": (_I32, [_I64, _P, _P, _P]),
     "gsmi_exp_spec_nonpos": (_I32, [_I64, _P, _P, _P]),
     "gsmi_div_staged": (_I32, [_I64, _P, _P, _P, _P, _P]),
+    "gsmi_tile_order_from_costs": (_I32, [_I32, _P, _P, _P, _SZ, _P]),
     "gsmi_sched_arena_run": (_I32, [_I64, _I32, _P, _P]),
     "gsmi_sched_select_batch": (_I32, [_I32, _P, _P, _I32, _P, _I32, _I64, _P, _P, _P]),
     "gsmi_sched_admit": (_I32, [_I32, _P, _P, _P, ctypes.c_uint64, _P, _P, _P]),

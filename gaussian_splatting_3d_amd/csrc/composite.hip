@@ -2142,14 +2142,9 @@ static size_t order_ws_bytes(int n_tiles) {
   return 2 * align_up((size_t)(n_tiles > 0 ? n_tiles : 0) * 4) + align_up(kOrderWords * 4);
 }
 
-// Fills a.tile_order from the saved n_done (workspace: costs, then order).
-static int make_tile_order(RasterArgs& a, void* ws, hipStream_t st) {
-  const int n = a.n_tiles_total;
-  uint32_t* cost = reinterpret_cast<uint32_t*>(ws);
-  int32_t* order = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ws) + align_up((size_t)n * 4));
-  uint32_t* cw = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + 2 * align_up((size_t)n * 4));
-  a.tile_order = nullptr;
-  hipLaunchKernelGGL(k_tile_cost, dim3((unsigned)((4 * (int64_t)n + 255) / 256)), dim3(256), 0, st, a, cost);
+// Order of the tiles by cost[] (most expensive level first, stable within a
+// level); cw: the counter words of grids beyond kOrderSingleMax tiles.
+static void place_tile_order(const uint32_t* cost, int n, int32_t* order, uint32_t* cw, hipStream_t st) {
   if (n <= kOrderSingleMax) {
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, st, cost, n, order);
   } else {
@@ -2161,6 +2156,17 @@ static int make_tile_order(RasterArgs& a, void* ws, hipStream_t st) {
     hipLaunchKernelGGL(k_order_place, dim3((n + kOrderThreads - 1) / kOrderThreads), dim3(kOrderThreads), 0, st,
                        cost, n, cw, order);
   }
+}
+
+// Fills a.tile_order from the saved n_done (workspace: costs, then order).
+static int make_tile_order(RasterArgs& a, void* ws, hipStream_t st) {
+  const int n = a.n_tiles_total;
+  uint32_t* cost = reinterpret_cast<uint32_t*>(ws);
+  int32_t* order = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ws) + align_up((size_t)n * 4));
+  uint32_t* cw = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + 2 * align_up((size_t)n * 4));
+  a.tile_order = nullptr;
+  hipLaunchKernelGGL(k_tile_cost, dim3((unsigned)((4 * (int64_t)n + 255) / 256)), dim3(256), 0, st, a, cost);
+  place_tile_order(cost, n, order, cw, st);
   GSMI_CHECK_LAUNCH();
   a.tile_order = order;
   return GSMI_OK;
@@ -3040,6 +3046,20 @@ __global__ void k_div_staged(int64_t n, const float* __restrict__ q, const float
   const float y = staged_recip(di);
   out[i] = radial_staged(qi, di, y, qi * y);
   if (gauss) gauss[i] = gauss_staged(qi, di, y, qi * y);
+}
+
+// The backward's launch order for given tile costs (test hook): order[0..n)
+// as the backward computes it from its tile costs; workspace as for the
+// backward's order.
+extern "C" int gsmi_tile_order_from_costs(int n, const uint32_t* cost, int32_t* order, void* workspace,
+                                          size_t workspace_bytes, void* stream) {
+  if (n < 0 || (n > 0 && (!cost || !order || !workspace))) return GSMI_ERR_ARG;
+  if (n == 0) return GSMI_OK;
+  if (workspace_bytes < order_ws_bytes(n)) return GSMI_ERR_WORKSPACE;
+  uint32_t* cw = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(workspace) + 2 * align_up((size_t)n * 4));
+  place_tile_order(cost, n, order, cw, (hipStream_t)stream);
+  GSMI_CHECK_LAUNCH();
+  return GSMI_OK;
 }
 
 extern "C" int gsmi_div_staged(int64_t n, const float* q, const float* det, float* out, float* gauss,

@@ -697,6 +697,14 @@ int gsmi_exp_spec_nonpos(int64_t n, const float *x, float *y, void *stream);
  * reference.  Used by the tests to check both bit for bit. */
 int gsmi_div_staged(int64_t n, const float *q, const float *det, float *out, float *gauss, void *stream);
 
+/* The backward's launch order for given tile costs cost[0..n): the tiles
+ * most expensive first by cost level, the order the backward derives from
+ * its tile costs (workspace of gsmi_backward_order_workspace_size(n)
+ * bytes); no counterpart in the reference.  Used by the tests to check the
+ * order is a permutation sorted by level, stable within a level. */
+int gsmi_tile_order_from_costs(int n, const uint32_t *cost, int32_t *order, void *workspace, size_t workspace_bytes,
+                               void *stream);
+
 /* The render server's scheduling policy on the host alone (no device use);
  * no counterpart in the reference.  gsmi_sched_arena_run: an arena of `size`
  * bytes driven by n_ops operations (op, x, y) — op 0 allocates x bytes (out:

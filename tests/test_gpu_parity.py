@@ -87,6 +87,47 @@ def test_exp_spec_nonpos_bit_exact(orc):
     assert same.all(), (x[~same][:5], got[~same][:5], want[~same][:5])
 
 
+LEVELS = 64  # cost levels of the backward's launch order
+
+
+def order_levels(cost):
+    """Level of each tile as the launch order defines it (0 = most expensive):
+    63 - min(int(float32(c) * (63 / float32(max cost))), 63) in float32."""
+    c = np.asarray(cost, np.uint32)
+    mx = np.float32(max(int(c.max()), 1))
+    scale = np.float32(LEVELS - 1) / mx
+    q = (c.astype(np.float32) * scale).astype(np.float32).astype(np.int64)
+    return (LEVELS - 1) - np.minimum(q, LEVELS - 1)
+
+
+@pytest.mark.parametrize("n", [1, 7, 1000, 8160, 32768, 32769, 259200])
+def test_tile_order_is_sorted_permutation(n):
+    """The backward's launch order: a permutation of the tiles whose cost
+    levels never increase from most expensive to least, with ties (equal
+    levels) in index order: over the whole grid when one workgroup orders
+    it (up to 32,768 tiles), within each workgroup's 1024 tiles beyond."""
+    rng = np.random.default_rng(n)
+    cost = (rng.pareto(1.5, n) * 200).astype(np.uint32) + 32
+    cost[rng.random(n) < 0.1] = 0  # empty tiles
+    if n > 4:
+        cost[:3] = [2 ** 32 - 1, 2 ** 32 - 2, 2 ** 31]  # the scale at its largest
+    ct = torch.from_numpy(cost.view(np.int32)).to(DEV)
+    order = torch.empty(n, dtype=torch.int32, device=DEV)
+    L = _lib.load()
+    ws = torch.empty(int(L.gsmi_backward_order_workspace_size(n)), dtype=torch.uint8, device=DEV)
+    _lib.check(L.gsmi_tile_order_from_costs(n, _lib.ptr(ct), _lib.ptr(order), _lib.ptr(ws), ws.numel(),
+                                            _lib.stream_of()), "tile order")
+    o = order.cpu().numpy().astype(np.int64)
+    assert np.array_equal(np.sort(o), np.arange(n))
+    lv = order_levels(cost)[o]
+    assert (np.diff(lv) >= 0).all()
+    group = o // 1024 if n > 32768 else np.zeros(n, np.int64)
+    for g in np.unique(group):
+        for level in np.unique(lv[group == g]):
+            sel = o[(group == g) & (lv == level)]
+            assert (np.diff(sel) > 0).all(), (g, level)
+
+
 def test_div_staged_matches_ieee_division(orc):
     """The forward's q / det from the staged reciprocal (two FMA residual
     steps, IEEE division where they give NaN or |q| < 2^-100) equals numpy's
